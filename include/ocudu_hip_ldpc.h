@@ -1,0 +1,131 @@
+/*
+ * ocudu_hip_ldpc -- C ABI of the MI355X (gfx950) PUSCH LDPC receive chain.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes (0 = success,
+ * negative = error, message in lh_last_error()).  No exceptions cross this boundary.
+ * Device pointers ("d_*") are HIP device allocations; "stream" is a hipStream_t (NULL =
+ * the null stream).  A context is single-threaded: one caller at a time, like the
+ * reference's decoder instances (pusch_codeblock_decoder_cuda_batch.h decode_mutex_).
+ *
+ * Every entry point replaces one reference interface (paths relative to the reference
+ * repository root); the C++ adapters in ocudu-dapp-platform_amd/host implement the
+ * reference's class interfaces on top of these functions.
+ */
+#ifndef OCUDU_HIP_LDPC_H
+#define OCUDU_HIP_LDPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LH_OK 0
+#define LH_ERR_INVALID (-1)  /* invalid argument (a contract break in the reference) */
+#define LH_ERR_HIP (-2)      /* HIP runtime error */
+#define LH_ERR_CAPACITY (-3) /* batch exceeds the context capacity */
+
+/* CRC generator polynomials, include/ocudu/phy/upper/channel_coding/crc_calculator.h:17-24. */
+#define LH_CRC24A 0x1864cfbu
+#define LH_CRC24B 0x1800063u
+#define LH_CRC16 0x11021u
+
+/* Decoder flags: ldpc_decoder_factory_configuration{force_decoding, early_stop_syndrome}
+ * (include/ocudu/phy/upper/channel_coding/channel_coding_factories.h:35-54). */
+#define LH_FORCE_DECODING 1u
+#define LH_EARLY_STOP_SYNDROME 2u
+
+typedef struct lh_ctx lh_ctx;
+
+/* Context: device tables, staging for descriptors, capacity.  Replaces the device-side
+ * state of pusch_codeblock_decoder_cuda_batch (constructed from pusch_decoder_capacity,
+ * lib/phy/upper/channel_coding/ldpc/cuda/pusch_decoder_capacity.h:14-45).
+ * max_codeblocks bounds one batch call. */
+int lh_ctx_create(lh_ctx** ctx, int device, unsigned max_codeblocks);
+int lh_ctx_destroy(lh_ctx* ctx);
+/* Message of the last error on the calling thread (never NULL). */
+const char* lh_last_error(void);
+
+/* ldpc_decoder::configuration, include/ocudu/phy/upper/channel_coding/ldpc/ldpc_decoder.h:23-34. */
+typedef struct lh_decoder_config {
+  uint32_t base_graph;      /* 1 = BG1, 2 = BG2 */
+  uint32_t lifting_size;    /* one of the 51 NR lifting sizes */
+  uint32_t nof_filler_bits; /* F */
+  uint32_t nof_crc_bits;    /* 16 or 24 */
+  uint32_t max_iterations;  /* 1..255 */
+} lh_decoder_config;
+
+/* One ldpc_decoder::decode() call of a batch. */
+typedef struct lh_decode_desc {
+  uint64_t llr_offset;  /* byte offset of this codeblock's LLRs in d_llrs */
+  uint64_t bits_offset; /* byte offset of its packed output (K_b*Z bits, MSB first) */
+  uint32_t nof_llrs;    /* input length: (K_b+2)*Z .. N_short*Z */
+  uint32_t crc_poly;    /* LH_CRC24A / LH_CRC24B / LH_CRC16, or 0 = no CRC calculator */
+  lh_decoder_config cfg;
+} lh_decode_desc;
+
+/* Batched ldpc_decoder::decode(output, input, crc, cfg)
+ * (include/ocudu/phy/upper/channel_coding/ldpc/ldpc_decoder.h:52-57, behaviour of
+ * lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145 with the generic
+ * arithmetic of ldpc_decoder_generic.cpp).  d_results[i] = number of iterations, or 0
+ * for std::nullopt.  Bit-exact to the reference generic decoder.  flags: LH_FORCE_*.
+ * descs is host memory; d_llrs, d_bits, d_results are device memory. */
+int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const int8_t* d_llrs, uint8_t* d_bits,
+                   int32_t* d_results, uint32_t flags, void* stream);
+
+/* codeblock_metadata of one codeblock (include/ocudu/phy/upper/codeblock_metadata.h:24-58)
+ * plus its buffers. */
+typedef struct lh_rm_desc {
+  uint64_t cw_offset;   /* cb_specific.cw_offset: its E LLRs in the codeword buffer */
+  uint64_t rm_offset;   /* offset of its N_short*Z soft-bit (HARQ) buffer */
+  uint32_t rm_length;   /* cb_specific.rm_length (E), multiple of modulation_order */
+  uint32_t base_graph;  /* tb_common.base_graph: 1 / 2 */
+  uint32_t lifting_size;
+  uint32_t rv;          /* 0..3 */
+  uint32_t modulation_order; /* bits per symbol: 1, 2, 4, 6, 8 */
+  uint32_t Nref;        /* 0 = no limited buffer */
+  uint32_t nof_filler_bits;
+} lh_rm_desc;
+
+/* Batched ldpc_rate_dematcher::rate_dematch(output, input, new_data, cfg)
+ * (include/ocudu/phy/upper/channel_coding/ldpc/ldpc_rate_dematcher.h:36-39, behaviour of
+ * lib/phy/upper/channel_coding/ldpc/ldpc_rate_dematcher_impl.cpp:29-250): deinterleave,
+ * place from k0 skipping fillers, fillers := +127, copy or saturated HARQ combine.
+ * d_rm is read (when !new_data) and written in place. */
+int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                    int new_data, void* stream);
+
+/* One codeblock of a PUSCH batch: described_rx_codeblock = (LLR view, codeblock_metadata)
+ * (codeblock_metadata.h:144) plus its HARQ buffer and output placement. */
+typedef struct lh_pusch_cb_desc {
+  lh_rm_desc rm;        /* dematching parameters and buffers */
+  uint64_t bits_offset; /* packed message bits of this codeblock */
+  uint32_t nof_crc_bits;/* cb_specific.nof_crc_bits */
+  uint32_t crc_poly;    /* CB CRC: 24B when C > 1, else the TB CRC (24A or 16) */
+} lh_pusch_cb_desc;
+
+/* pusch_resident_codeblock_decoder::decode_batch(codeblocks, cb_data, rm_buffers,
+ * new_data, crc_poly, use_early_stop, nof_ldpc_iterations)
+ * (include/ocudu/phy/upper/channel_processors/pusch/pusch_resident_codeblock_decoder.h:62-149)
+ * with the per-codeblock semantics of pusch_codeblock_decoder::decode
+ * (lib/phy/upper/channel_processors/pusch/pusch_codeblock_decoder.cpp:17-58): rate
+ * dematch into the HARQ buffer d_rm (combined soft bits are written back), then decode;
+ * with use_early_stop the CRC stops the iterations, otherwise the syndrome gates a final
+ * CRC.  d_results[i] = iterations or 0 (CRC failed).  Mixed BG/Z/E/rate in one batch. */
+int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                          uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                          unsigned nof_ldpc_iterations, void* stream);
+
+/* Device-side CRC of packed bit buffers: crc_calculator::calculate(bit_buffer)
+ * (include/ocudu/phy/upper/channel_coding/crc_calculator.h:59-81, behaviour of
+ * lib/phy/upper/channel_coding/crc_calculator_generic_impl.cpp:69-91).
+ * Buffer i is d_bits + offsets[i] with nbits[i] bits; d_out[i] = remainder. */
+int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,
+                     const uint8_t* d_bits, uint32_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,251 @@
+"""Python access to the HIP PUSCH LDPC receive chain (C ABI of include/ocudu_hip_ldpc.h).
+
+The library is the product; this module is a thin ctypes binding used by tests and
+bench.py.  Device memory, streams and events come straight from the HIP runtime
+(libamdhip64) -- no torch.  Loading fails loudly when the built library is missing.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "lib" / "libocudu_hip_ldpc.so"
+
+LH_CRC24A, LH_CRC24B, LH_CRC16 = 0x1864CFB, 0x1800063, 0x11021
+LH_FORCE_DECODING, LH_EARLY_STOP_SYNDROME = 1, 2
+
+_hip = None
+_lib = None
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class LhError(RuntimeError):
+    pass
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        h = C.CDLL(os.environ.get("LH_HIP_RUNTIME", "libamdhip64.so"))
+        h.hipGetErrorString.restype = C.c_char_p
+        h.hipGetErrorString.argtypes = [C.c_int]
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        h.hipDeviceSynchronize.argtypes = []
+        h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        h.hipStreamDestroy.argtypes = [C.c_void_p]
+        h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        h.hipEventDestroy.argtypes = [C.c_void_p]
+        h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        h.hipEventSynchronize.argtypes = [C.c_void_p]
+        h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        h.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        h.hipSetDevice.argtypes = [C.c_int]
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostFree.argtypes = [C.c_void_p]
+        _hip = h
+    return _hip
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise HipError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+def device_count():
+    n = C.c_int(0)
+    try:
+        rc = hip().hipGetDeviceCount(C.byref(n))
+    except OSError:
+        return 0
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """A hipMalloc'd byte buffer."""
+
+    def __init__(self, nbytes):
+        self.nbytes = max(int(nbytes), 1)
+        p = C.c_void_p()
+        _chk(hip().hipMalloc(C.byref(p), self.nbytes), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        _chk(hip().hipMemcpy(self.ptr + offset, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
+    def download(self, dtype, count, offset=0):
+        out = np.empty(count, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        _chk(hip().hipMemcpy(out.ctypes.data, self.ptr + offset, out.nbytes, D2H), "hipMemcpy D2H")
+        return out
+
+    def zero(self):
+        _chk(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        _chk(hip().hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        self.handle = s.value
+
+    def sync(self):
+        _chk(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _chk(hip().hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.handle = e.value
+
+    def record(self, stream=None):
+        _chk(hip().hipEventRecord(self.handle, stream.handle if stream else None), "hipEventRecord")
+
+    def sync(self):
+        _chk(hip().hipEventSynchronize(self.handle), "hipEventSynchronize")
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        _chk(hip().hipEventElapsedTime(C.byref(ms), self.handle, later.handle), "hipEventElapsedTime")
+        return ms.value
+
+
+# ---- C ABI structures ------------------------------------------------------------
+class DecoderConfig(C.Structure):
+    _fields_ = [("base_graph", C.c_uint32), ("lifting_size", C.c_uint32), ("nof_filler_bits", C.c_uint32),
+                ("nof_crc_bits", C.c_uint32), ("max_iterations", C.c_uint32)]
+
+
+class DecodeDesc(C.Structure):
+    _fields_ = [("llr_offset", C.c_uint64), ("bits_offset", C.c_uint64), ("nof_llrs", C.c_uint32),
+                ("crc_poly", C.c_uint32), ("cfg", DecoderConfig)]
+
+
+class RmDesc(C.Structure):
+    _fields_ = [("cw_offset", C.c_uint64), ("rm_offset", C.c_uint64), ("rm_length", C.c_uint32),
+                ("base_graph", C.c_uint32), ("lifting_size", C.c_uint32), ("rv", C.c_uint32),
+                ("modulation_order", C.c_uint32), ("Nref", C.c_uint32), ("nof_filler_bits", C.c_uint32)]
+
+
+class PuschCbDesc(C.Structure):
+    _fields_ = [("rm", RmDesc), ("bits_offset", C.c_uint64), ("nof_crc_bits", C.c_uint32),
+                ("crc_poly", C.c_uint32)]
+
+
+EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
+            "lh_pusch_decode_batch", "lh_crc_calculate"]
+
+
+def lib():
+    """Loads the built HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LhError(f"{LIB_PATH} is missing: build it with ocudu-dapp-platform_amd/build.py")
+        L = C.CDLL(str(LIB_PATH))
+        L.lh_last_error.restype = C.c_char_p
+        L.lh_ctx_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_uint]
+        L.lh_ctx_destroy.argtypes = [C.c_void_p]
+        L.lh_ldpc_decode.argtypes = [C.c_void_p, C.POINTER(DecodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_uint32, C.c_void_p]
+        L.lh_rate_dematch.argtypes = [C.c_void_p, C.POINTER(RmDesc), C.c_uint, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_void_p]
+        L.lh_pusch_decode_batch.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_void_p]
+        L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                       C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _lh(rc, what):
+    if rc != 0:
+        raise LhError(f"{what} failed ({rc}): {lib().lh_last_error().decode()}")
+
+
+class Context:
+    def __init__(self, device=0, max_codeblocks=4096):
+        h = C.c_void_p()
+        _lh(lib().lh_ctx_create(C.byref(h), device, max_codeblocks), "lh_ctx_create")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            lib().lh_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ldpc_decode(self, descs, d_llrs, d_bits, d_results, flags=0, stream=None):
+        arr = (DecodeDesc * len(descs))(*descs)
+        _lh(lib().lh_ldpc_decode(self.handle, arr, len(descs), d_llrs.ptr, d_bits.ptr, d_results.ptr, flags,
+                                 stream.handle if stream else None), "lh_ldpc_decode")
+
+    def rate_dematch(self, descs, d_cw, d_rm, new_data=True, stream=None):
+        arr = (RmDesc * len(descs))(*descs)
+        _lh(lib().lh_rate_dematch(self.handle, arr, len(descs), d_cw.ptr, d_rm.ptr, int(new_data),
+                                  stream.handle if stream else None), "lh_rate_dematch")
+
+    def pusch_decode_batch(self, cbs, d_cw, d_rm, d_bits, d_results, new_data=True, use_early_stop=True,
+                           nof_ldpc_iterations=6, stream=None, arr=None):
+        if arr is None:
+            arr = (PuschCbDesc * len(cbs))(*cbs)
+        _lh(lib().lh_pusch_decode_batch(self.handle, arr, len(arr), d_cw.ptr, d_rm.ptr, d_bits.ptr, d_results.ptr,
+                                        int(new_data), int(use_early_stop), nof_ldpc_iterations,
+                                        stream.handle if stream else None), "lh_pusch_decode_batch")
+
+    def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
+        n = len(offsets)
+        offs = (C.c_uint64 * n)(*offsets)
+        nb = (C.c_uint32 * n)(*nbits)
+        _lh(lib().lh_crc_calculate(self.handle, poly, offs, nb, n, d_bits.ptr, d_out.ptr,
+                                   stream.handle if stream else None), "lh_crc_calculate")
+
+
+def k_b(bg):
+    return 22 if bg == 1 else 10
+
+
+def n_short(bg):
+    return 66 if bg == 1 else 50
+
+
+def msg_bytes(bg, Z):
+    return (k_b(bg) * Z + 7) // 8

@@ -1,0 +1,255 @@
+"""Vectorised synthetic-codeblock generator (numpy): CRC attach, LDPC encoding, rate
+matching, QAM mapping, AWGN and max-log soft demapping to int8 LLRs.
+
+Follows the reference's transmit/receive conventions so that its decoder sees the LLRs
+it would see in the PUSCH chain:
+  * CRC: MSB-first long division, zero init (crc_calculator_generic_impl.cpp:17-91);
+  * encoder: systematic NR LDPC (TS38.212 5.3.2), output shortened by 2Z
+    (ldpc_encoder_impl.cpp:21-55);
+  * rate matcher: k0 = floor(k0_factor[rv] * Ncb / N) * Z, fillers skipped, bit
+    interleaving by Qm (ldpc_rate_matcher_impl.cpp);
+  * mapper: TS38.211 5.1 Gray QAM (modulation_mapper_lut_impl.cpp);
+  * demapper: max-log, quantised with range 24 (QPSK) / 20 (QAM) to +-120
+    (demodulation_mapper_qpsk.cpp, _qam16/64/256.cpp, log_likelihood_ratio.cpp:72-81).
+"""
+import re
+from functools import lru_cache
+from pathlib import Path
+
+import numpy as np
+
+TABLES = Path(__file__).resolve().parent.parent / "csrc" / "bg_tables.inc"
+LIFTING_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40,
+                 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240,
+                 256, 288, 320, 352, 384]
+CRC24A, CRC24B, CRC16 = 0x1864CFB, 0x1800063, 0x11021
+
+
+@lru_cache(maxsize=None)
+def base_graph(bg):
+    """(row_start, cols, V[8][nnz]) of TS38.212 Table 5.3.2-2/-3."""
+    txt = TABLES.read_text()
+
+    def arr(name):
+        m = re.search(name + r"\[[^\]]*\](?:\[[^\]]*\])? = \{(.*?)\};", txt, re.S)
+        return [int(x) for x in re.findall(r"\d+", m.group(1))]
+
+    rs = np.array(arr(f"lh_bg{bg}_row_start"), np.int64)
+    cols = np.array(arr(f"lh_bg{bg}_col"), np.int64)
+    v = np.array(arr(f"lh_bg{bg}_v"), np.int64).reshape(8, -1)
+    return rs, cols, v
+
+
+def lifting_index(Z):
+    for i, a in enumerate((2, 3, 5, 7, 9, 11, 13, 15)):
+        v = a
+        while v <= 384:
+            if v == Z:
+                return i
+            v *= 2
+    raise ValueError(f"invalid lifting size {Z}")
+
+
+def k_b(bg):
+    return 22 if bg == 1 else 10
+
+
+def m_rows(bg):
+    return 46 if bg == 1 else 42
+
+
+def n_short(bg):
+    return 66 if bg == 1 else 50
+
+
+def crc_bits_table(poly, nbits):
+    """Per-bit remainders x^(nbits-1-i+order) mod G, i = 0..nbits-1 (CRC is GF(2)-linear)."""
+    order = poly.bit_length() - 1
+    out = np.zeros(nbits, np.int64)
+    v = 1
+    pw = []
+    for _ in range(nbits + order):
+        pw.append(v)
+        v <<= 1
+        if v >> order & 1:
+            v ^= poly
+    for i in range(nbits):
+        out[i] = pw[nbits - 1 - i + order]
+    return out
+
+
+def crc(bits, poly):
+    """CRC remainders of a batch of unpacked bit rows (n, L)."""
+    bits = np.asarray(bits, np.uint8)
+    tab = crc_bits_table(poly, bits.shape[-1])
+    r = np.zeros(bits.shape[:-1], np.int64)
+    for i in range(bits.shape[-1]):
+        r ^= np.where(bits[..., i] != 0, tab[i], 0)
+    return r
+
+
+def attach_crc(data, poly):
+    order = poly.bit_length() - 1
+    r = crc(data, poly)
+    crc_b = ((r[:, None] >> np.arange(order - 1, -1, -1)[None, :]) & 1).astype(np.uint8)
+    return np.concatenate([data, crc_b], axis=1)
+
+
+def _edges(bg, Z):
+    rs, cols, v = base_graph(bg)
+    sh = v[lifting_index(Z)] % Z
+    return rs, cols, sh
+
+
+def encode(msg, bg, Z):
+    """msg (n, K_b*Z) bits (fillers zero) -> full codeword (n, (K_b+M)*Z), unshortened.
+    Lifted check (i, r): XOR_j c_j[(r + shift_ij) mod Z] = 0."""
+    msg = np.asarray(msg, np.uint8)
+    n = msg.shape[0]
+    K, M = k_b(bg), m_rows(bg)
+    rs, cols, sh = _edges(bg, Z)
+    cw = np.zeros((n, K + M, Z), np.uint8)
+    cw[:, :K] = msg.reshape(n, K, Z)
+    r_idx = np.arange(Z)
+
+    def term(col, s):
+        return cw[:, col][:, (r_idx + s) % Z]
+
+    lam = np.zeros((4, n, Z), np.uint8)
+    core = {}
+    for i in range(4):
+        for e in range(rs[i], rs[i + 1]):
+            if cols[e] < K:
+                lam[i] ^= term(cols[e], sh[e])
+            else:
+                core[(i, cols[e] - K)] = sh[e]
+    # sum of the four core rows leaves one circulant on the first core column
+    col0 = [core[(i, 0)] for i in range(4) if (i, 0) in core]
+    odd = [s for s in set(col0) if col0.count(s) % 2 == 1]
+    assert len(odd) == 1
+    t = odd[0]
+    tot = lam[0] ^ lam[1] ^ lam[2] ^ lam[3]
+    cw[:, K] = tot[:, (r_idx - t) % Z]
+    known = {0}
+    while len(known) < 4:
+        progress = False
+        for i in range(4):
+            unk = [c for c in range(4) if (i, c) in core and c not in known]
+            if len(unk) != 1:
+                continue
+            acc = lam[i].copy()
+            for c in range(4):
+                if (i, c) in core and c in known:
+                    acc ^= term(K + c, core[(i, c)])
+            u = unk[0]
+            s = core[(i, u)]
+            cw[:, K + u][:, (r_idx + s) % Z] = acc
+            known.add(u)
+            progress = True
+        assert progress
+    for i in range(4, M):
+        acc = np.zeros((n, Z), np.uint8)
+        own = None
+        for e in range(rs[i], rs[i + 1]):
+            if cols[e] == K + i:
+                own = sh[e]
+            else:
+                acc ^= term(cols[e], sh[e])
+        cw[:, K + i][:, (r_idx + own) % Z] = acc
+    return cw.reshape(n, -1)
+
+
+def rate_match_indices(bg, Z, E, rv=0, Qm=2, Nref=0, F=0):
+    """Indices into the shortened codeword d (N_short*Z) of the E transmitted bits,
+    after bit interleaving."""
+    N = n_short(bg) * Z
+    Ncb = min(Nref, N) if Nref > 0 else N
+    f = (0, 17, 33, 56) if bg == 1 else (0, 13, 25, 43)
+    k0 = int(np.floor((f[rv] * Ncb) / N)) * Z
+    nsys = (k_b(bg) - 2) * Z
+    pos = np.array([p for p in range(Ncb) if not (nsys - F <= p < nsys)], np.int64)
+    start = np.searchsorted(pos, k0)
+    sel = pos[(start + np.arange(E)) % pos.size]
+    Kq = E // Qm
+    # f[i*Qm + q] = e[q*Kq + i]
+    inter = sel.reshape(Qm, Kq).T.reshape(-1)
+    return inter
+
+
+def _pam_levels(half):
+    """Amplitude (unnormalised) of each bit pattern of one component, TS38.211 5.1."""
+    npts = 1 << half
+    lev = np.zeros(npts)
+    bits = np.zeros((npts, half), np.uint8)
+    for p in range(npts):
+        b = [(p >> k) & 1 for k in range(half)]
+        bits[p] = b
+        v = 1
+        for k in range(half - 1, 0, -1):
+            v = (1 << (half - k)) - (1 - 2 * b[k]) * v
+        lev[p] = (1 - 2 * b[0]) * v
+    return lev, bits
+
+
+_NORM = {1: 1 / np.sqrt(2), 2: 1 / np.sqrt(2), 4: 1 / np.sqrt(10), 6: 1 / np.sqrt(42), 8: 1 / np.sqrt(170)}
+
+
+def modulate(bits, Qm):
+    """bits (n, E) -> complex symbols as (n, E/Qm, 2) float32 (I, Q)."""
+    n, E = bits.shape
+    half = Qm // 2
+    lev, _ = _pam_levels(half)
+    b = bits.reshape(n, E // Qm, half, 2)  # [.., k, comp] = bit 2k+comp
+    idx_i = (b[..., 0] * (1 << np.arange(half))).sum(-1)
+    idx_q = (b[..., 1] * (1 << np.arange(half))).sum(-1)
+    return (np.stack([lev[idx_i], lev[idx_q]], -1) * _NORM[Qm]).astype(np.float32)
+
+
+def quantize(l, range_limit):
+    """log_likelihood_ratio::quantize: round(clip(l, +-r) / r * 120), half away from zero."""
+    c = np.clip(l, -range_limit, range_limit) / range_limit * np.float32(120)
+    return (np.sign(c) * np.floor(np.abs(c) + 0.5)).astype(np.int8)
+
+
+def demap(iq, Qm, n0):
+    """Max-log LLRs per bit: (min_{b=1} d^2 - min_{b=0} d^2) / n0, then quantised."""
+    n, nsym, _ = iq.shape
+    half = Qm // 2
+    lev, lb = _pam_levels(half)
+    lev = (lev * _NORM[Qm]).astype(np.float64)
+    out = np.zeros((n, nsym, half, 2), np.int8)
+    rng = 24.0 if Qm == 2 else 20.0
+    for comp in range(2):
+        x = iq[..., comp].astype(np.float64)[..., None]
+        d2 = (x - lev[None, None, :]) ** 2
+        for k in range(half):
+            d1 = np.where(lb[:, k] == 1, d2, np.inf).min(-1)
+            d0 = np.where(lb[:, k] == 0, d2, np.inf).min(-1)
+            out[..., k, comp] = quantize(((d1 - d0) / n0).astype(np.float32), np.float32(rng))
+    return out.reshape(n, nsym * Qm)
+
+
+def make_batch(n_cb, bg, Z, E, Qm, esn0_db, seed, rv=0, F=0, crc_poly=CRC24B, n_unique=None):
+    """Seeded codeblocks: random data + CB CRC (+F zero fillers), encoded, rate matched,
+    mapped, AWGN at Es/N0 (Es = 1), demapped.  Returns (message bits (n, K_b*Z) uint8,
+    LLRs (n, E) int8).  n_unique bounds the distinct messages (noise stays independent)."""
+    rng = np.random.default_rng(seed)
+    K = k_b(bg) * Z
+    order = crc_poly.bit_length() - 1
+    k_data = K - F - order
+    nu = n_cb if n_unique is None else min(n_unique, n_cb)
+    data = rng.integers(0, 2, (nu, k_data), dtype=np.uint8)
+    msg = np.zeros((nu, K), np.uint8)
+    msg[:, :K - F] = attach_crc(data, crc_poly)
+    cw = encode(msg, bg, Z)
+    d = cw[:, 2 * Z:]
+    tx = d[:, rate_match_indices(bg, Z, E, rv, Qm, 0, F)]
+    sel = np.arange(n_cb) % nu
+    n0 = 10.0 ** (-esn0_db / 10.0)
+    llr = np.empty((n_cb, E), np.int8)
+    for c0 in range(0, n_cb, 64):
+        c1 = min(n_cb, c0 + 64)
+        iq = modulate(tx[sel[c0:c1]], Qm)
+        iq = iq + rng.normal(0.0, np.sqrt(n0 / 2), iq.shape).astype(np.float32)
+        llr[c0:c1] = demap(iq, Qm, n0)
+    return msg[sel], llr

@@ -1,0 +1,242 @@
+"""ctypes access to the CPU oracle (liboracle.so) and to the reference build
+(_ref/libref_ldpc.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of bench.py as the
+checker.  The HIP product never imports this module.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+CRC24A, CRC24B, CRC16 = 0x1864CFB, 0x1800063, 0x11021
+LIFTING_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40,
+                 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240,
+                 256, 288, 320, 352, 384]
+
+_i8p = C.POINTER(C.c_int8)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class _DecCfg(C.Structure):
+    _fields_ = [("bg", C.c_int), ("Z", C.c_uint), ("nof_filler_bits", C.c_uint), ("nof_crc_bits", C.c_uint),
+                ("max_iterations", C.c_uint)]
+
+
+class _RmCfg(C.Structure):
+    _fields_ = [("rv", C.c_uint), ("Qm", C.c_uint), ("Nref", C.c_uint), ("nof_filler_bits", C.c_uint)]
+
+
+class _Seg(C.Structure):
+    _fields_ = [("nof_segments", C.c_uint), ("Z", C.c_uint), ("segment_length", C.c_uint),
+                ("nof_filler_bits", C.c_uint), ("nof_crc_bits", C.c_uint), ("nof_tb_crc_bits", C.c_uint),
+                ("full_length", C.c_uint), ("rm_length", C.c_uint * 256), ("cw_offset", C.c_uint * 256)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built (make -f oracle/Makefile)")
+        L = C.CDLL(str(path))
+        L.orc_llr_add.restype = C.c_int8
+        L.orc_llr_add.argtypes = [C.c_int8, C.c_int8]
+        L.orc_llr_promotion_sum.restype = C.c_int8
+        L.orc_llr_promotion_sum.argtypes = [C.c_int8, C.c_int8]
+        L.orc_llr_quantize.restype = C.c_int8
+        L.orc_llr_quantize.argtypes = [C.c_float, C.c_float]
+        L.orc_crc_packed.restype = C.c_uint32
+        L.orc_crc_packed.argtypes = [C.c_uint32, _u8p, C.c_uint]
+        L.orc_crc_unpacked.restype = C.c_uint32
+        L.orc_crc_unpacked.argtypes = [C.c_uint32, _u8p, C.c_uint]
+        L.orc_graph_shift.restype = C.c_int
+        L.orc_graph_shift.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint]
+        L.orc_lifting_index.argtypes = [C.c_uint]
+        L.orc_ldpc_decode.restype = C.c_uint
+        L.orc_ldpc_decode.argtypes = [C.POINTER(_DecCfg), C.c_int, C.c_int, C.c_uint32, _i8p, C.c_uint, _u8p]
+        L.orc_rate_dematch.argtypes = [_i8p, C.c_uint, _i8p, C.c_uint, C.c_int, C.POINTER(_RmCfg)]
+        L.orc_segment_rx.argtypes = [C.POINTER(_Seg), C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+        L.orc_ldpc_encode.argtypes = [C.c_int, C.c_uint, _u8p, _u8p]
+        L.orc_syndrome_weight.restype = C.c_uint
+        L.orc_syndrome_weight.argtypes = [C.c_int, C.c_uint, _u8p]
+        L.orc_rate_match.argtypes = [_u8p, C.c_uint, _u8p, C.c_uint, C.c_int, C.c_uint, C.c_uint, C.c_uint,
+                                     C.c_uint, C.c_uint]
+        L.orc_modulate_awgn_demap.argtypes = [_i8p, _u8p, C.c_uint, C.c_uint, C.c_double, C.c_uint64]
+        L.orc_tb_concat_crc.restype = C.c_uint32
+        L.orc_tb_concat_crc.argtypes = [_u8p, C.c_uint, _u8p, C.c_uint, C.POINTER(_Seg)]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own CPU code built from source, or None when not built."""
+    global _ref
+    if _ref is None:
+        path = HERE / "_ref" / "libref_ldpc.so"
+        if not path.exists():
+            return None
+        R = C.CDLL(str(path))
+        R.ref_decoder_create.restype = C.c_void_p
+        R.ref_decoder_create.argtypes = [C.c_int, C.c_int, C.c_int]
+        R.ref_decoder_destroy.argtypes = [C.c_void_p]
+        R.ref_decoder_kind.argtypes = [C.c_int]
+        R.ref_decoder_decode.restype = C.c_uint
+        R.ref_decoder_decode.argtypes = [C.c_void_p, _i8p, C.c_uint, C.c_int, C.c_uint, C.c_uint, C.c_uint,
+                                         C.c_uint, C.c_uint32, _u8p]
+        R.ref_rate_dematch.argtypes = [_i8p, C.c_uint, _i8p, C.c_uint, C.c_int, C.c_int, C.c_uint, C.c_uint,
+                                       C.c_uint, C.c_uint, C.c_uint]
+        R.ref_encode_rate_match.argtypes = [_u8p, C.c_int, C.c_uint, _u8p, _u8p, C.c_uint, C.c_uint, C.c_uint,
+                                            C.c_uint, C.c_uint]
+        R.ref_crc.restype = C.c_uint32
+        R.ref_crc.argtypes = [C.c_uint32, _u8p, C.c_uint]
+        R.ref_graph_shift.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint]
+        R.ref_llr_add.restype = C.c_int8
+        R.ref_llr_add.argtypes = [C.c_int8, C.c_int8]
+        R.ref_llr_promotion_sum.restype = C.c_int8
+        R.ref_llr_promotion_sum.argtypes = [C.c_int8, C.c_int8]
+        R.ref_llr_quantize.restype = C.c_int8
+        R.ref_llr_quantize.argtypes = [C.c_float, C.c_float]
+        R.ref_decode_throughput.restype = C.c_double
+        R.ref_decode_throughput.argtypes = [C.c_int, C.c_int, _i8p, C.c_uint, C.c_uint, C.c_int, C.c_uint,
+                                            C.c_uint, C.c_uint, C.c_uint32, C.c_double,
+                                            C.POINTER(C.c_ulong), C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
+        _ref = R
+    return _ref
+
+
+def K_b(bg):
+    return 22 if bg == 1 else 10
+
+
+def N_short(bg):
+    return 66 if bg == 1 else 50
+
+
+def M_rows(bg):
+    return 46 if bg == 1 else 42
+
+
+# ---------------------------------------------------------------------------------
+def decode(llrs, bg, Z, F=0, crc_poly=CRC24B, max_iter=6, crc_bits=24, force=False, es_syn=False, out=None):
+    """Oracle decoder; returns (iterations or 0, packed bits)."""
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    nbytes = (K_b(bg) * Z + 7) // 8
+    if out is None:
+        out = np.zeros(nbytes, np.uint8)
+    cfg = _DecCfg(bg, Z, F, crc_bits, max_iter)
+    r = lib().orc_ldpc_decode(C.byref(cfg), int(force), int(es_syn), crc_poly, _p(llrs, _i8p), llrs.size,
+                              _p(out, _u8p))
+    return int(r), out
+
+
+def ref_decode(llrs, bg, Z, F=0, crc_poly=CRC24B, max_iter=6, crc_bits=24, kind=0, force=False, es_syn=False,
+               out=None, dec=None):
+    R = ref()
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    nbytes = (K_b(bg) * Z + 7) // 8
+    if out is None:
+        out = np.zeros(nbytes, np.uint8)
+    own = dec is None
+    if own:
+        dec = R.ref_decoder_create(kind, int(force), int(es_syn))
+    try:
+        r = R.ref_decoder_decode(dec, _p(llrs, _i8p), llrs.size, bg, Z, F, crc_bits, max_iter, crc_poly,
+                                 _p(out, _u8p))
+    finally:
+        if own:
+            R.ref_decoder_destroy(dec)
+    return int(r), out
+
+
+def rate_dematch(llrs_e, bg, Z, rv=0, Qm=2, Nref=0, F=0, new_data=True, harq=None):
+    N = N_short(bg) * Z
+    out = np.zeros(N, np.int8) if harq is None else np.array(harq, dtype=np.int8, copy=True)
+    llrs_e = np.ascontiguousarray(llrs_e, dtype=np.int8)
+    cfg = _RmCfg(rv, Qm, Nref, F)
+    rc = lib().orc_rate_dematch(_p(out, _i8p), N, _p(llrs_e, _i8p), llrs_e.size, int(new_data), C.byref(cfg))
+    if rc != 0:
+        raise ValueError("invalid dematcher configuration")
+    return out
+
+
+def ref_rate_dematch(llrs_e, bg, Z, rv=0, Qm=2, Nref=0, F=0, new_data=True, harq=None):
+    N = N_short(bg) * Z
+    out = np.zeros(N, np.int8) if harq is None else np.array(harq, dtype=np.int8, copy=True)
+    llrs_e = np.ascontiguousarray(llrs_e, dtype=np.int8)
+    ref().ref_rate_dematch(_p(out, _i8p), N, _p(llrs_e, _i8p), llrs_e.size, int(new_data), bg, Z, rv, Qm, Nref, F)
+    return out
+
+
+def crc(poly, packed, nbits):
+    packed = np.ascontiguousarray(packed, dtype=np.uint8)
+    return int(lib().orc_crc_packed(poly, _p(packed, _u8p), nbits))
+
+
+def encode(msg_bits, bg, Z):
+    msg_bits = np.ascontiguousarray(msg_bits, dtype=np.uint8)
+    cw = np.zeros((K_b(bg) + M_rows(bg)) * Z, np.uint8)
+    rc = lib().orc_ldpc_encode(bg, Z, _p(msg_bits, _u8p), _p(cw, _u8p))
+    if rc != 0:
+        raise RuntimeError(f"encode failed ({rc})")
+    return cw
+
+
+def syndrome_weight(cw, bg, Z):
+    cw = np.ascontiguousarray(cw, dtype=np.uint8)
+    return int(lib().orc_syndrome_weight(bg, Z, _p(cw, _u8p)))
+
+
+def rate_match(d, E, bg, Z, rv=0, Qm=2, Nref=0, F=0):
+    d = np.ascontiguousarray(d, dtype=np.uint8)
+    out = np.zeros(E, np.uint8)
+    rc = lib().orc_rate_match(_p(out, _u8p), E, _p(d, _u8p), d.size, bg, Z, rv, Qm, Nref, F)
+    if rc != 0:
+        raise ValueError("invalid rate matcher configuration")
+    return out
+
+
+def awgn_llrs(bits, Qm, esn0_db, seed):
+    bits = np.ascontiguousarray(bits, dtype=np.uint8)
+    out = np.zeros(bits.size, np.int8)
+    n0 = 10.0 ** (-esn0_db / 10.0)
+    lib().orc_modulate_awgn_demap(_p(out, _i8p), _p(bits, _u8p), bits.size, Qm, n0, seed)
+    return out
+
+
+def pack_bits(bits):
+    return np.packbits(np.asarray(bits, dtype=np.uint8))
+
+
+def make_codeblocks(n_cb, bg, Z, E, Qm, esn0_db, seed, rv=0, F=0, crc_poly=CRC24B, crc_bits=24, Nref=0):
+    """Seeded synthetic codeblocks: random info bits + CB CRC (+ F zero fillers), LDPC
+    encoded, rate matched to E, mapped, AWGN at Es/N0, max-log demapped to int8.
+    Returns (info_bits_packed[n_cb, K_b*Z/8], llrs_e[n_cb, E])."""
+    rng = np.random.default_rng(seed)
+    K = K_b(bg) * Z
+    k_data = K - F - crc_bits
+    msgs = np.zeros((n_cb, (K + 7) // 8), np.uint8)
+    llrs = np.zeros((n_cb, E), np.int8)
+    for c in range(n_cb):
+        m = np.zeros(K, np.uint8)
+        m[:k_data] = rng.integers(0, 2, k_data, dtype=np.uint8)
+        pk = pack_bits(m[:k_data])
+        r = crc(crc_poly, pk, k_data)
+        for i in range(crc_bits):
+            m[k_data + i] = (r >> (crc_bits - 1 - i)) & 1
+        cw = encode(m, bg, Z)
+        d = cw[2 * Z: (K_b(bg) + M_rows(bg)) * Z]
+        e = rate_match(d, E, bg, Z, rv=rv, Qm=Qm, Nref=Nref, F=F)
+        llrs[c] = awgn_llrs(e, Qm, esn0_db, int(rng.integers(0, 2 ** 63)))
+        msgs[c] = pack_bits(m)
+    return msgs, llrs
