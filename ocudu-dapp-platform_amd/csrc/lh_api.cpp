@@ -15,13 +15,15 @@
 #include <string>
 #include <vector>
 
+#include "bg_tables.inc"
 #include "lh_internal.h"
 
 namespace lh {
 size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt);
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, const int8_t* in,
-                         uint8_t* bits, int32_t* results, const uint32_t* crc_x, hipStream_t s);
-hipError_t launch_rate_dematch(const rm_job* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
+hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int8_t* rm,
+                         const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
+                         const uint32_t* gtab, hipStream_t s);
+hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
@@ -48,6 +50,28 @@ int fail(int code, const char* fmt, ...)
 int hip_fail(hipError_t e, const char* what)
 {
   return fail(LH_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+const unsigned kLiftingSizes[kNofLiftingSizes] = {
+    2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12,  13,  14,  15,  16,  18,  20,  22,  24,  26,  28,  30,  32,  36,  40,
+    44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352,
+    384};
+
+int lifting_position(unsigned Z)
+{
+  for (int i = 0; i < kNofLiftingSizes; ++i) {
+    if (kLiftingSizes[i] == Z) {
+      return i;
+    }
+  }
+  return -1;
+}
+
+// Offset of the lifted edge table of graph (bg, Z) in the device table.
+uint32_t graph_offset(unsigned bg, unsigned Z)
+{
+  const int pos = lifting_position(Z);
+  return bg == 1 ? (uint32_t)pos * 316u : (uint32_t)(kNofLiftingSizes * 316 + pos * 197);
 }
 
 int lifting_index(unsigned Z)
@@ -98,6 +122,7 @@ struct lh_ctx {
   unsigned max_cb = 0;
   size_t slot_bytes = 0;
   uint32_t* d_crc_x = nullptr;
+  uint32_t* d_gtab = nullptr;
   slot ring[kRing];
   int next = 0;
 };
@@ -110,7 +135,7 @@ slot* acquire_slot(lh_ctx* ctx)
   slot* s = &ctx->ring[ctx->next];
   ctx->next = (ctx->next + 1) % kRing;
   if (s->used) {
-    hipEventSynchronize(s->done);
+    (void)hipEventSynchronize(s->done);
   }
   return s;
 }
@@ -170,8 +195,9 @@ int check_decoder_cfg(const lh_decoder_config& c, unsigned i)
   return LH_OK;
 }
 
-// Fills an rm_job from a descriptor; returns the largest possibly-nonzero position + 1.
-int make_rm_job(const lh_rm_desc& d, bool new_data, rm_job& j, uint64_t& max_pos1, unsigned i)
+// Fills the dematching parameters from a descriptor; max_pos1 receives the largest
+// position that can hold a nonzero soft bit, plus one.
+int make_rm_params(const lh_rm_desc& d, bool new_data, rm_params& j, uint64_t& max_pos1, unsigned i)
 {
   static const double k0_bg1[4] = {0, 17, 33, 56};
   static const double k0_bg2[4] = {0, 13, 25, 43};
@@ -202,29 +228,31 @@ int make_rm_job(const lh_rm_desc& d, bool new_data, rm_job& j, uint64_t& max_pos
   if (Ncb < nsys + 1) {
     return fail(LH_ERR_INVALID, "cb %u: N_ref %u shorter than the systematic part", i, d.Nref);
   }
+  // k0 (ldpc_rate_dematcher_impl.cpp:87-88, TS38.212 Table 5.4.2.1-2)
   const double* f = d.base_graph == 1 ? k0_bg1 : k0_bg2;
   const unsigned k0 = (unsigned)std::floor((f[d.rv] * Ncb) / N) * Z;
-  j = rm_job{};
+  const unsigned F = d.nof_filler_bits;
+  j = rm_params{};
   j.cw_off = d.cw_offset;
   j.rm_off = d.rm_offset;
   j.E = d.rm_length;
   j.N = N;
   j.Ncb = Ncb;
-  j.k0 = k0;
-  j.info = nsys - d.nof_filler_bits;
+  j.info = nsys - F;
   j.nsys = nsys;
+  j.P = Ncb - F;
+  j.start = k0 < j.info ? k0 : (k0 < nsys ? j.info : k0 - F);
+  j.Kq = d.rm_length / qm;
+  j.rcp_kq = 1.0f / (float)j.Kq;
   j.Qm = (uint16_t)qm;
   j.flags = new_data ? kFlagNewData : 0;
-  // Bound of the written positions.
   if (!new_data) {
     max_pos1 = N;
   } else {
-    const uint64_t F = d.nof_filler_bits, P = Ncb - F;
-    const uint64_t start = k0 < j.info ? k0 : (k0 < nsys ? j.info : k0 - F);
-    if (start + d.rm_length >= P) {
+    if ((uint64_t)j.start + d.rm_length >= j.P) {
       max_pos1 = Ncb;
     } else {
-      const uint64_t q = start + d.rm_length - 1;
+      const uint64_t q = (uint64_t)j.start + d.rm_length - 1;
       max_pos1 = (q < j.info ? q : q + F) + 1;
     }
     max_pos1 = std::max<uint64_t>(max_pos1, nsys); // fillers hold +127
@@ -238,8 +266,8 @@ struct group {
   int lmax = 4;
 };
 
-int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, const int8_t* in, uint8_t* bits, int32_t* results,
-                  hipStream_t stream)
+int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, int8_t* rm, const int8_t* cw, uint8_t* bits,
+                  int32_t* results, hipStream_t stream)
 {
   for (int bg = 1; bg <= 2; ++bg) {
     group& gr = g[bg - 1];
@@ -251,7 +279,7 @@ int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, const int8_t* in, 
       return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", gr.maxZ, gr.lmax);
     }
     hipError_t e = launch_decode(bg, reinterpret_cast<const dec_job*>(s->dev + off), (unsigned)gr.jobs.size(),
-                                 gr.maxZ, gr.lmax, in, bits, results, ctx->d_crc_x, stream);
+                                 gr.maxZ, gr.lmax, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab, stream);
     if (e != hipSuccess) {
       return hip_fail(e, "decoder launch");
     }
@@ -300,7 +328,7 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   lh_ctx* c = new lh_ctx();
   c->device = device;
   c->max_cb = max_codeblocks;
-  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_job)) * 2 + 4096;
+  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params)) + 4096;
   std::vector<uint32_t> x(3 * (size_t)kCrcXLen);
   crc_x_table(LH_CRC24A, x.data());
   crc_x_table(LH_CRC24B, x.data() + kCrcXLen);
@@ -308,6 +336,25 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   e = hipMalloc(&c->d_crc_x, x.size() * 4);
   if (e == hipSuccess) {
     e = hipMemcpy(c->d_crc_x, x.data(), x.size() * 4, hipMemcpyHostToDevice);
+  }
+  // Lifted edge tables: (col * Z) | ((V mod Z) << 16) per edge, rows in order
+  // (ldpc_graph_impl.cpp:11-39, ldpc_luts_impl.cpp:4505-4523).
+  std::vector<uint32_t> gt(kGraphTableLen);
+  for (int pos = 0; pos < kNofLiftingSizes; ++pos) {
+    const unsigned Z = kLiftingSizes[pos];
+    const int ils = lifting_index(Z);
+    for (int k = 0; k < 316; ++k) {
+      gt[graph_offset(1, Z) + k] = (uint32_t)(lh_bg1_col[k] * Z) | ((uint32_t)(lh_bg1_v[ils][k] % Z) << 16);
+    }
+    for (int k = 0; k < 197; ++k) {
+      gt[graph_offset(2, Z) + k] = (uint32_t)(lh_bg2_col[k] * Z) | ((uint32_t)(lh_bg2_v[ils][k] % Z) << 16);
+    }
+  }
+  if (e == hipSuccess) {
+    e = hipMalloc(&c->d_gtab, gt.size() * 4);
+  }
+  if (e == hipSuccess) {
+    e = hipMemcpy(c->d_gtab, gt.data(), gt.size() * 4, hipMemcpyHostToDevice);
   }
   for (int i = 0; i < kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(&c->ring[i].host, c->slot_bytes, hipHostMallocDefault);
@@ -331,21 +378,24 @@ int lh_ctx_destroy(lh_ctx* c)
   if (c == nullptr) {
     return LH_OK;
   }
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   for (auto& s : c->ring) {
     if (s.done) {
-      hipEventSynchronize(s.done);
-      hipEventDestroy(s.done);
+      (void)hipEventSynchronize(s.done);
+      (void)hipEventDestroy(s.done);
     }
     if (s.host) {
-      hipHostFree(s.host);
+      (void)hipHostFree(s.host);
     }
     if (s.dev) {
-      hipFree(s.dev);
+      (void)hipFree(s.dev);
     }
   }
   if (c->d_crc_x) {
-    hipFree(c->d_crc_x);
+    (void)hipFree(c->d_crc_x);
+  }
+  if (c->d_gtab) {
+    (void)hipFree(c->d_gtab);
   }
   delete c;
   return LH_OK;
@@ -391,11 +441,12 @@ int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const i
     j.crc_idx = (uint8_t)ci;
     j.flags = (uint8_t)(((flags & LH_FORCE_DECODING) ? kFlagForce : 0) |
                         ((flags & LH_EARLY_STOP_SYNDROME) ? kFlagEsSyndrome : 0));
-    j.lbound = layer_bound(b, Z, d.nof_llrs);
+    j.lbound = (uint16_t)layer_bound(b, Z, d.nof_llrs);
+    j.gt_off = graph_offset(d.cfg.base_graph, Z);
     j.out_idx = i;
     add_job(g, d.cfg.base_graph, j);
   }
-  hipSetDevice(ctx->device);
+  (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
   size_t bytes = stage_groups(s, 0, g);
@@ -403,7 +454,7 @@ int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const i
   if (rc != LH_OK) {
     return rc;
   }
-  return launch_groups(ctx, s, 0, g, d_llrs, d_bits, d_results, st);
+  return launch_groups(ctx, s, 0, g, const_cast<int8_t*>(d_llrs), nullptr, d_bits, d_results, st);
 }
 
 int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
@@ -418,25 +469,25 @@ int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8
   if (n == 0) {
     return LH_OK;
   }
-  std::vector<rm_job> jobs(n);
+  std::vector<rm_params> jobs(n);
   unsigned maxN = 0;
   for (unsigned i = 0; i < n; ++i) {
     uint64_t mp = 0;
-    int rc = make_rm_job(descs[i], new_data != 0, jobs[i], mp, i);
+    int rc = make_rm_params(descs[i], new_data != 0, jobs[i], mp, i);
     if (rc != LH_OK) {
       return rc;
     }
     maxN = std::max(maxN, jobs[i].N);
   }
-  hipSetDevice(ctx->device);
+  (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
-  std::memcpy(s->host, jobs.data(), n * sizeof(rm_job));
-  int rc = upload(ctx, s, n * sizeof(rm_job), st);
+  std::memcpy(s->host, jobs.data(), n * sizeof(rm_params));
+  int rc = upload(ctx, s, n * sizeof(rm_params), st);
   if (rc != LH_OK) {
     return rc;
   }
-  hipError_t e = launch_rate_dematch(reinterpret_cast<const rm_job*>(s->dev), n, maxN, d_cw, d_rm, st);
+  hipError_t e = launch_rate_dematch(reinterpret_cast<const rm_params*>(s->dev), n, maxN, d_cw, d_rm, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "dematcher launch");
 }
 
@@ -457,13 +508,12 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
   if (n == 0) {
     return LH_OK;
   }
-  std::vector<rm_job> rjobs(n);
   group g[2];
-  unsigned maxN = 0;
   for (unsigned i = 0; i < n; ++i) {
     const lh_pusch_cb_desc& d = cbs[i];
     uint64_t mp = 0;
-    int rc = make_rm_job(d.rm, new_data != 0, rjobs[i], mp, i);
+    dec_job j{};
+    int rc = make_rm_params(d.rm, new_data != 0, j.rm, mp, i);
     if (rc != LH_OK) {
       return rc;
     }
@@ -474,38 +524,32 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
     if (d.nof_crc_bits != 16 && d.nof_crc_bits != 24) {
       return fail(LH_ERR_INVALID, "cb %u: invalid number of CRC bits %u", i, d.nof_crc_bits);
     }
-    maxN = std::max(maxN, rjobs[i].N);
     bg_info b = bginfo(d.rm.base_graph);
     const unsigned Z = d.rm.lifting_size;
-    dec_job j{};
     j.in_off = d.rm.rm_offset;
     j.bits_off = d.bits_offset;
-    j.n_in = rjobs[i].N;
+    j.n_in = j.rm.N;
     j.Z = (uint16_t)Z;
     j.ils = (uint8_t)lifting_index(Z);
     j.max_iter = (uint8_t)nof_ldpc_iterations;
     j.F = (uint16_t)d.rm.nof_filler_bits;
     j.crc_idx = (uint8_t)ci;
-    j.flags = use_early_stop ? 0 : kFlagCrcAfter;
-    j.lbound = layer_bound(b, Z, mp);
+    j.flags = (uint8_t)((use_early_stop ? 0 : kFlagCrcAfter) | kFlagFused | kFlagHarqWrite |
+                        (new_data ? kFlagNewData : 0));
+    j.lbound = (uint16_t)layer_bound(b, Z, mp);
+    j.gt_off = graph_offset(d.rm.base_graph, Z);
     j.out_idx = i;
     add_job(g, d.rm.base_graph, j);
   }
-  hipSetDevice(ctx->device);
+  (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
-  std::memcpy(s->host, rjobs.data(), n * sizeof(rm_job));
-  size_t off = n * sizeof(rm_job);
-  size_t end = stage_groups(s, off, g);
+  size_t end = stage_groups(s, 0, g);
   int rc = upload(ctx, s, end, st);
   if (rc != LH_OK) {
     return rc;
   }
-  hipError_t e = launch_rate_dematch(reinterpret_cast<const rm_job*>(s->dev), n, maxN, d_cw, d_rm, st);
-  if (e != hipSuccess) {
-    return hip_fail(e, "dematcher launch");
-  }
-  return launch_groups(ctx, s, off, g, d_rm, d_bits, d_results, st);
+  return launch_groups(ctx, s, 0, g, d_rm, d_cw, d_bits, d_results, st);
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,
@@ -527,7 +571,7 @@ int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const 
       return fail(LH_ERR_INVALID, "buffer %u: %u bits exceed the supported %d", i, nbits[i], kCrcXLen - 24);
     }
   }
-  hipSetDevice(ctx->device);
+  (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
   std::memcpy(s->host, offsets, n * 8);

@@ -9,45 +9,56 @@ constexpr int kMaxThreads = 384;
 // x^e mod G tables: e in [0, 22*384 + 24).
 constexpr int kCrcXLen = 22 * 384 + 24;
 constexpr uint32_t kMaxRmLength = 22 * 384 * 35; // MAX_CODEBLOCK_RM_SIZE
+constexpr int kNofLiftingSizes = 51;
+// Lifted edge tables of all graphs: BG1 graphs first (316 edges each), then BG2 (197).
+constexpr int kGraphTableLen = kNofLiftingSizes * (316 + 197);
 
 // job flags
 constexpr uint8_t kFlagForce = 1;      // force_decoding
 constexpr uint8_t kFlagEsSyndrome = 2; // early_stop_syndrome (no CRC)
 constexpr uint8_t kFlagCrcAfter = 4;   // PUSCH without early stop: syndrome, then CRC
 constexpr uint8_t kFlagNewData = 8;    // rate dematch: copy instead of combine
+constexpr uint8_t kFlagFused = 16;     // decoder dematches its own input from the codeword LLRs
+constexpr uint8_t kFlagHarqWrite = 32; // fused: write the combined soft bits back (HARQ buffer)
 
-// One codeblock decode, as seen by the decoder kernel (48 bytes).
+// Rate-dematching parameters of one codeblock (64 bytes).
+struct alignas(16) rm_params {
+  uint64_t cw_off; // E LLRs, interleaved by Qm
+  uint64_t rm_off; // N soft bits (HARQ buffer)
+  uint32_t E;
+  uint32_t N;      // N_short * Z
+  uint32_t Ncb;    // circular buffer length
+  uint32_t start;  // rank of k0 among the non-filler positions
+  uint32_t info;   // (K_b-2)Z - F: first filler position
+  uint32_t nsys;   // (K_b-2)Z
+  uint32_t P;      // Ncb - F: non-filler positions of the circular buffer
+  uint32_t Kq;     // E / Qm
+  float rcp_kq;    // 1 / Kq
+  uint16_t Qm;
+  uint8_t flags;
+  uint8_t pad0;
+  uint64_t pad1;
+};
+static_assert(sizeof(rm_params) == 64, "rm_params layout");
+
+// One codeblock decode, as seen by the decoder kernel (112 bytes).
 struct alignas(16) dec_job {
-  uint64_t in_off;   // decoder input (rate-dematched soft bits) byte offset
+  uint64_t in_off;   // non-fused: decoder input (rate-dematched soft bits) byte offset
   uint64_t bits_off; // packed output byte offset
-  uint32_t n_in;     // number of input soft bits
+  uint32_t n_in;     // number of input soft bits (N when fused)
+  uint32_t gt_off;   // offset of this graph's lifted edge table
   uint16_t Z;
   uint8_t ils;       // lifting set index 0..7
   uint8_t max_iter;
   uint16_t F;        // filler bits
   uint8_t crc_idx;   // 0 none, 1 CRC24A, 2 CRC24B, 3 CRC16
   uint8_t flags;
-  uint32_t lbound;   // upper bound of the layer count for this job (4..M)
+  uint16_t lbound;   // upper bound of the layer count for this job (4..M)
+  uint16_t pad0;
   uint32_t out_idx;  // index of this codeblock's result
-  uint64_t pad1;
+  rm_params rm;      // fused dematching parameters
 };
-static_assert(sizeof(dec_job) == 48, "dec_job layout");
-
-// One codeblock rate dematch (48 bytes).
-struct alignas(16) rm_job {
-  uint64_t cw_off;   // E LLRs, interleaved by Qm
-  uint64_t rm_off;   // N soft bits
-  uint32_t E;
-  uint32_t N;        // N_short * Z
-  uint32_t Ncb;      // circular buffer length
-  uint32_t k0;
-  uint32_t info;     // (K_b-2)Z - F: first filler position
-  uint32_t nsys;     // (K_b-2)Z
-  uint16_t Qm;
-  uint8_t flags;
-  uint8_t pad[5];
-};
-static_assert(sizeof(rm_job) == 48, "rm_job layout");
+static_assert(sizeof(dec_job) == 112, "dec_job layout");
 
 inline int crc_index(uint32_t poly)
 {

@@ -6,7 +6,9 @@
 // edge becomes the LDS address (col*Z + (r + shift) mod Z) of the variable-node soft
 // bit that row r reads and writes.  Soft bits live in LDS as int8; the four core
 // layers keep their compressed check-to-variable state in VGPRs, the extension layers
-// in thread-private LDS words.  Barriers separate layers (strict layer order).
+// in thread-private LDS words.  The lifted edge table (col*Z, shift) of each graph is
+// read with scalar loads (uniform per workgroup).  Barriers separate layers (strict
+// layer order of the reference).
 #include <hip/hip_runtime.h>
 
 #include "bg_tables.inc"
@@ -21,92 +23,101 @@ template <>
 struct bgt<1> {
   static constexpr int K = 22, M = 46, NNZ = 316, DEG_EXT = 10, NSHORT = 66;
   __host__ __device__ static constexpr int rs(int i) { return lh_bg1_row_start[i]; }
-  __host__ __device__ static constexpr int col(int i) { return lh_bg1_col[i]; }
-  __host__ __device__ static constexpr int v(int ils, int i) { return lh_bg1_v[ils][i]; }
 };
 
 template <>
 struct bgt<2> {
   static constexpr int K = 10, M = 42, NNZ = 197, DEG_EXT = 6, NSHORT = 50;
   __host__ __device__ static constexpr int rs(int i) { return lh_bg2_row_start[i]; }
-  __host__ __device__ static constexpr int col(int i) { return lh_bg2_col[i]; }
-  __host__ __device__ static constexpr int v(int ils, int i) { return lh_bg2_v[ils][i]; }
 };
 
-// --- int8 log_likelihood_ratio arithmetic (values held in int) --------------------
-// Infinite LLR: |x| > 120 (only +-127 occurs).
-__device__ __forceinline__ bool llr_isinf(int x) { return (unsigned)(x + 120) > 240u; }
-// Saturated sum with the infinity rules; callers guarantee b is finite.
+// --- int8 log_likelihood_ratio arithmetic, values held in int -----------------------
+// (lib/phy/upper/log_likelihood_ratio.cpp:22-70).  Infinite LLR: |x| > 120.
+__device__ __forceinline__ int inf_mask(int x) { return -(int)((unsigned)(x + 120) > 240u); }
 __device__ __forceinline__ int llr_sat(int x) { return min(max(x, -120), 120); }
-// Promotion: |x| > 120 becomes +-127.
 __device__ __forceinline__ int llr_promote(int x) { return x > 120 ? 127 : (x < -120 ? -127 : x); }
-// round(0.8 * m) for 0 <= m <= 127 (no ties occur: 0.8m has fraction 0, .2, .4, .6, .8).
+__device__ __forceinline__ int sel(int m, int a, int b) { return (a & m) | (b & ~m); }
+// Saturated sum with the special cases in the reference's order.
+__device__ __forceinline__ int llr_add(int a, int b)
+{
+  int s = llr_sat(a + b);
+  s = sel(inf_mask(b), b, s);
+  s = sel(inf_mask(a), a, s);
+  return a == -b ? 0 : s;
+}
+// round(0.8 * m) for 0 <= m <= 127 (0.8m never ends in .5).
 __device__ __forceinline__ int scale08(int m) { return (4 * m + 2) / 5; }
 
-// One layer update for check row r.  DEG compile-time (core rows) or a runtime deg
-// <= DEG for extension rows.  State word layout: bits 0-6 scaled min1, 7-13 scaled
-// min2, 14-18 argmin edge, signs of the check-to-variable messages in `sgn`.
-// A zero state reproduces the reference's first visit (v2c = soft).
-template <int DEG, bool RUNTIME_DEG>
-__device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const uint32_t* __restrict__ et_row, int deg,
-                                             int r, int Z, uint32_t& st, uint32_t& sgn)
+// One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
+// two-min over composite keys; pass 2: c2v, promoted soft-bit update).  DEG is the
+// compile-time degree (core rows) or the maximum degree with a runtime `deg`.  State
+// word: bits 0-6 scaled min1, 7-13 scaled min2, 14-18 argmin edge; `sgn` holds the signs
+// of the check-to-variable messages.  A zero state reproduces the reference's first
+// visit (v2c = soft).
+template <int DEG, bool RT>
+__device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const uint32_t* __restrict__ gt, int deg,
+                                             unsigned r, unsigned Z, uint32_t& st, uint32_t& sgn)
 {
-  int v[DEG];
   int addr[DEG];
-  const int s1o = st & 127, s2o = (st >> 7) & 127, amo = (st >> 14) & 31;
-  // Composite keys |v2c| << 5 | edge: min over keys gives min1 and the first index
-  // reaching it (the reference's strict '<'); inits are the reference's LLR_MAX.
-  uint32_t m1 = 120u << 5, m2 = (120u << 5) | 31u;
-  int sp = 0;
+  int sv[DEG];
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
-    if (RUNTIME_DEG && e >= deg) {
-      break;
+    if (!RT || e < deg) {
+      const uint32_t g = gt[e];
+      unsigned j = r + (g >> 16);
+      j = min(j, j - Z);
+      addr[e] = (int)((g & 0xffffu) + j);
+      sv[e] = soft[addr[e]];
     }
-    const uint32_t et = et_row[e];
-    int j = r + (int)(et >> 16);
-    j = j >= Z ? j - Z : j;
-    addr[e] = (int)(et & 0xffffu) + j;
-    const int sv = soft[addr[e]];
-    const int mag = (e == amo) ? s2o : s1o;
-    const int c = ((sgn >> e) & 1u) ? -mag : mag;
-    const int x = llr_isinf(sv) ? sv : llr_sat(sv - c);
-    v[e] = x;
-    const uint32_t key = ((uint32_t)abs(x) << 5) | (uint32_t)e;
-    m2 = max(m1, min(m2, key));
-    m1 = min(m1, key);
-    sp ^= x;
+  }
+  const int s1o = st & 127, s2o = (st >> 7) & 127, amo = (st >> 14) & 31;
+  // Composite keys |v2c| << 5 | edge: the minimum key gives min1 and the first edge
+  // reaching it (the reference's strict '<'); both inits are the reference's LLR_MAX.
+  uint32_t m1 = 120u << 5, m2 = (120u << 5) | 31u;
+  int sp = 0;
+  int v[DEG];
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (!RT || e < deg) {
+      const int mag = (e == amo) ? s2o : s1o;
+      const int neg = -(int)((sgn >> e) & 1u);
+      const int c = (mag ^ neg) - neg;
+      const int x = sel(inf_mask(sv[e]), sv[e], llr_sat(sv[e] - c));
+      v[e] = x;
+      const uint32_t key = ((uint32_t)abs(x) << 5) | (uint32_t)e;
+      m2 = max(m1, min(m2, key));
+      m1 = min(m1, key);
+      sp ^= x;
+    }
   }
   const int amin = (int)(m1 & 31u);
   const int s1 = scale08((int)(m1 >> 5)), s2 = scale08((int)(m2 >> 5));
   uint32_t ns = 0;
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
-    if (RUNTIME_DEG && e >= deg) {
-      break;
+    if (!RT || e < deg) {
+      const int mag = (e == amin) ? s2 : s1;
+      const int neg = (sp ^ v[e]) >> 31; // 0 or -1: sign product without edge e
+      const int c = (mag ^ neg) - neg;
+      ns |= (uint32_t)(neg & 1) << e;
+      const int x = v[e];
+      soft[addr[e]] = (int8_t)sel(inf_mask(x), x, llr_promote(c + x));
     }
-    const int mag = (e == amin) ? s2 : s1;
-    const int neg = (sp ^ v[e]) >> 31; // 0 or -1: sign product excluding edge e
-    const int c = (mag ^ neg) - neg;
-    ns |= (uint32_t)(neg & 1) << e;
-    const int x = v[e];
-    const int sb = llr_isinf(x) ? x : llr_promote(c + x);
-    soft[addr[e]] = (int8_t)sb;
   }
   st = (uint32_t)s1 | ((uint32_t)s2 << 7) | ((uint32_t)amin << 14);
   sgn = ns;
 }
 
 template <int BG, int ROW>
-__device__ __forceinline__ void core_layer(int8_t* soft, const uint32_t* etab, int r, int Z, uint32_t& st,
+__device__ __forceinline__ void core_layer(int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, uint32_t& st,
                                            uint32_t& sgn)
 {
   constexpr int E0 = bgt<BG>::rs(ROW);
   constexpr int DEG = bgt<BG>::rs(ROW + 1) - E0;
-  layer_update<DEG, false>(soft, etab + E0, DEG, r, Z, st, sgn);
+  layer_update<DEG, false>(soft, gt + E0, DEG, r, Z, st, sgn);
 }
 
-// Wave64 + workgroup reductions.  `red` must hold 2 * (NT / 64) words.
+// Wave64 + workgroup reductions.  `red` holds 16 words.
 __device__ __forceinline__ void block_reduce_xor_or(uint32_t& x, uint32_t& z, uint32_t* red)
 {
 #pragma unroll
@@ -147,9 +158,9 @@ __device__ __forceinline__ int block_reduce_max(int x, int* red)
 }
 
 // Parity of every lifted check row of layers < L and the zero-soft-bit veto
-// (check_syndrome, ldpc_decoder_impl.cpp:329-387).  Returns nonzero when it fails.
+// (check_syndrome, ldpc_decoder_impl.cpp:329-387).  Nonzero when it fails.
 template <int BG>
-__device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* etab, int r, int Z, int L)
+__device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, int L)
 {
   uint32_t bad = 0;
   if (r < Z) {
@@ -157,10 +168,10 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* etab, int 
       const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
       uint32_t p = 0;
       for (int e = e0; e < e1; ++e) {
-        const uint32_t et = etab[e];
-        int j = r + (int)(et >> 16);
-        j = j >= Z ? j - Z : j;
-        const int s = soft[(int)(et & 0xffffu) + j];
+        const uint32_t g = gt[e];
+        unsigned j = r + (g >> 16);
+        j = min(j, j - Z);
+        const int s = soft[(g & 0xffffu) + j];
         bad |= (s == 0);
         p ^= (s <= 0);
       }
@@ -170,8 +181,8 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* etab, int 
   return bad;
 }
 
-// Writes the hard decisions of soft[0, nbits) MSB-first; bits of the last byte beyond
-// nbits keep their previous value.  all_ones: write ones instead (no-CRC failure path).
+// Hard decisions of soft[0, nbits), MSB-first; bits of the last byte beyond nbits keep
+// their previous value.  all_ones: ones instead (no-CRC failure path).
 __device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones)
 {
   const int nbytes = (nbits + 7) >> 3;
@@ -186,66 +197,114 @@ __device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, boo
     }
     const int valid = nbits - 8 * b;
     if (valid < 8) {
-      const uint32_t keep = 0xffu >> valid;
-      byte |= out[b] & keep;
+      byte |= out[b] & (0xffu >> valid);
     }
     out[b] = (uint8_t)byte;
   }
 }
 
+// Rate-dematched soft bit at position p of the N-long buffer (behaviour of
+// ldpc_rate_dematcher_impl.cpp:110-189 in gather form): fillers are +127; position p
+// with non-filler rank q receives the deinterleaved inputs i = q - start (mod P),
+// + P, + 2P, ... < E, each added with saturation onto 0 (new data) or the stored value.
+__device__ __forceinline__ int dematch_at(const rm_params& rp, const int8_t* __restrict__ in, int old, uint32_t p,
+                                          bool new_data)
+{
+  if (p >= rp.info && p < rp.nsys) {
+    return 127;
+  }
+  int val = new_data ? 0 : old;
+  if (p >= rp.Ncb) {
+    return val;
+  }
+  const uint32_t q = p < rp.info ? p : p - (rp.nsys - rp.info);
+  for (uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start; i < rp.E; i += rp.P) {
+    // e[i] = in[(i mod Kq) * Qm + i div Kq] (deinterleave_bits_Qm, :192-202); the
+    // quotient is below Qm, the float estimate is corrected to the exact value.
+    int qt = (int)((float)i * rp.rcp_kq);
+    int rm = (int)i - qt * (int)rp.Kq;
+    if (rm < 0) {
+      --qt;
+      rm += (int)rp.Kq;
+    } else if (rm >= (int)rp.Kq) {
+      ++qt;
+      rm -= (int)rp.Kq;
+    }
+    val = llr_add(val, in[rm * (int)rp.Qm + qt]);
+  }
+  return val;
+}
+
 // Layered normalized min-sum decoder, behaviour of ldpc_decoder_impl::decode
 // (lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145) with the arithmetic
-// of ldpc_decoder_generic.cpp:12-104.
+// of ldpc_decoder_generic.cpp:12-104.  With kFlagFused the workgroup first rate-
+// dematches its codeblock (writing the HARQ buffer when asked) instead of reading a
+// dematched buffer.
 template <int BG>
 __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __restrict__ jobs,
-                                                             const int8_t* __restrict__ in_base,
+                                                             int8_t* __restrict__ rm_base,
+                                                             const int8_t* __restrict__ cw_base,
                                                              uint8_t* __restrict__ bits_base,
                                                              int32_t* __restrict__ results,
-                                                             const uint32_t* __restrict__ crc_x, int lmax)
+                                                             const uint32_t* __restrict__ crc_x,
+                                                             const uint32_t* __restrict__ gtab, int lmax)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
-  constexpr int NNZ_PAD = (T::NNZ + 3) & ~3;
   const dec_job jb = jobs[blockIdx.x];
-  const int Z = jb.Z;
-  const int r = threadIdx.x, NT = blockDim.x;
+  const unsigned Z = jb.Z;
+  const unsigned r = threadIdx.x, NT = blockDim.x;
+  const uint32_t* __restrict__ gt = gtab + jb.gt_off;
 
   extern __shared__ __align__(16) uint8_t lds[];
-  const int soft_bytes = ((K + lmax) * Z + 15) & ~15;
+  const int soft_bytes = ((K + lmax) * (int)Z + 15) & ~15;
   int8_t* soft = reinterpret_cast<int8_t*>(lds);
-  uint32_t* etab = reinterpret_cast<uint32_t*>(lds + soft_bytes);
-  uint32_t* ext = etab + NNZ_PAD;
+  uint32_t* ext = reinterpret_cast<uint32_t*>(lds + soft_bytes);
   uint32_t* red = ext + (lmax - 4) * NT;
 
-  // Lifted edge table: (col*Z) | (shift << 16), shift = V mod Z (ldpc_luts_impl.cpp:4505-4523).
-  for (int e = r; e < T::NNZ; e += NT) {
-    etab[e] = (uint32_t)(T::col(e) * Z) | ((uint32_t)(T::v(jb.ils, e) % Z) << 16);
-  }
-
-  // load_soft_bits (ldpc_decoder_impl.cpp:147-188) and the last nonzero input (:67-71).
-  const int8_t* in = in_base + jb.in_off;
+  // Soft-bit load (load_soft_bits, ldpc_decoder_impl.cpp:147-188): two punctured nodes at
+  // zero, full nodes clamped to [-64, 64], a partial last node unclamped; track the last
+  // nonzero input (:67-71).
   const int n_in = (int)jb.n_in;
-  const int full = (n_in / Z) * Z;
-  const int cap = (K + lmax - 2) * Z;
+  const int full = (n_in / (int)Z) * (int)Z;
+  const int cap = (K + lmax - 2) * (int)Z;
   int last = -1;
-  for (int i = r; i < n_in; i += NT) {
-    const int x = in[i];
-    if (x != 0) {
-      last = i;
+  if (jb.flags & kFlagFused) {
+    const rm_params rp = jb.rm;
+    const int8_t* in = cw_base + rp.cw_off;
+    int8_t* harq = rm_base + rp.rm_off;
+    const bool new_data = jb.flags & kFlagNewData;
+    const bool wr = jb.flags & kFlagHarqWrite;
+    for (int p = (int)r; p < n_in; p += (int)NT) {
+      const int old = new_data ? 0 : harq[p];
+      const int x = dematch_at(rp, in, old, (uint32_t)p, new_data);
+      if (wr) {
+        harq[p] = (int8_t)x;
+      }
+      last = x != 0 ? p : last;
+      if (p < cap) {
+        soft[2 * Z + p] = (int8_t)min(max(x, -64), 64);
+      }
     }
-    if (i < cap) {
-      soft[2 * Z + i] = (int8_t)(i < full ? min(max(x, -64), 64) : x);
+  } else {
+    const int8_t* in = rm_base + jb.in_off;
+    for (int p = (int)r; p < n_in; p += (int)NT) {
+      const int x = in[p];
+      last = x != 0 ? p : last;
+      if (p < cap) {
+        soft[2 * Z + p] = (int8_t)(p < full ? min(max(x, -64), 64) : x);
+      }
     }
   }
-  for (int i = r; i < 2 * Z; i += NT) {
+  for (int i = (int)r; i < 2 * (int)Z; i += (int)NT) {
     soft[i] = 0;
   }
-  for (int i = n_in + r; i < cap; i += NT) {
+  for (int i = n_in + (int)r; i < cap; i += (int)NT) {
     soft[2 * Z + i] = 0;
   }
   const int input_size = block_reduce_max(last, reinterpret_cast<int*>(red)) + 1;
 
-  const int msg = K * Z;
+  const int msg = K * (int)Z;
   uint8_t* out = bits_base + jb.bits_off;
   if (input_size < msg && !(jb.flags & kFlagForce)) {
     // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
@@ -257,9 +316,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
     }
     return;
   }
-  int cbl = max(input_size + 2 * Z, msg + 4 * Z);
-  cbl = (cbl + Z - 1) / Z * Z;
-  const int L = cbl / Z - K;
+  int cbl = max(input_size + 2 * (int)Z, msg + 4 * (int)Z);
+  cbl = (cbl + (int)Z - 1) / (int)Z * (int)Z;
+  const int L = cbl / (int)Z - K;
   if (L > lmax) { // host bound violated: refuse rather than overrun LDS
     if (r == 0) {
       results[jb.out_idx] = -1;
@@ -279,7 +338,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   uint32_t xw[K];
 #pragma unroll
   for (int n = 0; n < K; ++n) {
-    const int i = n * Z + r;
+    const int i = n * (int)Z + (int)r;
     xw[n] = (use_crc && r < Z && i < len) ? crc_x[(jb.crc_idx - 1) * kCrcXLen + (len - 1 - i + order)] : 0u;
   }
   __syncthreads();
@@ -290,27 +349,27 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   const bool es_syn = !use_crc && (jb.flags & kFlagEsSyndrome);
   for (int it = 0; it < max_iter; ++it) {
     if (r < Z) {
-      core_layer<BG, 0>(soft, etab, r, Z, st0, sg0);
+      core_layer<BG, 0>(soft, gt, r, Z, st0, sg0);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 1>(soft, etab, r, Z, st1, sg1);
+      core_layer<BG, 1>(soft, gt, r, Z, st1, sg1);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 2>(soft, etab, r, Z, st2, sg2);
+      core_layer<BG, 2>(soft, gt, r, Z, st2, sg2);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 3>(soft, etab, r, Z, st3, sg3);
+      core_layer<BG, 3>(soft, gt, r, Z, st3, sg3);
     }
     __syncthreads();
     for (int l = 4; l < L; ++l) {
       if (r < Z) {
         const int e0 = T::rs(l);
-        uint32_t w = ext[(l - 4) * NT + r];
+        const uint32_t w = ext[(l - 4) * NT + r];
         uint32_t st = w & 0x7ffffu, sg = w >> 19;
-        layer_update<T::DEG_EXT, true>(soft, etab + e0, T::rs(l + 1) - e0, r, Z, st, sg);
+        layer_update<T::DEG_EXT, true>(soft, gt + e0, T::rs(l + 1) - e0, r, Z, st, sg);
         ext[(l - 4) * NT + r] = st | (sg << 19);
       }
       __syncthreads();
@@ -320,11 +379,15 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
       // get_hard_bits + CRC (ldpc_decoder_impl.cpp:112-119).
       uint32_t x = 0, z = 0;
       if (r < Z) {
+        int s[K];
 #pragma unroll
         for (int n = 0; n < K; ++n) {
-          const int s = soft[n * Z + r];
-          z |= (s == 0);
-          x ^= (s <= 0) ? xw[n] : 0u;
+          s[n] = soft[n * Z + r];
+        }
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+          z |= (s[n] == 0);
+          x ^= (s[n] <= 0) ? xw[n] : 0u;
         }
       }
       block_reduce_xor_or(x, z, red);
@@ -339,14 +402,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
           z |= (soft[n * Z + r] == 0);
         }
       }
-      z |= syndrome_fail<BG>(soft, etab, r, Z, L);
+      z |= syndrome_fail<BG>(soft, gt, r, Z, L);
       block_reduce_xor_or(x, z, red);
       if (z == 0) {
         result = it + 1;
         break;
       }
     }
-    __syncthreads();
   }
 
   if (!es_crc && !es_syn) {
@@ -360,7 +422,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
         x ^= (s <= 0) ? xw[n] : 0u;
       }
     }
-    z |= syndrome_fail<BG>(soft, etab, r, Z, L);
+    z |= syndrome_fail<BG>(soft, gt, r, Z, L);
     block_reduce_xor_or(x, z, red);
     result = (z == 0 && (!use_crc || x == 0)) ? max_iter : 0;
   }
@@ -371,46 +433,18 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   }
 }
 
-// Rate dematcher, behaviour of ldpc_rate_dematcher_impl::rate_dematch
-// (ldpc_rate_dematcher_impl.cpp:29-250) in gather form: output position p (not a
-// filler, below Ncb) has rank q among the non-filler positions; input e-index i lands
-// on rank (start + i) mod P, P = Ncb - F.  New data starts from 0, otherwise from the
-// stored soft bit; every input reaching p is added with saturation, in input order.
-// Grid: x = position chunks, y = codeblock.
-__global__ void __launch_bounds__(256) k_rate_dematch(const rm_job* __restrict__ jobs, const int8_t* __restrict__ cw,
-                                                      int8_t* __restrict__ rm)
+// Standalone rate dematcher (ldpc_rate_dematcher::rate_dematch).  Grid: x = position
+// chunks, y = codeblock.
+__global__ void __launch_bounds__(256) k_rate_dematch(const rm_params* __restrict__ jobs,
+                                                      const int8_t* __restrict__ cw, int8_t* __restrict__ rm)
 {
-  const rm_job jb = jobs[blockIdx.y];
-  const int8_t* in = cw + jb.cw_off;
-  int8_t* out = rm + jb.rm_off;
-  const uint32_t F = jb.nsys - jb.info;
-  const uint32_t P = jb.Ncb - F;
-  const uint32_t start = jb.k0 < jb.info ? jb.k0 : (jb.k0 < jb.nsys ? jb.info : jb.k0 - F);
-  const uint32_t Kq = jb.E / jb.Qm;
-  const bool new_data = jb.flags & kFlagNewData;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < jb.N; p += gridDim.x * blockDim.x) {
-    int val;
-    if (p >= jb.info && p < jb.nsys) {
-      val = 127;
-    } else if (p >= jb.Ncb) {
-      val = new_data ? 0 : out[p];
-    } else {
-      const uint32_t q = p < jb.info ? p : p - F;
-      val = new_data ? 0 : out[p];
-      for (uint32_t i = (q + P - start) % P; i < jb.E; i += P) {
-        const int x = in[(i % Kq) * jb.Qm + i / Kq];
-        // saturated sum with the infinity rules (log_likelihood_ratio.cpp:22-55)
-        if (val == -x) {
-          val = 0;
-        } else if (llr_isinf(val)) {
-        } else if (llr_isinf(x)) {
-          val = x;
-        } else {
-          val = llr_sat(val + x);
-        }
-      }
-    }
-    out[p] = (int8_t)val;
+  const rm_params rp = jobs[blockIdx.y];
+  const int8_t* in = cw + rp.cw_off;
+  int8_t* out = rm + rp.rm_off;
+  const bool new_data = rp.flags & kFlagNewData;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < rp.N; p += gridDim.x * blockDim.x) {
+    const int old = new_data ? 0 : out[p];
+    out[p] = (int8_t)dematch_at(rp, in, old, p, new_data);
   }
 }
 
@@ -438,34 +472,31 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt)
 {
   const int K = bg == 1 ? 22 : 10;
-  const int nnz_pad = bg == 1 ? 316 : 200;
-  return (size_t)(((K + lmax) * maxZ + 15) & ~15) + (size_t)nnz_pad * 4 + (size_t)(lmax - 4) * nt * 4 + 16 * 4;
+  return (size_t)(((K + lmax) * maxZ + 15) & ~15) + (size_t)(lmax - 4) * nt * 4 + 16 * 4;
 }
 
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, const int8_t* in,
-                         uint8_t* bits, int32_t* results, const uint32_t* crc_x, hipStream_t s)
+hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int8_t* rm,
+                         const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
+                         const uint32_t* gtab, hipStream_t s)
 {
   const int nt = ((maxZ + 63) / 64) * 64;
   const size_t lds = decode_lds_bytes(bg, maxZ, lmax, nt);
+  const void* fn = bg == 1 ? (const void*)k_ldpc_decode<1> : (const void*)k_ldpc_decode<2>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) {
+    return e;
+  }
   if (bg == 1) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_ldpc_decode<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) {
-      return e;
-    }
-    hipLaunchKernelGGL(k_ldpc_decode<1>, dim3(n), dim3(nt), lds, s, d_jobs, in, bits, results, crc_x, lmax);
+    hipLaunchKernelGGL(k_ldpc_decode<1>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
+                       lmax);
   } else {
-    hipError_t e = hipFuncSetAttribute((const void*)k_ldpc_decode<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) {
-      return e;
-    }
-    hipLaunchKernelGGL(k_ldpc_decode<2>, dim3(n), dim3(nt), lds, s, d_jobs, in, bits, results, crc_x, lmax);
+    hipLaunchKernelGGL(k_ldpc_decode<2>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
+                       lmax);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_rate_dematch(const rm_job* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
+hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s)
 {
   const unsigned chunks = (maxN + 1023) / 1024;

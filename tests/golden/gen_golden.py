@@ -1,0 +1,81 @@
+#!/usr/bin/env python3
+"""Writes the decoder golden fixtures from the reference's own generic decoder.
+
+The reference CPU code is built from its sources into oracle/_ref/libref_ldpc.so
+(oracle/Makefile.ref); this script feeds it seeded synthetic codeblocks and stores the
+inputs (rate-matched int8 LLRs + dematching parameters) and the reference outputs
+(rate-dematched buffer checksum, hard bits, iteration count / CRC verdict).  The
+fixtures are data only; the tests compare the oracle and the HIP path against them.
+
+    python tests/golden/gen_golden.py
+"""
+import sys
+import zlib
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent.parent / "oracle"))
+import oracle as O  # noqa: E402
+
+
+def run_case(dec, bg, Z, E, Qm, snr, seed, rv=0, F=0, max_iter=6):
+    K = O.K_b(bg) * Z
+    poly, cb = (O.CRC24B, 24) if K > 48 else (O.CRC16, 16)
+    _, le = O.make_codeblocks(1, bg, Z, E, Qm, snr, seed=seed, rv=rv, F=F, crc_poly=poly, crc_bits=cb)
+    rm = O.ref_rate_dematch(le[0], bg, Z, rv=rv, Qm=Qm, F=F)
+    r, bits = O.ref_decode(rm, bg, Z, F=F, crc_poly=poly, max_iter=max_iter, crc_bits=cb, dec=dec)
+    return le[0], rm, poly, cb, r, bits
+
+
+def main():
+    R = O.ref()
+    if R is None:
+        raise SystemExit("build oracle/_ref first (make -f oracle/Makefile.ref)")
+    dec = R.ref_decoder_create(0, 0, 0)  # generic, force_decoding=false, early_stop_syndrome=false
+
+    # cfg1: 64 CBs BG1 Z=384, E=16896, QPSK, Es/N0 2 dB, rv0, 6 iterations, CRC24B.
+    msgs, llrs = O.make_codeblocks(64, 1, 384, 16896, 2, 2.0, seed=20260101)
+    res, bits = [], []
+    for c in range(64):
+        rm = O.ref_rate_dematch(llrs[c], 1, 384, Qm=2)
+        r, b = O.ref_decode(rm, 1, 384, crc_poly=O.CRC24B, max_iter=6, dec=dec)
+        res.append(r)
+        bits.append(b)
+    np.savez_compressed(HERE / "cfg1_bg1_z384_qpsk_2db.npz", llrs=llrs, msgs=msgs, iters=np.array(res, np.int32),
+                        bits=np.array(bits), seed=20260101)
+
+    # sweep: every (BG, Z), two rates, two SNRs, random rv/Qm/fillers.
+    rng = np.random.default_rng(20260102)
+    rows = []
+    llr_blob = []
+    bits_blob = []
+    for bg in (1, 2):
+        for Z in O.LIFTING_SIZES:
+            N = O.N_short(bg) * Z
+            nsys = (O.K_b(bg) - 2) * Z
+            for frac in (0.45, 1.1):
+                for snr_off in (0.0, 2.5):
+                    Qm = int(rng.choice([2, 4, 6, 8]))
+                    rv = int(rng.integers(0, 4))
+                    F = int(rng.choice([0, 8, 24])) if nsys > 64 else 0
+                    E = max(Qm, int(N * frac) // Qm * Qm)
+                    snr = {2: 1.0, 4: 6.0, 6: 10.5, 8: 15.0}[Qm] + snr_off + (3.0 if frac < 1 else 0.0)
+                    seed = int(rng.integers(0, 2 ** 31))
+                    le, rm, poly, cb, r, b = run_case(dec, bg, Z, E, Qm, snr, seed, rv=rv, F=F)
+                    rows.append((bg, Z, E, Qm, rv, F, poly, cb, r, zlib.crc32(rm.tobytes())))
+                    llr_blob.append(le)
+                    bits_blob.append(b)
+    meta = np.array([r[:10] for r in rows], np.int64)
+    offs = np.cumsum([0] + [a.size for a in llr_blob])
+    boffs = np.cumsum([0] + [a.size for a in bits_blob])
+    np.savez_compressed(HERE / "sweep_bg_z.npz", meta=meta, llr=np.concatenate(llr_blob), llr_off=offs,
+                        bits=np.concatenate(bits_blob), bits_off=boffs,
+                        columns=np.array("bg Z E Qm rv F crc_poly crc_bits iters rm_crc32".split()))
+    R.ref_decoder_destroy(dec)
+    print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+if __name__ == "__main__":
+    main()

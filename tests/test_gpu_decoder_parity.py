@@ -45,11 +45,12 @@ def test_all_lifting_sizes(ctx, bg):
     for k, Z in enumerate(O.LIFTING_SIZES):
         K = O.K_b(bg) * Z
         E = ((O.N_short(bg) * Z * 2 // 3) // 4) * 4
+        poly, cb = (O.CRC24B, 24) if K > 48 else (O.CRC16, 16)
         for snr, seed in ((0.5, 3 * k), (3.0, 3 * k + 1)):
-            _, le = O.make_codeblocks(1, bg, Z, E, 4, snr, seed=seed + 1000 * bg)
+            _, le = O.make_codeblocks(1, bg, Z, E, 4, snr, seed=seed + 1000 * bg, crc_poly=poly, crc_bits=cb)
             llrs.append(O.rate_dematch(le[0], bg, Z, Qm=4))
-            cfgs.append((bg, Z, 0, 24, 8))
-            polys.append(O.CRC24B)
+            cfgs.append((bg, Z, 0, cb, 8))
+            polys.append(poly)
     _check(ctx, llrs, cfgs, polys)
 
 
