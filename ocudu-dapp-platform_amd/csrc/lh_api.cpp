@@ -19,8 +19,8 @@
 #include "lh_internal.h"
 
 namespace lh {
-size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt);
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int8_t* rm,
+size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt, int stage_cap);
+hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int stage_cap, int8_t* rm,
                          const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
                          const uint32_t* gtab, hipStream_t s);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
@@ -264,7 +264,11 @@ struct group {
   std::vector<dec_job> jobs;
   int maxZ = 0;
   int lmax = 4;
+  int maxE = 0; // fused jobs: largest rate-matched length (LDS staging)
 };
+
+// Codeword LLRs are staged in LDS when the largest E of a launch fits this budget.
+constexpr int kStageBudget = 48 * 1024;
 
 int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, int8_t* rm, const int8_t* cw, uint8_t* bits,
                   int32_t* results, hipStream_t stream)
@@ -275,11 +279,16 @@ int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, int8_t* rm, const 
       continue;
     }
     const int nt = ((gr.maxZ + 63) / 64) * 64;
-    if (decode_lds_bytes(bg, gr.maxZ, gr.lmax, nt) > 160 * 1024) {
+    if (decode_lds_bytes(bg, gr.maxZ, gr.lmax, nt, 0) > 160 * 1024) {
       return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", gr.maxZ, gr.lmax);
     }
+    int stage_cap = gr.maxE <= kStageBudget ? ((gr.maxE + 15) & ~15) : 0;
+    if (decode_lds_bytes(bg, gr.maxZ, gr.lmax, nt, stage_cap) > 160 * 1024) {
+      stage_cap = 0;
+    }
     hipError_t e = launch_decode(bg, reinterpret_cast<const dec_job*>(s->dev + off), (unsigned)gr.jobs.size(),
-                                 gr.maxZ, gr.lmax, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab, stream);
+                                 gr.maxZ, gr.lmax, stage_cap, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
+                                 stream);
     if (e != hipSuccess) {
       return hip_fail(e, "decoder launch");
     }
@@ -305,6 +314,9 @@ void add_job(group* g, unsigned bg, const dec_job& j)
   gr.jobs.push_back(j);
   gr.maxZ = std::max<int>(gr.maxZ, j.Z);
   gr.lmax = std::max<int>(gr.lmax, (int)j.lbound);
+  if (j.flags & kFlagFused) {
+    gr.maxE = std::max<int>(gr.maxE, (int)j.rm.E);
+  }
 }
 
 } // namespace

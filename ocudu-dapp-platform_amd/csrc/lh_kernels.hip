@@ -45,19 +45,51 @@ __device__ __forceinline__ int llr_add(int a, int b)
   s = sel(inf_mask(a), a, s);
   return a == -b ? 0 : s;
 }
+__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c)
+{
+  return max(min(a, b), min(max(a, b), c));
+}
+// x * x for |x| < 2^11 with the full-rate 24-bit multiplier.
+__device__ __forceinline__ uint32_t sq24(int x)
+{
+  uint32_t d;
+  asm("v_mul_i32_i24 %0, %1, %1" : "=v"(d) : "v"(x));
+  return d;
+}
+__device__ __forceinline__ uint32_t med3u_asm(uint32_t a, uint32_t b, uint32_t c)
+{
+  uint32_t d;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+__device__ __forceinline__ int med3i(int a, int lo, int hi) { return min(max(a, lo), hi); }
+// sqrt of a perfect square n^2 <= 14400 (rounded, exact).
+__device__ __forceinline__ int isqrt_sq(uint32_t q) { return (int)(__fsqrt_rn((float)q) + 0.5f); }
 // round(0.8 * m) for 0 <= m <= 127 (0.8m never ends in .5).
 __device__ __forceinline__ int scale08(int m) { return (4 * m + 2) / 5; }
 
+// v2c from a soft bit and the previous check-to-variable message: saturated difference
+// for a finite soft bit; an infinite soft bit (|sv| = 127) passes through as +-(269..358),
+// which never becomes a minimum and promotes back to +-127 whatever the new message.
+__device__ __forceinline__ int v2c_of(int sv, int c)
+{
+  const int ex = sv - med3i(sv, -120, 120); // 0, or +-7 for an infinite soft bit
+  return med3i(sv - c, -120, 120) + 34 * ex;
+}
+
 // One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
 // two-min over composite keys; pass 2: c2v, promoted soft-bit update).  DEG is the
-// compile-time degree (core rows) or the maximum degree with a runtime `deg`.  State
-// word: bits 0-6 scaled min1, 7-13 scaled min2, 14-18 argmin edge; `sgn` holds the signs
-// of the check-to-variable messages.  A zero state reproduces the reference's first
-// visit (v2c = soft).
-template <int DEG, bool RT>
+// compile-time degree (core rows) or the maximum degree with a runtime `deg`.
+// Check-to-variable state, two forms:
+//  * PACKED: the DEG messages themselves, int8, four per word (core layers, in VGPRs);
+//  * otherwise one word: bits 0-6 scaled min1, 7-13 scaled min2, 14-18 argmin edge,
+//    19-31 the message signs (extension layers, degree <= 13, in LDS).
+// A zero state reproduces the reference's first visit (v2c = soft).
+template <int DEG, bool RT, bool PACKED>
 __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const uint32_t* __restrict__ gt, int deg,
-                                             unsigned r, unsigned Z, uint32_t& st, uint32_t& sgn)
+                                             unsigned r, unsigned Z, uint32_t* __restrict__ state)
 {
+  constexpr int NW = (DEG + 3) / 4;
   int addr[DEG];
   int sv[DEG];
 #pragma unroll
@@ -70,51 +102,79 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const ui
       sv[e] = soft[addr[e]];
     }
   }
-  const int s1o = st & 127, s2o = (st >> 7) & 127, amo = (st >> 14) & 31;
-  // Composite keys |v2c| << 5 | edge: the minimum key gives min1 and the first edge
+  int s1o = 0, s2o = 0, amo = 0;
+  uint32_t sgo = 0;
+  if (!PACKED) {
+    const uint32_t st = state[0];
+    s1o = st & 127;
+    s2o = (st >> 7) & 127;
+    amo = (st >> 14) & 31;
+    sgo = st >> 19;
+  }
+  // Composite keys v2c^2 << 5 | edge: the minimum key gives min1 and the first edge
   // reaching it (the reference's strict '<'); both inits are the reference's LLR_MAX.
-  uint32_t m1 = 120u << 5, m2 = (120u << 5) | 31u;
+  uint32_t m1 = (120u * 120u) << 5, m2 = ((120u * 120u) << 5) | 31u;
   int sp = 0;
   int v[DEG];
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (!RT || e < deg) {
-      const int mag = (e == amo) ? s2o : s1o;
-      const int neg = -(int)((sgn >> e) & 1u);
-      const int c = (mag ^ neg) - neg;
-      const int x = sel(inf_mask(sv[e]), sv[e], llr_sat(sv[e] - c));
+      int c;
+      if (PACKED) {
+        c = (int)(int8_t)(state[e >> 2] >> (8 * (e & 3)));
+      } else {
+        const int mag = (e == amo) ? s2o : s1o;
+        const int neg = -(int)((sgo >> e) & 1u);
+        c = (mag ^ neg) - neg;
+      }
+      const int x = v2c_of(sv[e], c);
       v[e] = x;
-      const uint32_t key = ((uint32_t)abs(x) << 5) | (uint32_t)e;
-      m2 = max(m1, min(m2, key));
+      const uint32_t key = (sq24(x) << 5) | (uint32_t)e;
+      m2 = med3u_asm(m1, m2, key); // second smallest of {m1 <= m2, key}
       m1 = min(m1, key);
       sp ^= x;
     }
   }
   const int amin = (int)(m1 & 31u);
-  const int s1 = scale08((int)(m1 >> 5)), s2 = scale08((int)(m2 >> 5));
+  int s1 = scale08(isqrt_sq(m1 >> 5)), s2 = scale08(isqrt_sq(m2 >> 5));
+  // Materialise the two scaled magnitudes once (keeps the per-edge selects cheap).
+  asm volatile("" : "+v"(s1), "+v"(s2));
   uint32_t ns = 0;
+  uint32_t w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    w[k] = 0;
+  }
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (!RT || e < deg) {
       const int mag = (e == amin) ? s2 : s1;
       const int neg = (sp ^ v[e]) >> 31; // 0 or -1: sign product without edge e
       const int c = (mag ^ neg) - neg;
-      ns |= (uint32_t)(neg & 1) << e;
-      const int x = v[e];
-      soft[addr[e]] = (int8_t)sel(inf_mask(x), x, llr_promote(c + x));
+      if (PACKED) {
+        w[e >> 2] |= (uint32_t)(c & 0xff) << (8 * (e & 3));
+      } else {
+        ns |= (uint32_t)(neg & 1) << e;
+      }
+      soft[addr[e]] = (int8_t)llr_promote(c + v[e]);
     }
   }
-  st = (uint32_t)s1 | ((uint32_t)s2 << 7) | ((uint32_t)amin << 14);
-  sgn = ns;
+  if (PACKED) {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      state[k] = w[k];
+    }
+  } else {
+    state[0] = (uint32_t)s1 | ((uint32_t)s2 << 7) | ((uint32_t)amin << 14) | (ns << 19);
+  }
 }
 
 template <int BG, int ROW>
-__device__ __forceinline__ void core_layer(int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, uint32_t& st,
-                                           uint32_t& sgn)
+__device__ __forceinline__ void core_layer(int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, uint32_t* cpk)
 {
   constexpr int E0 = bgt<BG>::rs(ROW);
   constexpr int DEG = bgt<BG>::rs(ROW + 1) - E0;
-  layer_update<DEG, false>(soft, gt + E0, DEG, r, Z, st, sgn);
+  layer_update<DEG, false, true>(soft, gt + E0, DEG, r, Z, cpk);
 }
 
 // Wave64 + workgroup reductions.  `red` holds 16 words.
@@ -235,6 +295,49 @@ __device__ __forceinline__ int dematch_at(const rm_params& rp, const int8_t* __r
   return val;
 }
 
+// Fused dematching of positions [0, n_in): four consecutive positions per thread, one
+// dword HARQ access when aligned; writes the clamped soft bits of positions < cap.
+__device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_t* __restrict__ src,
+                                                int8_t* __restrict__ harq, bool new_data, bool wr, int n_in, int cap,
+                                                int8_t* __restrict__ soft2, unsigned r, unsigned NT, int& last)
+{
+  const bool al4 = ((reinterpret_cast<uintptr_t>(harq)) & 3) == 0;
+  for (int p0 = 4 * (int)r; p0 < n_in; p0 += 4 * (int)NT) {
+    uint32_t oldw = 0;
+    if (!new_data) {
+      if (al4 && p0 + 4 <= n_in) {
+        oldw = *reinterpret_cast<const uint32_t*>(harq + p0);
+      } else {
+        for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
+          oldw |= (uint32_t)(uint8_t)harq[p0 + k] << (8 * k);
+        }
+      }
+    }
+    uint32_t outw = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = p0 + k;
+      if (p < n_in) {
+        const int x = dematch_at(rp, src, (int)(int8_t)(oldw >> (8 * k)), (uint32_t)p, new_data);
+        outw |= (uint32_t)(uint8_t)x << (8 * k);
+        last = x != 0 ? p : last;
+        if (p < cap) {
+          soft2[p] = (int8_t)min(max(x, -64), 64);
+        }
+      }
+    }
+    if (wr) {
+      if (al4 && p0 + 4 <= n_in) {
+        *reinterpret_cast<uint32_t*>(harq + p0) = outw;
+      } else {
+        for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
+          harq[p0 + k] = (int8_t)(outw >> (8 * k));
+        }
+      }
+    }
+  }
+}
+
 // Layered normalized min-sum decoder, behaviour of ldpc_decoder_impl::decode
 // (lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145) with the arithmetic
 // of ldpc_decoder_generic.cpp:12-104.  With kFlagFused the workgroup first rate-
@@ -247,7 +350,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
                                                              uint8_t* __restrict__ bits_base,
                                                              int32_t* __restrict__ results,
                                                              const uint32_t* __restrict__ crc_x,
-                                                             const uint32_t* __restrict__ gtab, int lmax)
+                                                             const uint32_t* __restrict__ gtab, int lmax,
+                                                             int stage_cap)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
@@ -261,6 +365,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   int8_t* soft = reinterpret_cast<int8_t*>(lds);
   uint32_t* ext = reinterpret_cast<uint32_t*>(lds + soft_bytes);
   uint32_t* red = ext + (lmax - 4) * NT;
+  int8_t* stage = reinterpret_cast<int8_t*>(red + 16);
 
   // Soft-bit load (load_soft_bits, ldpc_decoder_impl.cpp:147-188): two punctured nodes at
   // zero, full nodes clamped to [-64, 64], a partial last node unclamped; track the last
@@ -275,24 +380,53 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
     int8_t* harq = rm_base + rp.rm_off;
     const bool new_data = jb.flags & kFlagNewData;
     const bool wr = jb.flags & kFlagHarqWrite;
-    for (int p = (int)r; p < n_in; p += (int)NT) {
-      const int old = new_data ? 0 : harq[p];
-      const int x = dematch_at(rp, in, old, (uint32_t)p, new_data);
-      if (wr) {
-        harq[p] = (int8_t)x;
+    // Stage the E codeword LLRs in LDS with aligned 16-byte loads (a 16-byte block that
+    // holds a byte of the codeblock lies in that byte's page).
+    const int8_t* src = in;
+    if (stage_cap >= (int)rp.E) {
+      const uintptr_t a = reinterpret_cast<uintptr_t>(in);
+      const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
+      const int lead = (int)(a & 15);
+      const int nblk = (lead + (int)rp.E + 15) >> 4;
+      for (int b = (int)r; b < nblk; b += (int)NT) {
+        const uint4 w = blk[b];
+        const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int idx = 16 * b + k - lead;
+          if (idx >= 0 && idx < (int)rp.E) {
+            stage[idx] = (int8_t)(ww[k >> 2] >> (8 * (k & 3)));
+          }
+        }
       }
-      last = x != 0 ? p : last;
-      if (p < cap) {
-        soft[2 * Z + p] = (int8_t)min(max(x, -64), 64);
-      }
+      __syncthreads();
+      src = stage;
+    }
+    if (src == stage) {
+      fused_positions(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, NT, last);
+    } else {
+      fused_positions(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, NT, last);
     }
   } else {
+    // Aligned 16-byte loads of the dematched soft bits.
     const int8_t* in = rm_base + jb.in_off;
-    for (int p = (int)r; p < n_in; p += (int)NT) {
-      const int x = in[p];
-      last = x != 0 ? p : last;
-      if (p < cap) {
-        soft[2 * Z + p] = (int8_t)(p < full ? min(max(x, -64), 64) : x);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(in);
+    const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
+    const int lead = (int)(a & 15);
+    const int nblk = (lead + n_in + 15) >> 4;
+    for (int b = (int)r; b < nblk; b += (int)NT) {
+      const uint4 w = blk[b];
+      const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int p = 16 * b + k - lead;
+        const int x = (int)(int8_t)(ww[k >> 2] >> (8 * (k & 3)));
+        if (p >= 0 && p < n_in) {
+          last = x != 0 ? p : last;
+          if (p < cap) {
+            soft[2 * Z + p] = (int8_t)(p < full ? min(max(x, -64), 64) : x);
+          }
+        }
       }
     }
   }
@@ -329,18 +463,25 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   for (int l = 4; l < L; ++l) {
     ext[(l - 4) * NT + r] = 0;
   }
-  uint32_t st0 = 0, sg0 = 0, st1 = 0, sg1 = 0, st2 = 0, sg2 = 0, st3 = 0, sg3 = 0;
+  // Core-layer check-to-variable messages, int8 packed four per word (VGPRs).
+  constexpr int CW0 = (bgt<BG>::rs(1) - bgt<BG>::rs(0) + 3) / 4, CW1 = (bgt<BG>::rs(2) - bgt<BG>::rs(1) + 3) / 4;
+  constexpr int CW2 = (bgt<BG>::rs(3) - bgt<BG>::rs(2) + 3) / 4, CW3 = (bgt<BG>::rs(4) - bgt<BG>::rs(3) + 3) / 4;
+  uint32_t c0[CW0], c1[CW1], c2[CW2], c3[CW3];
+#pragma unroll
+  for (int k = 0; k < CW0; ++k) c0[k] = 0;
+#pragma unroll
+  for (int k = 0; k < CW1; ++k) c1[k] = 0;
+#pragma unroll
+  for (int k = 0; k < CW2; ++k) c2[k] = 0;
+#pragma unroll
+  for (int k = 0; k < CW3; ++k) c3[k] = 0;
 
-  // CRC weights: hard bit i of the message contributes x^(len-1-i+order) mod G.
+  // CRC weights: hard bit i of the message contributes x^(len-1-i+order) mod G; thread r
+  // reads the weights of bits n*Z + r (n < K) from the table when it checks the CRC.
   const int len = msg - (int)jb.F;
   const bool use_crc = jb.crc_idx != 0;
   const int order = jb.crc_idx == 3 ? 16 : 24;
-  uint32_t xw[K];
-#pragma unroll
-  for (int n = 0; n < K; ++n) {
-    const int i = n * (int)Z + (int)r;
-    xw[n] = (use_crc && r < Z && i < len) ? crc_x[(jb.crc_idx - 1) * kCrcXLen + (len - 1 - i + order)] : 0u;
-  }
+  const uint32_t* __restrict__ xtab = crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0) + (len - 1 + order) - (int)r;
   __syncthreads();
 
   int result = 0;
@@ -349,28 +490,25 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   const bool es_syn = !use_crc && (jb.flags & kFlagEsSyndrome);
   for (int it = 0; it < max_iter; ++it) {
     if (r < Z) {
-      core_layer<BG, 0>(soft, gt, r, Z, st0, sg0);
+      core_layer<BG, 0>(soft, gt, r, Z, c0);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 1>(soft, gt, r, Z, st1, sg1);
+      core_layer<BG, 1>(soft, gt, r, Z, c1);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 2>(soft, gt, r, Z, st2, sg2);
+      core_layer<BG, 2>(soft, gt, r, Z, c2);
     }
     __syncthreads();
     if (r < Z) {
-      core_layer<BG, 3>(soft, gt, r, Z, st3, sg3);
+      core_layer<BG, 3>(soft, gt, r, Z, c3);
     }
     __syncthreads();
     for (int l = 4; l < L; ++l) {
       if (r < Z) {
         const int e0 = T::rs(l);
-        const uint32_t w = ext[(l - 4) * NT + r];
-        uint32_t st = w & 0x7ffffu, sg = w >> 19;
-        layer_update<T::DEG_EXT, true>(soft, gt + e0, T::rs(l + 1) - e0, r, Z, st, sg);
-        ext[(l - 4) * NT + r] = st | (sg << 19);
+        layer_update<T::DEG_EXT, true, false>(soft, gt + e0, T::rs(l + 1) - e0, r, Z, &ext[(l - 4) * NT + r]);
       }
       __syncthreads();
     }
@@ -380,9 +518,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
       uint32_t x = 0, z = 0;
       if (r < Z) {
         int s[K];
+        uint32_t xw[K];
+        const uint32_t* xt = xtab;
+        asm volatile("" : "+v"(xt)); // re-read the weights each check (keeps them out of VGPRs)
 #pragma unroll
         for (int n = 0; n < K; ++n) {
           s[n] = soft[n * Z + r];
+          xw[n] = (n * (int)Z + (int)r < len) ? xt[-n * (int)Z] : 0u;
         }
 #pragma unroll
         for (int n = 0; n < K; ++n) {
@@ -419,7 +561,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
       for (int n = 0; n < K; ++n) {
         const int s = soft[n * Z + r];
         z |= (s == 0);
-        x ^= (s <= 0) ? xw[n] : 0u;
+        x ^= (use_crc && s <= 0 && n * (int)Z + (int)r < len) ? xtab[-n * (int)Z] : 0u;
       }
     }
     z |= syndrome_fail<BG>(soft, gt, r, Z, L);
@@ -469,18 +611,18 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 }
 
 // ---- host-side launchers ------------------------------------------------------------
-size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt)
+size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt, int stage_cap)
 {
   const int K = bg == 1 ? 22 : 10;
-  return (size_t)(((K + lmax) * maxZ + 15) & ~15) + (size_t)(lmax - 4) * nt * 4 + 16 * 4;
+  return (size_t)(((K + lmax) * maxZ + 15) & ~15) + (size_t)(lmax - 4) * nt * 4 + 16 * 4 + (size_t)stage_cap;
 }
 
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int8_t* rm,
+hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int stage_cap, int8_t* rm,
                          const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
                          const uint32_t* gtab, hipStream_t s)
 {
   const int nt = ((maxZ + 63) / 64) * 64;
-  const size_t lds = decode_lds_bytes(bg, maxZ, lmax, nt);
+  const size_t lds = decode_lds_bytes(bg, maxZ, lmax, nt, stage_cap);
   const void* fn = bg == 1 ? (const void*)k_ldpc_decode<1> : (const void*)k_ldpc_decode<2>;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
@@ -488,10 +630,10 @@ hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, in
   }
   if (bg == 1) {
     hipLaunchKernelGGL(k_ldpc_decode<1>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
-                       lmax);
+                       lmax, stage_cap);
   } else {
     hipLaunchKernelGGL(k_ldpc_decode<2>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
-                       lmax);
+                       lmax, stage_cap);
   }
   return hipGetLastError();
 }
