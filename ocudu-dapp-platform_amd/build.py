@@ -44,6 +44,19 @@ def build(verbose=True):
         objs.append(o)
     out = LIB / "libocudu_hip_ldpc.so"
     run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out, *objs])
+    # C++ adapters implementing the reference's class interfaces over the C ABI.
+    adapters = LIB / "libocudu_hip_adapters.so"
+    run([HIPCC, *COMMON, f"-I{PKG / 'host'}", "-shared", PKG / "host" / "hip_ldpc_adapters.cpp", "-o", adapters,
+         f"-L{LIB}", "-locudu_hip_ldpc", f"-Wl,-rpath,{LIB}"])
+    return out
+
+
+def build_tests(oracle_lib):
+    """The adapter chain test (tests/cpp): links the adapters and the CPU oracle."""
+    out = ROOT / "tests" / "cpp" / "adapter_chain_test"
+    run([HIPCC, *COMMON, f"-I{PKG / 'host'}", ROOT / "tests" / "cpp" / "adapter_chain_test.cpp", "-o", out,
+         f"-L{LIB}", "-locudu_hip_adapters", "-locudu_hip_ldpc", f"-Wl,-rpath,{LIB}",
+         f"-L{oracle_lib.parent}", "-loracle", f"-Wl,-rpath,{oracle_lib.parent}"])
     return out
 
 
