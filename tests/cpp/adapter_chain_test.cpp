@@ -180,8 +180,8 @@ int main()
     EXPECT((r ? static_cast<int>(*r) : 0) == expect_iters[k], "decode case %zu iters %d vs %d", k, r ? (int)*r : 0,
            expect_iters[k]);
     EXPECT(out == expect_bits[k], "decode case %zu bits", k);
-    if (c.clean) {
-      // Noise-free: decodes, and the message is the transmitted one (chain test).
+    if (c.clean && c.rv == 0) {
+      // Noise-free and self-decodable (rv0): decodes to the transmitted message (chain test).
       std::vector<uint8_t> pk((K + 7) / 8, 0);
       for (unsigned i = 0; i != K; ++i) {
         pk[i / 8] |= msgs[k][i] << (7 - i % 8);
