@@ -19,10 +19,11 @@
 #include "lh_internal.h"
 
 namespace lh {
-size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt, int stage_cap);
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int stage_cap, int8_t* rm,
-                         const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
-                         const uint32_t* gtab, hipStream_t s);
+size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage);
+hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
+                         int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
+                         const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
+int decode_occupancy(int nt, size_t lds);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
@@ -119,6 +120,7 @@ struct slot {
 
 struct lh_ctx {
   int device = 0;
+  int nof_cus = 256;
   unsigned max_cb = 0;
   size_t slot_bytes = 0;
   uint32_t* d_crc_x = nullptr;
@@ -260,63 +262,106 @@ int make_rm_params(const lh_rm_desc& d, bool new_data, rm_params& j, uint64_t& m
   return LH_OK;
 }
 
-struct group {
+// Jobs of one decoder launch, turned into the (BG, Z)-sorted work queue of the
+// persistent kernel.
+struct launch_plan {
   std::vector<dec_job> jobs;
-  int maxZ = 0;
-  int lmax = 4;
-  int maxE = 0; // fused jobs: largest rate-matched length (LDS staging)
 };
 
-// Codeword LLRs are staged in LDS when the largest E of a launch fits this budget.
-constexpr int kStageBudget = 48 * 1024;
+// LDS budget for staging the codeword LLRs of one item.
+constexpr size_t kLdsBudget = 160 * 1024;
 
-int launch_groups(lh_ctx* ctx, slot* s, size_t off, group* g, int8_t* rm, const int8_t* cw, uint8_t* bits,
-                  int32_t* results, hipStream_t stream)
+// Codeblocks per work item: as many as fit the workgroup (G*Z <= threads).
+int groups_per_item(int Z, int nt)
 {
-  for (int bg = 1; bg <= 2; ++bg) {
-    group& gr = g[bg - 1];
-    if (gr.jobs.empty()) {
-      continue;
-    }
-    const int nt = ((gr.maxZ + 63) / 64) * 64;
-    if (decode_lds_bytes(bg, gr.maxZ, gr.lmax, nt, 0) > 160 * 1024) {
-      return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", gr.maxZ, gr.lmax);
-    }
-    int stage_cap = gr.maxE <= kStageBudget ? ((gr.maxE + 15) & ~15) : 0;
-    if (decode_lds_bytes(bg, gr.maxZ, gr.lmax, nt, stage_cap) > 160 * 1024) {
-      stage_cap = 0;
-    }
-    hipError_t e = launch_decode(bg, reinterpret_cast<const dec_job*>(s->dev + off), (unsigned)gr.jobs.size(),
-                                 gr.maxZ, gr.lmax, stage_cap, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
-                                 stream);
-    if (e != hipSuccess) {
-      return hip_fail(e, "decoder launch");
-    }
-    off += gr.jobs.size() * sizeof(dec_job);
-  }
-  return LH_OK;
+  return std::max(1, std::min(kMaxGroups, nt / Z));
 }
 
-size_t stage_groups(slot* s, size_t off, group* g)
+int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
+             hipStream_t stream, slot* s, size_t off)
 {
-  for (int k = 0; k < 2; ++k) {
-    if (!g[k].jobs.empty()) {
-      std::memcpy(s->host + off, g[k].jobs.data(), g[k].jobs.size() * sizeof(dec_job));
-      off += g[k].jobs.size() * sizeof(dec_job);
+  std::vector<dec_job>& jobs = plan.jobs;
+  if (jobs.empty()) {
+    return LH_OK;
+  }
+  int maxZ = 0;
+  for (const dec_job& j : jobs) {
+    maxZ = std::max<int>(maxZ, j.Z);
+  }
+  const int nt = ((maxZ + 63) / 64) * 64;
+  // Largest work first (longest-processing-time order for the dynamic queue); equal
+  // (BG, Z) stay adjacent so they pack into shared items.
+  std::stable_sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
+    const int ka = (int)a.Z * (a.lbound + 22), kb = (int)b.Z * (b.lbound + 22);
+    const int bga = a.gt_off < kNofLiftingSizes * 316 ? 1 : 2, bgb = b.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
+    if (ka != kb) {
+      return ka > kb;
     }
+    if (bga != bgb) {
+      return bga < bgb;
+    }
+    return a.Z > b.Z;
+  });
+  std::vector<dec_item> items;
+  size_t lds = 0;
+  for (size_t i = 0; i < jobs.size();) {
+    const int bg = jobs[i].gt_off < kNofLiftingSizes * 316 ? 1 : 2;
+    const int Z = jobs[i].Z;
+    const int G = groups_per_item(Z, nt);
+    dec_item it{};
+    it.first_job = (uint32_t)i;
+    it.bg = (uint8_t)bg;
+    int lmax = 4, maxE = 0;
+    bool fused = false;
+    size_t k = i;
+    while (k < jobs.size() && (int)(k - i) < G && jobs[k].Z == Z && jobs[k].gt_off == jobs[i].gt_off &&
+           jobs[k].lbound == jobs[i].lbound) {
+      lmax = std::max<int>(lmax, jobs[k].lbound);
+      if (jobs[k].flags & kFlagFused) {
+        fused = true;
+        maxE = std::max<int>(maxE, (int)jobs[k].rm.E);
+      }
+      ++k;
+    }
+    it.ngroups = (uint16_t)(k - i);
+    it.lmax = (uint16_t)lmax;
+    size_t need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, 0);
+    if (need > kLdsBudget) {
+      return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", Z, lmax);
+    }
+    const int stage = fused ? ((maxE + 15) & ~15) : 0;
+    if (stage > 0 && decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage) <= kLdsBudget) {
+      it.stage = (uint32_t)stage;
+      need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage);
+    }
+    lds = std::max(lds, need);
+    items.push_back(it);
+    i = k;
   }
-  return off;
-}
-
-void add_job(group* g, unsigned bg, const dec_job& j)
-{
-  group& gr = g[bg - 1];
-  gr.jobs.push_back(j);
-  gr.maxZ = std::max<int>(gr.maxZ, j.Z);
-  gr.lmax = std::max<int>(gr.lmax, (int)j.lbound);
-  if (j.flags & kFlagFused) {
-    gr.maxE = std::max<int>(gr.maxE, (int)j.rm.E);
+  // Device block: header | items | jobs.
+  launch_header hdr{};
+  hdr.next_item = 0;
+  hdr.n_items = (uint32_t)items.size();
+  const size_t o_items = off + sizeof(launch_header);
+  const size_t o_jobs = o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
+  const size_t end = o_jobs + jobs.size() * sizeof(dec_job);
+  if (end > ctx->slot_bytes) {
+    return fail(LH_ERR_CAPACITY, "descriptor block exceeds the staging capacity");
   }
+  std::memcpy(s->host + off, &hdr, sizeof(hdr));
+  std::memcpy(s->host + o_items, items.data(), items.size() * sizeof(dec_item));
+  std::memcpy(s->host + o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
+  int rc = upload(ctx, s, end, stream);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  const int per_cu = std::max(1, decode_occupancy(nt, lds));
+  const unsigned grid = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu);
+  hipError_t e = launch_decode(reinterpret_cast<launch_header*>(s->dev + off),
+                               reinterpret_cast<const dec_item*>(s->dev + o_items),
+                               reinterpret_cast<const dec_job*>(s->dev + o_jobs), grid, nt, lds, rm, cw, bits, results,
+                               ctx->d_crc_x, ctx->d_gtab, stream);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
 } // namespace
@@ -340,7 +385,8 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   lh_ctx* c = new lh_ctx();
   c->device = device;
   c->max_cb = max_codeblocks;
-  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params)) + 4096;
+  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params) + sizeof(dec_item)) + 4096;
+  (void)hipDeviceGetAttribute(&c->nof_cus, hipDeviceAttributeMultiprocessorCount, device);
   std::vector<uint32_t> x(3 * (size_t)kCrcXLen);
   crc_x_table(LH_CRC24A, x.data());
   crc_x_table(LH_CRC24B, x.data() + kCrcXLen);
@@ -425,7 +471,7 @@ int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const i
   if (n == 0) {
     return LH_OK;
   }
-  group g[2];
+  launch_plan plan;
   for (unsigned i = 0; i < n; ++i) {
     const lh_decode_desc& d = descs[i];
     int rc = check_decoder_cfg(d.cfg, i);
@@ -456,17 +502,12 @@ int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const i
     j.lbound = (uint16_t)layer_bound(b, Z, d.nof_llrs);
     j.gt_off = graph_offset(d.cfg.base_graph, Z);
     j.out_idx = i;
-    add_job(g, d.cfg.base_graph, j);
+    plan.jobs.push_back(j);
   }
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
-  size_t bytes = stage_groups(s, 0, g);
-  int rc = upload(ctx, s, bytes, st);
-  if (rc != LH_OK) {
-    return rc;
-  }
-  return launch_groups(ctx, s, 0, g, const_cast<int8_t*>(d_llrs), nullptr, d_bits, d_results, st);
+  return run_plan(ctx, plan, const_cast<int8_t*>(d_llrs), nullptr, d_bits, d_results, st, s, 0);
 }
 
 int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
@@ -520,7 +561,7 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
   if (n == 0) {
     return LH_OK;
   }
-  group g[2];
+  launch_plan plan;
   for (unsigned i = 0; i < n; ++i) {
     const lh_pusch_cb_desc& d = cbs[i];
     uint64_t mp = 0;
@@ -551,17 +592,12 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
     j.lbound = (uint16_t)layer_bound(b, Z, mp);
     j.gt_off = graph_offset(d.rm.base_graph, Z);
     j.out_idx = i;
-    add_job(g, d.rm.base_graph, j);
+    plan.jobs.push_back(j);
   }
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
-  size_t end = stage_groups(s, 0, g);
-  int rc = upload(ctx, s, end, st);
-  if (rc != LH_OK) {
-    return rc;
-  }
-  return launch_groups(ctx, s, 0, g, d_rm, d_cw, d_bits, d_results, st);
+  return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0);
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,

@@ -60,6 +60,27 @@ struct alignas(16) dec_job {
 };
 static_assert(sizeof(dec_job) == 112, "dec_job layout");
 
+// Work item of the persistent decoder: `ngroups` consecutive jobs with the same (BG, Z);
+// group g runs on threads [g*Z, (g+1)*Z) of the workgroup (16 bytes).
+constexpr int kMaxGroups = 64;
+struct alignas(16) dec_item {
+  uint32_t first_job;
+  uint16_t ngroups;
+  uint8_t bg;
+  uint8_t pad0;
+  uint16_t lmax;  // layer bound of the item (max over its jobs)
+  uint16_t pad1;
+  uint32_t stage; // LDS bytes per group for the codeword LLRs (0: read them from HBM)
+};
+static_assert(sizeof(dec_item) == 16, "dec_item layout");
+
+// Header of the device descriptor block of one decoder launch.
+struct alignas(16) launch_header {
+  uint32_t next_item; // work-queue counter, zero at upload
+  uint32_t n_items;
+  uint32_t pad[2];
+};
+
 inline int crc_index(uint32_t poly)
 {
   return poly == 0x1864cfbu ? 1 : poly == 0x1800063u ? 2 : poly == 0x11021u ? 3 : (poly == 0 ? 0 : -1);

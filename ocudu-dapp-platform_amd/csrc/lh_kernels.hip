@@ -86,8 +86,8 @@ __device__ __forceinline__ int v2c_of(int sv, int c)
 //    19-31 the message signs (extension layers, degree <= 13, in LDS).
 // A zero state reproduces the reference's first visit (v2c = soft).
 template <int DEG, bool RT, bool PACKED>
-__device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const uint32_t* __restrict__ gt, int deg,
-                                             unsigned r, unsigned Z, uint32_t* __restrict__ state)
+__device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff, const uint32_t* __restrict__ gt,
+                                             int deg, unsigned r, unsigned Z, uint32_t* __restrict__ state)
 {
   constexpr int NW = (DEG + 3) / 4;
   int addr[DEG];
@@ -98,7 +98,7 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const ui
       const uint32_t g = gt[e];
       unsigned j = r + (g >> 16);
       j = min(j, j - Z);
-      addr[e] = (int)((g & 0xffffu) + j);
+      addr[e] = (int)((g & 0xffffu) + j) + goff; // one three-input add
       sv[e] = soft[addr[e]];
     }
   }
@@ -170,11 +170,12 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, const ui
 }
 
 template <int BG, int ROW>
-__device__ __forceinline__ void core_layer(int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, uint32_t* cpk)
+__device__ __forceinline__ void core_layer(int8_t* soft, int goff, const uint32_t* gt, unsigned r, unsigned Z,
+                                           uint32_t* cpk)
 {
   constexpr int E0 = bgt<BG>::rs(ROW);
   constexpr int DEG = bgt<BG>::rs(ROW + 1) - E0;
-  layer_update<DEG, false, true>(soft, gt + E0, DEG, r, Z, cpk);
+  layer_update<DEG, false, true>(soft, goff, gt + E0, DEG, r, Z, cpk);
 }
 
 // Wave64 + workgroup reductions.  `red` holds 16 words.
@@ -220,7 +221,7 @@ __device__ __forceinline__ int block_reduce_max(int x, int* red)
 // Parity of every lifted check row of layers < L and the zero-soft-bit veto
 // (check_syndrome, ldpc_decoder_impl.cpp:329-387).  Nonzero when it fails.
 template <int BG>
-__device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* gt, unsigned r, unsigned Z, int L)
+__device__ uint32_t syndrome_fail(const int8_t* soft, int goff, const uint32_t* gt, unsigned r, unsigned Z, int L)
 {
   uint32_t bad = 0;
   if (r < Z) {
@@ -231,7 +232,7 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* gt, unsign
         const uint32_t g = gt[e];
         unsigned j = r + (g >> 16);
         j = min(j, j - Z);
-        const int s = soft[(g & 0xffffu) + j];
+        const int s = soft[(int)((g & 0xffffu) + j) + goff];
         bad |= (s == 0);
         p ^= (s <= 0);
       }
@@ -243,10 +244,10 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, const uint32_t* gt, unsign
 
 // Hard decisions of soft[0, nbits), MSB-first; bits of the last byte beyond nbits keep
 // their previous value.  all_ones: ones instead (no-CRC failure path).
-__device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones)
+__device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones, int first, int stride)
 {
   const int nbytes = (nbits + 7) >> 3;
-  for (int b = threadIdx.x; b < nbytes; b += blockDim.x) {
+  for (int b = first; b < nbytes; b += stride) {
     uint32_t byte = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -338,41 +339,114 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
   }
 }
 
-// Layered normalized min-sum decoder, behaviour of ldpc_decoder_impl::decode
-// (lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145) with the arithmetic
-// of ldpc_decoder_generic.cpp:12-104.  With kFlagFused the workgroup first rate-
-// dematches its codeblock (writing the HARQ buffer when asked) instead of reading a
-// dematched buffer.
+// Per-group reductions of the persistent decoder: LDS atomics, with a wave-level
+// shuffle reduction first when the whole wave belongs to one group.
+__device__ __forceinline__ bool wave_in_one_group(int g)
+{
+  return __shfl(g, 0) == __shfl(g, 63);
+}
+
+__device__ __forceinline__ void group_xor_or(uint32_t x, uint32_t z, int g, bool active, uint32_t* xa, uint32_t* za)
+{
+  if (wave_in_one_group(g)) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      x ^= (uint32_t)__shfl_xor((int)x, o);
+      z |= (uint32_t)__shfl_xor((int)z, o);
+    }
+    if ((threadIdx.x & 63) == 0 && active) {
+      atomicXor(&xa[g], x);
+      atomicOr(&za[g], z);
+    }
+  } else if (active) {
+    if (x) {
+      atomicXor(&xa[g], x);
+    }
+    if (z) {
+      atomicOr(&za[g], z);
+    }
+  }
+}
+
+__device__ __forceinline__ void group_max(int v, int g, bool active, int* ma)
+{
+  if (wave_in_one_group(g)) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      v = max(v, __shfl_xor(v, o));
+    }
+    if ((threadIdx.x & 63) == 0 && active) {
+      atomicMax(&ma[g], v);
+    }
+  } else if (active) {
+    atomicMax(&ma[g], v);
+  }
+}
+
+// Per-group control words in LDS (structure of arrays, kMaxGroups each).
+struct grp_ctl {
+  int last[kMaxGroups];        // last nonzero soft bit
+  uint32_t crc[2][kMaxGroups]; // CRC remainder accumulators (iteration parity)
+  uint32_t zf[2][kMaxGroups];  // zero-soft-bit / syndrome flags (iteration parity)
+  int ndone;                   // groups finished
+  int lmax_item;               // largest layer count over the groups
+};
+
+constexpr int kCtlBytes = (int)sizeof(grp_ctl);
+
+// Decodes one work item (G codeblocks of one (BG, Z)), behaviour of
+// ldpc_decoder_impl::decode (lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145)
+// with the arithmetic of ldpc_decoder_generic.cpp:12-104 for every codeblock.  With
+// kFlagFused the group first rate-dematches its codeblock (writing the HARQ buffer when
+// asked) instead of reading a dematched buffer.  Thread t works on row t mod Z of group
+// t / Z; the threads beyond G*Z only take part in barriers.
 template <int BG>
-__global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __restrict__ jobs,
-                                                             int8_t* __restrict__ rm_base,
-                                                             const int8_t* __restrict__ cw_base,
-                                                             uint8_t* __restrict__ bits_base,
-                                                             int32_t* __restrict__ results,
-                                                             const uint32_t* __restrict__ crc_x,
-                                                             const uint32_t* __restrict__ gtab, int lmax,
-                                                             int stage_cap)
+__device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs, int8_t* __restrict__ rm_base,
+                            const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
+                            int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x,
+                            const uint32_t* __restrict__ gtab, uint8_t* lds)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
-  const dec_job jb = jobs[blockIdx.x];
-  const unsigned Z = jb.Z;
-  const unsigned r = threadIdx.x, NT = blockDim.x;
+  const int G = item.ngroups;
+  const int lmax = item.lmax;
+  const int t = (int)threadIdx.x;
+  const int Z = jobs[item.first_job].Z;
+  const int g = t / Z;
+  const bool active = g < G;
+  const int gg = active ? g : 0;
+  const unsigned r = (unsigned)(t - g * Z);
+  const dec_job jb = jobs[item.first_job + gg];
   const uint32_t* __restrict__ gt = gtab + jb.gt_off;
 
-  extern __shared__ __align__(16) uint8_t lds[];
-  const int soft_bytes = ((K + lmax) * (int)Z + 15) & ~15;
-  int8_t* soft = reinterpret_cast<int8_t*>(lds);
-  uint32_t* ext = reinterpret_cast<uint32_t*>(lds + soft_bytes);
-  uint32_t* red = ext + (lmax - 4) * NT;
-  int8_t* stage = reinterpret_cast<int8_t*>(red + 16);
+  // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
+  const int soft_stride = ((K + lmax) * Z + 15) & ~15;
+  const int ext_words = (lmax - 4) * Z;
+  const int stage_stride = (int)item.stage;
+  int8_t* const soft0 = reinterpret_cast<int8_t*>(lds); // uniform base of the soft bits
+  const int goff = gg * soft_stride;                      // this group's soft bits
+  int8_t* soft = soft0 + goff;
+  uint32_t* ext = reinterpret_cast<uint32_t*>(lds + G * soft_stride) + gg * ext_words;
+  int8_t* stage = reinterpret_cast<int8_t*>(lds + G * soft_stride + 4 * G * ext_words) + gg * stage_stride;
+  grp_ctl* ctl = reinterpret_cast<grp_ctl*>(lds + G * soft_stride + 4 * G * ext_words + G * stage_stride);
+
+  if (t < G) {
+    ctl->last[t] = -1;
+    ctl->crc[0][t] = ctl->crc[1][t] = 0;
+    ctl->zf[0][t] = ctl->zf[1][t] = 0;
+  }
+  if (t == 0) {
+    ctl->ndone = 0;
+    ctl->lmax_item = 4;
+  }
+  __syncthreads();
 
   // Soft-bit load (load_soft_bits, ldpc_decoder_impl.cpp:147-188): two punctured nodes at
   // zero, full nodes clamped to [-64, 64], a partial last node unclamped; track the last
   // nonzero input (:67-71).
   const int n_in = (int)jb.n_in;
-  const int full = (n_in / (int)Z) * (int)Z;
-  const int cap = (K + lmax - 2) * (int)Z;
+  const int full = (n_in / Z) * Z;
+  const int cap = (K + lmax - 2) * Z;
   int last = -1;
   if (jb.flags & kFlagFused) {
     const rm_params rp = jb.rm;
@@ -382,13 +456,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
     const bool wr = jb.flags & kFlagHarqWrite;
     // Stage the E codeword LLRs in LDS with aligned 16-byte loads (a 16-byte block that
     // holds a byte of the codeblock lies in that byte's page).
-    const int8_t* src = in;
-    if (stage_cap >= (int)rp.E) {
+    const bool staged = stage_stride > 0;
+    if (staged && active) {
       const uintptr_t a = reinterpret_cast<uintptr_t>(in);
       const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
       const int lead = (int)(a & 15);
       const int nblk = (lead + (int)rp.E + 15) >> 4;
-      for (int b = (int)r; b < nblk; b += (int)NT) {
+      for (int b = (int)r; b < nblk; b += Z) {
         const uint4 w = blk[b];
         const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -399,22 +473,25 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
           }
         }
       }
+    }
+    if (staged) {
       __syncthreads();
-      src = stage;
     }
-    if (src == stage) {
-      fused_positions(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, NT, last);
-    } else {
-      fused_positions(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, NT, last);
+    if (active) {
+      if (staged) {
+        fused_positions(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
+      } else {
+        fused_positions(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
+      }
     }
-  } else {
+  } else if (active) {
     // Aligned 16-byte loads of the dematched soft bits.
     const int8_t* in = rm_base + jb.in_off;
     const uintptr_t a = reinterpret_cast<uintptr_t>(in);
     const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
     const int lead = (int)(a & 15);
     const int nblk = (lead + n_in + 15) >> 4;
-    for (int b = (int)r; b < nblk; b += (int)NT) {
+    for (int b = (int)r; b < nblk; b += Z) {
       const uint4 w = blk[b];
       const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -430,39 +507,59 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
       }
     }
   }
-  for (int i = (int)r; i < 2 * (int)Z; i += (int)NT) {
-    soft[i] = 0;
+  if (active) {
+    for (int i = (int)r; i < 2 * Z; i += Z) {
+      soft[i] = 0;
+    }
+    for (int i = n_in + (int)r; i < cap; i += Z) {
+      soft[2 * Z + i] = 0;
+    }
   }
-  for (int i = n_in + (int)r; i < cap; i += (int)NT) {
-    soft[2 * Z + i] = 0;
-  }
-  const int input_size = block_reduce_max(last, reinterpret_cast<int*>(red)) + 1;
+  group_max(last, g, active, ctl->last);
+  __syncthreads();
+  const int input_size = ctl->last[gg] + 1;
 
-  const int msg = K * (int)Z;
+  // Per-group decode state.
+  const int msg = K * Z;
   uint8_t* out = bits_base + jb.bits_off;
-  if (input_size < msg && !(jb.flags & kFlagForce)) {
-    // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
-    if (jb.crc_idx == 0 || (jb.flags & kFlagCrcAfter)) {
-      write_hard_bits(soft, out, msg, true);
+  const bool use_crc = jb.crc_idx != 0;
+  const bool es_crc = use_crc && !(jb.flags & kFlagCrcAfter);
+  const bool es_syn = !use_crc && (jb.flags & kFlagEsSyndrome);
+  const int max_iter = jb.max_iter;
+  int result = 0;
+  bool done = !active;
+  int L = 4;
+  if (active) {
+    if (input_size < msg && !(jb.flags & kFlagForce)) {
+      // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
+      if (jb.crc_idx == 0 || (jb.flags & kFlagCrcAfter)) {
+        write_hard_bits(soft, out, msg, true, (int)r, Z);
+      }
+      done = true;
+      result = 0;
+    } else {
+      int cbl = max(input_size + 2 * Z, msg + 4 * Z);
+      cbl = (cbl + Z - 1) / Z * Z;
+      L = cbl / Z - K;
+      if (L > lmax) { // host bound violated: refuse rather than overrun LDS
+        done = true;
+        result = -1;
+      }
     }
-    if (r == 0) {
-      results[jb.out_idx] = 0;
-    }
-    return;
   }
-  int cbl = max(input_size + 2 * (int)Z, msg + 4 * (int)Z);
-  cbl = (cbl + (int)Z - 1) / (int)Z * (int)Z;
-  const int L = cbl / (int)Z - K;
-  if (L > lmax) { // host bound violated: refuse rather than overrun LDS
-    if (r == 0) {
-      results[jb.out_idx] = -1;
+  if (active && !done) {
+    for (int l = 4; l < L; ++l) {
+      ext[(l - 4) * Z + (int)r] = 0;
     }
-    return;
+  }
+  if (active && r == 0) {
+    if (done) {
+      atomicAdd(&ctl->ndone, 1);
+    } else {
+      atomicMax(&ctl->lmax_item, L);
+    }
   }
 
-  for (int l = 4; l < L; ++l) {
-    ext[(l - 4) * NT + r] = 0;
-  }
   // Core-layer check-to-variable messages, int8 packed four per word (VGPRs).
   constexpr int CW0 = (bgt<BG>::rs(1) - bgt<BG>::rs(0) + 3) / 4, CW1 = (bgt<BG>::rs(2) - bgt<BG>::rs(1) + 3) / 4;
   constexpr int CW2 = (bgt<BG>::rs(3) - bgt<BG>::rs(2) + 3) / 4, CW3 = (bgt<BG>::rs(4) - bgt<BG>::rs(3) + 3) / 4;
@@ -479,99 +576,151 @@ __global__ void __launch_bounds__(kMaxThreads) k_ldpc_decode(const dec_job* __re
   // CRC weights: hard bit i of the message contributes x^(len-1-i+order) mod G; thread r
   // reads the weights of bits n*Z + r (n < K) from the table when it checks the CRC.
   const int len = msg - (int)jb.F;
-  const bool use_crc = jb.crc_idx != 0;
   const int order = jb.crc_idx == 3 ? 16 : 24;
-  const uint32_t* __restrict__ xtab = crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0) + (len - 1 + order) - (int)r;
+  const uint32_t* __restrict__ xtab =
+      crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0) + (len - 1 + order) - (int)r;
   __syncthreads();
+  const int Litem = ctl->lmax_item;
 
-  int result = 0;
-  const int max_iter = jb.max_iter;
-  const bool es_crc = use_crc && !(jb.flags & kFlagCrcAfter);
-  const bool es_syn = !use_crc && (jb.flags & kFlagEsSyndrome);
-  for (int it = 0; it < max_iter; ++it) {
-    if (r < Z) {
-      core_layer<BG, 0>(soft, gt, r, Z, c0);
+  for (int it = 0;; ++it) {
+    // Uniform exit: every group finished (read after the barrier that ordered the updates).
+    if (ctl->ndone >= G) {
+      break;
+    }
+    const bool run = !done;
+    const int par = it & 1;
+    if (run) {
+      core_layer<BG, 0>(soft0, goff, gt, r, (unsigned)Z, c0);
     }
     __syncthreads();
-    if (r < Z) {
-      core_layer<BG, 1>(soft, gt, r, Z, c1);
+    if (run) {
+      core_layer<BG, 1>(soft0, goff, gt, r, (unsigned)Z, c1);
     }
     __syncthreads();
-    if (r < Z) {
-      core_layer<BG, 2>(soft, gt, r, Z, c2);
+    if (run) {
+      core_layer<BG, 2>(soft0, goff, gt, r, (unsigned)Z, c2);
     }
     __syncthreads();
-    if (r < Z) {
-      core_layer<BG, 3>(soft, gt, r, Z, c3);
+    if (run) {
+      core_layer<BG, 3>(soft0, goff, gt, r, (unsigned)Z, c3);
     }
     __syncthreads();
-    for (int l = 4; l < L; ++l) {
-      if (r < Z) {
+    for (int l = 4; l < Litem; ++l) {
+      if (run && l < L) {
         const int e0 = T::rs(l);
-        layer_update<T::DEG_EXT, true, false>(soft, gt + e0, T::rs(l + 1) - e0, r, Z, &ext[(l - 4) * NT + r]);
+        layer_update<T::DEG_EXT, true, false>(soft0, goff, gt + e0, T::rs(l + 1) - e0, r, (unsigned)Z,
+                                              &ext[(l - 4) * Z + (int)r]);
       }
       __syncthreads();
     }
 
-    if (es_crc) {
-      // get_hard_bits + CRC (ldpc_decoder_impl.cpp:112-119).
-      uint32_t x = 0, z = 0;
-      if (r < Z) {
-        int s[K];
-        uint32_t xw[K];
-        const uint32_t* xt = xtab;
-        asm volatile("" : "+v"(xt)); // re-read the weights each check (keeps them out of VGPRs)
+    // Per-iteration check: CRC (ldpc_decoder_impl.cpp:112-119), syndrome (:120-127), or
+    // the final syndrome (+ CRC for PUSCH without early stop) after the last iteration.
+    const bool last_it = it + 1 >= max_iter;
+    const bool check = run && (es_crc || es_syn || last_it);
+    uint32_t x = 0, z = 0;
+    if (check) {
+      const bool want_crc = use_crc && (es_crc || last_it);
+      const uint32_t* xt = xtab;
+      asm volatile("" : "+v"(xt)); // re-read the weights each check (keeps them out of VGPRs)
+      int s[K];
+      uint32_t xw[K];
 #pragma unroll
-        for (int n = 0; n < K; ++n) {
-          s[n] = soft[n * Z + r];
-          xw[n] = (n * (int)Z + (int)r < len) ? xt[-n * (int)Z] : 0u;
-        }
+      for (int n = 0; n < K; ++n) {
+        s[n] = soft0[goff + (int)r + n * Z];
+        xw[n] = (want_crc && n * Z + (int)r < len) ? xt[-n * Z] : 0u;
+      }
 #pragma unroll
-        for (int n = 0; n < K; ++n) {
-          z |= (s[n] == 0);
-          x ^= (s[n] <= 0) ? xw[n] : 0u;
-        }
+      for (int n = 0; n < K; ++n) {
+        z |= (s[n] == 0);
+        x ^= (s[n] <= 0) ? xw[n] : 0u;
       }
-      block_reduce_xor_or(x, z, red);
-      if (z == 0 && x == 0) {
-        result = it + 1;
-        break;
-      }
-    } else if (es_syn) {
-      uint32_t x = 0, z = 0;
-      if (r < Z) {
-        for (int n = 0; n < K; ++n) {
-          z |= (soft[n * Z + r] == 0);
-        }
-      }
-      z |= syndrome_fail<BG>(soft, gt, r, Z, L);
-      block_reduce_xor_or(x, z, red);
-      if (z == 0) {
-        result = it + 1;
-        break;
+      if (!es_crc) {
+        z |= syndrome_fail<BG>(soft0, goff, gt, r, (unsigned)Z, L);
       }
     }
+    group_xor_or(x, z, g, check, ctl->crc[par], ctl->zf[par]);
+    __syncthreads();
+    if (run) {
+      const uint32_t xr = ctl->crc[par][g], zr = ctl->zf[par][g];
+      bool fin = false;
+      if (es_crc) {
+        if (zr == 0 && xr == 0) {
+          result = it + 1;
+          fin = true;
+        }
+      } else if (es_syn) {
+        if (zr == 0) {
+          result = it + 1;
+          fin = true;
+        }
+      } else if (last_it) {
+        result = (zr == 0 && (!use_crc || xr == 0)) ? max_iter : 0;
+        fin = true;
+      }
+      if (!fin && last_it) {
+        result = 0;
+        fin = true;
+      }
+      if (fin) {
+        done = true;
+        if (r == 0) {
+          atomicAdd(&ctl->ndone, 1);
+        }
+      }
+    }
+    // Clear the other parity's accumulators for the next iteration.
+    if (t < G) {
+      ctl->crc[par ^ 1][t] = 0;
+      ctl->zf[par ^ 1][t] = 0;
+    }
+    __syncthreads();
   }
 
-  if (!es_crc && !es_syn) {
-    // No early stop: hard bits + syndrome after the last iteration (:130-144), then the
-    // codeblock CRC for the PUSCH path without early stop (pusch_codeblock_decoder.cpp:47-57).
-    uint32_t x = 0, z = 0;
-    if (r < Z) {
-      for (int n = 0; n < K; ++n) {
-        const int s = soft[n * Z + r];
-        z |= (s == 0);
-        x ^= (use_crc && s <= 0 && n * (int)Z + (int)r < len) ? xtab[-n * (int)Z] : 0u;
-      }
+  if (active) {
+    if (result >= 0 && !(input_size < msg && !(jb.flags & kFlagForce))) {
+      write_hard_bits(soft, out, msg, false, (int)r, Z);
     }
-    z |= syndrome_fail<BG>(soft, gt, r, Z, L);
-    block_reduce_xor_or(x, z, red);
-    result = (z == 0 && (!use_crc || x == 0)) ? max_iter : 0;
+    if (r == 0) {
+      results[jb.out_idx] = result;
+    }
   }
-  __syncthreads();
-  write_hard_bits(soft, out, msg, false);
-  if (r == 0) {
-    results[jb.out_idx] = result;
+  __syncthreads(); // the next item reuses the LDS
+}
+
+// Persistent decoder: workgroups pull work items from the (BG, Z)-sorted queue until it
+// is empty.  Items were sorted largest first by the host, so the dynamic pull balances
+// uneven codeblocks.
+__global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_ldpc_decode(launch_header* __restrict__ hdr,
+                                                             const dec_item* __restrict__ items,
+                                                             const dec_job* __restrict__ jobs,
+                                                             int8_t* __restrict__ rm_base,
+                                                             const int8_t* __restrict__ cw_base,
+                                                             uint8_t* __restrict__ bits_base,
+                                                             int32_t* __restrict__ results,
+                                                             const uint32_t* __restrict__ crc_x,
+                                                             const uint32_t* __restrict__ gtab)
+{
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ uint32_t item_id;
+  const uint32_t n_items = hdr->n_items;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      item_id = atomicAdd(&hdr->next_item, 1u);
+    }
+    __syncthreads();
+    const uint32_t id = item_id;
+    __syncthreads();
+    if (id >= n_items) {
+      break;
+    }
+    const dec_item item = items[id];
+    if (item.bg == 1) {
+      decode_item<1>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, lds);
+    } else {
+      decode_item<2>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, lds);
+    }
   }
 }
 
@@ -611,31 +760,32 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 }
 
 // ---- host-side launchers ------------------------------------------------------------
-size_t decode_lds_bytes(int bg, int maxZ, int lmax, int nt, int stage_cap)
+size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
 {
   const int K = bg == 1 ? 22 : 10;
-  return (size_t)(((K + lmax) * maxZ + 15) & ~15) + (size_t)(lmax - 4) * nt * 4 + 16 * 4 + (size_t)stage_cap;
+  return (size_t)G * ((((K + lmax) * Z + 15) & ~15) + 4 * (lmax - 4) * Z + stage) + kCtlBytes;
 }
 
-hipError_t launch_decode(int bg, const dec_job* d_jobs, unsigned n, int maxZ, int lmax, int stage_cap, int8_t* rm,
-                         const int8_t* cw, uint8_t* bits, int32_t* results, const uint32_t* crc_x,
-                         const uint32_t* gtab, hipStream_t s)
+hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
+                         int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
+                         const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s)
 {
-  const int nt = ((maxZ + 63) / 64) * 64;
-  const size_t lds = decode_lds_bytes(bg, maxZ, lmax, nt, stage_cap);
-  const void* fn = bg == 1 ? (const void*)k_ldpc_decode<1> : (const void*)k_ldpc_decode<2>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)k_ldpc_decode, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
     return e;
   }
-  if (bg == 1) {
-    hipLaunchKernelGGL(k_ldpc_decode<1>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
-                       lmax, stage_cap);
-  } else {
-    hipLaunchKernelGGL(k_ldpc_decode<2>, dim3(n), dim3(nt), lds, s, d_jobs, rm, cw, bits, results, crc_x, gtab,
-                       lmax, stage_cap);
-  }
+  hipLaunchKernelGGL(k_ldpc_decode, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, results,
+                     crc_x, gtab);
   return hipGetLastError();
+}
+
+int decode_occupancy(int nt, size_t lds)
+{
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_ldpc_decode, nt, lds) != hipSuccess) {
+    return 1;
+  }
+  return n;
 }
 
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,

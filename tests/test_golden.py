@@ -66,6 +66,7 @@ def test_gpu_sweep_fused():
         cw[o:o + a.size] = a
     d_cw = H.DeviceBuffer.from_array(cw)
     d_rm, d_bits, d_res = H.DeviceBuffer(rm_off), H.DeviceBuffer(bo), H.DeviceBuffer(4 * len(cases))
+    d_bits.zero()
     ctx.pusch_decode_batch(cbs, d_cw, d_rm, d_bits, d_res, nof_ldpc_iterations=6)
     res = d_res.download(np.int32, len(cases))
     rm = d_rm.download(np.int8, rm_off)

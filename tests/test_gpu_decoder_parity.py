@@ -82,6 +82,7 @@ def test_pusch_batch_fused_dematch(ctx):
     d_cw = H.DeviceBuffer.from_array(cw)
     d_rm = H.DeviceBuffer(rm_off)
     d_bits = H.DeviceBuffer(bits_off)
+    d_bits.zero()
     d_res = H.DeviceBuffer(4 * len(cbs))
     ctx.pusch_decode_batch(cbs, d_cw, d_rm, d_bits, d_res, new_data=True, use_early_stop=True,
                            nof_ldpc_iterations=6)

@@ -183,6 +183,7 @@ def test_fillers_and_harq_combining(ctx):
     d_rm = H.DeviceBuffer(rm_tot)
     bits_tot = sum(H.msg_bytes(s[0], s[1]) for s in specs)
     d_bits = H.DeviceBuffer(bits_tot)
+    d_bits.zero()
     d_res = H.DeviceBuffer(4 * n)
     harq_exp = [None] * n
     for rv, new_data in ((0, True), (2, False)):
@@ -237,6 +238,7 @@ def test_pusch_without_early_stop(ctx):
         cw[o:o + a.size] = a
     d_cw = H.DeviceBuffer.from_array(cw)
     d_rm, d_bits, d_res = H.DeviceBuffer(rm_off), H.DeviceBuffer(bo), H.DeviceBuffer(16)
+    d_bits.zero()
     ctx.pusch_decode_batch(cbs, d_cw, d_rm, d_bits, d_res, new_data=True, use_early_stop=False,
                            nof_ldpc_iterations=5)
     res = d_res.download(np.int32, 4)
