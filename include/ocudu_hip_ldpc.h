@@ -124,6 +124,33 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,
                      const uint8_t* d_bits, uint32_t* d_out, void* stream);
 
+/* One transport block of lh_tb_finalize: its codeblocks are consecutive entries of the
+ * decoder outputs (first_cb .. first_cb + nof_cbs - 1), their packed message bits
+ * cb_bits_stride bytes apart. */
+typedef struct lh_tb_desc {
+  uint64_t tb_offset;       /* output: TB bytes (tbs_bits / 8) */
+  uint64_t cb_bits_offset;  /* packed bits of the first codeblock */
+  uint32_t cb_bits_stride;  /* bytes between consecutive codeblocks' bits */
+  uint32_t tbs_bits;        /* transport block size, multiple of 8 */
+  uint32_t first_cb;        /* index of the first codeblock's decoder result */
+  uint32_t nof_cbs;
+  uint32_t cb_data_bits;    /* data bits per codeblock: K_b*Z - F - CB CRC bits (C > 1) */
+  uint32_t pad;
+} lh_tb_desc;
+
+#define LH_TB_CRC_OK 1       /* TB CRC passed (C == 1: the codeblock CRC passed) */
+#define LH_TB_CB_FAILED 0    /* at least one codeblock CRC failed: TB not assembled */
+#define LH_TB_CRC_FAILED (-1)/* all codeblock CRCs passed, TB CRC failed: reset the CB CRC flags */
+
+/* TB reassembly and TB CRC: pusch_decoder_impl::join_and_notify + concatenate_codeblocks
+ * (lib/phy/upper/channel_processors/pusch/pusch_decoder_impl.cpp:612-781).  With one
+ * codeblock the TB CRC is the codeblock CRC and the first tbs_bits are copied; with
+ * several, each codeblock contributes min(remaining, cb_data_bits) bits, the 24-bit TB
+ * checksum follows the data of the last codeblock, and CRC24A of the TB must match it.
+ * d_tb_status[i] = LH_TB_CRC_OK / LH_TB_CB_FAILED / LH_TB_CRC_FAILED. */
+int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits,
+                   const int32_t* d_cb_results, uint8_t* d_tb, int32_t* d_tb_status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

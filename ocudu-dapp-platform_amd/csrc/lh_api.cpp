@@ -24,6 +24,8 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
 int decode_occupancy(int nt, size_t lds);
+hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
+                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
@@ -125,6 +127,7 @@ struct lh_ctx {
   size_t slot_bytes = 0;
   uint32_t* d_crc_x = nullptr;
   uint32_t* d_gtab = nullptr;
+  uint32_t* d_tbcrc = nullptr; // CRC24A byte table (256) + x^(2^k) mod G (32)
   slot ring[kRing];
   int next = 0;
 };
@@ -408,6 +411,39 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
       gt[graph_offset(2, Z) + k] = (uint32_t)(lh_bg2_col[k] * Z) | ((uint32_t)(lh_bg2_v[ils][k] % Z) << 16);
     }
   }
+  // CRC24A byte table and x^(2^k) mod G for the transport-block CRC.
+  std::vector<uint32_t> tbc(256 + 32);
+  for (uint32_t b = 0; b < 256; ++b) {
+    uint32_t v = b << 16;
+    for (int k = 0; k < 8; ++k) {
+      v = (v << 1) ^ ((v & 0x800000u) ? (LH_CRC24A & 0xffffffu) : 0u);
+    }
+    tbc[b] = v & 0xffffffu;
+  }
+  {
+    uint32_t v = 2; // x^1
+    for (int k = 0; k < 32; ++k) {
+      tbc[256 + k] = v;
+      // square: v = v * v mod G
+      uint32_t res = 0;
+      for (int i = 23; i >= 0; --i) {
+        res <<= 1;
+        if (res & 0x1000000u) {
+          res ^= LH_CRC24A;
+        }
+        if ((v >> i) & 1u) {
+          res ^= v;
+        }
+      }
+      v = res;
+    }
+  }
+  if (e == hipSuccess) {
+    e = hipMalloc(&c->d_tbcrc, tbc.size() * 4);
+  }
+  if (e == hipSuccess) {
+    e = hipMemcpy(c->d_tbcrc, tbc.data(), tbc.size() * 4, hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) {
     e = hipMalloc(&c->d_gtab, gt.size() * 4);
   }
@@ -454,6 +490,9 @@ int lh_ctx_destroy(lh_ctx* c)
   }
   if (c->d_gtab) {
     (void)hipFree(c->d_gtab);
+  }
+  if (c->d_tbcrc) {
+    (void)hipFree(c->d_tbcrc);
   }
   delete c;
   return LH_OK;
@@ -598,6 +637,44 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0);
+}
+
+int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits, const int32_t* d_cb_results,
+                   uint8_t* d_tb, int32_t* d_tb_status, void* stream)
+{
+  static_assert(sizeof(lh_tb_desc) == sizeof(lh_tb_desc_dev), "TB descriptor layout");
+  if (ctx == nullptr ||
+      (n > 0 && (tbs == nullptr || d_cb_bits == nullptr || d_cb_results == nullptr || d_tb == nullptr ||
+                 d_tb_status == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_tb_finalize");
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u transport blocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_tb_desc& d = tbs[i];
+    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || d.nof_cbs == 0) {
+      return fail(LH_ERR_INVALID, "tb %u: invalid size", i);
+    }
+    if (d.nof_cbs > 1 && (d.cb_data_bits == 0 || (uint64_t)d.cb_data_bits * d.nof_cbs < (uint64_t)d.tbs_bits + 24 ||
+                          (uint64_t)d.cb_data_bits * (d.nof_cbs - 1) > d.tbs_bits)) {
+      return fail(LH_ERR_INVALID, "tb %u: codeblock data bits do not cover the TB and its CRC", i);
+    }
+  }
+  if (n == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  std::memcpy(s->host, tbs, n * sizeof(lh_tb_desc));
+  int rc = upload(ctx, s, n * sizeof(lh_tb_desc), st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), n, d_cb_bits, d_cb_results, d_tb,
+                                    d_tb_status, ctx->d_tbcrc, ctx->d_tbcrc + 256, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "TB finalize launch");
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,

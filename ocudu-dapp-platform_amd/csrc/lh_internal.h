@@ -81,6 +81,19 @@ struct alignas(16) launch_header {
   uint32_t pad[2];
 };
 
+// Transport-block descriptor as seen by the device (same layout as lh_tb_desc).
+struct alignas(8) lh_tb_desc_dev {
+  uint64_t tb_offset;
+  uint64_t cb_bits_offset;
+  uint32_t cb_bits_stride;
+  uint32_t tbs_bits;
+  uint32_t first_cb;
+  uint32_t nof_cbs;
+  uint32_t cb_data_bits;
+  uint32_t pad;
+};
+static_assert(sizeof(lh_tb_desc_dev) == 40, "lh_tb_desc_dev layout");
+
 inline int crc_index(uint32_t poly)
 {
   return poly == 0x1864cfbu ? 1 : poly == 0x1800063u ? 2 : poly == 0x11021u ? 3 : (poly == 0 ? 0 : -1);

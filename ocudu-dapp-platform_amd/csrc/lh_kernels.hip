@@ -759,6 +759,122 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
   }
 }
 
+// GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
+__device__ __forceinline__ uint32_t gf_mulx(uint32_t v, uint32_t poly, int order)
+{
+  v <<= 1;
+  return (v >> order) & 1u ? v ^ poly : v;
+}
+__device__ uint32_t gf_mulmod(uint32_t a, uint32_t b, uint32_t poly, int order)
+{
+  uint32_t res = 0;
+  for (int i = order - 1; i >= 0; --i) {
+    res = gf_mulx(res, poly, order);
+    if ((b >> i) & 1u) {
+      res ^= a;
+    }
+  }
+  return res;
+}
+// x^n mod G from the table of x^(2^k) mod G.
+__device__ uint32_t gf_xpow(uint32_t n, const uint32_t* pow2, uint32_t poly, int order)
+{
+  uint32_t res = 1;
+  for (int k = 0; n != 0; ++k, n >>= 1) {
+    if (n & 1u) {
+      res = gf_mulmod(res, pow2[k], poly, order);
+    }
+  }
+  return res;
+}
+
+// TB reassembly + TB CRC24A, one workgroup per transport block (pusch_decoder_impl.cpp:612-781).
+__global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
+                                                     const uint8_t* __restrict__ cb_bits,
+                                                     const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
+                                                     int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
+                                                     const uint32_t* __restrict__ pow2_24a)
+{
+  __shared__ uint32_t red[16];
+  __shared__ uint32_t lut[256];
+  __shared__ int all_ok;
+  const lh_tb_desc_dev d = tbs[blockIdx.x];
+  const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+  lut[t] = lut24a[t];
+  if (t == 0) {
+    all_ok = 1;
+  }
+  __syncthreads();
+  for (uint32_t c = (uint32_t)t; c < d.nof_cbs; c += (uint32_t)nt) {
+    if (cb_res[d.first_cb + c] <= 0) {
+      all_ok = 0;
+    }
+  }
+  __syncthreads();
+  uint8_t* out = tb_out + d.tb_offset;
+  const uint32_t nbytes = d.tbs_bits / 8;
+  const uint8_t* first = cb_bits + d.cb_bits_offset;
+  if (d.nof_cbs == 1) {
+    // One codeblock: its CRC is the TB CRC; copy the data on success.
+    if (all_ok) {
+      for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
+        out[b] = first[b];
+      }
+    }
+    if (t == 0) {
+      status[blockIdx.x] = all_ok ? 1 : 0;
+    }
+    return;
+  }
+  if (!all_ok) {
+    if (t == 0) {
+      status[blockIdx.x] = 0;
+    }
+    return;
+  }
+  // Concatenation: TB bit i comes from codeblock i / D, bit i mod D.
+  const uint32_t D = d.cb_data_bits;
+  for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = 8 * b + (uint32_t)k;
+      const uint32_t c = i / D, j = i - c * D;
+      const uint8_t* src = first + (size_t)c * d.cb_bits_stride;
+      v |= (uint32_t)((src[j >> 3] >> (7 - (j & 7))) & 1u) << (7 - k);
+    }
+    out[b] = (uint8_t)v;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // CRC24A: each thread the remainder of a chunk of bytes (table driven), moved to its
+  // place by x^(8 * bytes after the chunk), XOR-reduced.
+  const uint32_t poly = 0x1864cfbu;
+  const uint32_t chunk = (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
+  const uint32_t a0 = (uint32_t)t * chunk, a1 = min(nbytes, a0 + chunk);
+  uint32_t rem = 0;
+  for (uint32_t b = a0; b < a1; ++b) {
+    rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ out[b]) & 0xffu];
+  }
+  uint32_t x = 0, z = 0;
+  if (a0 < a1) {
+    x = gf_mulmod(rem, gf_xpow(8u * (nbytes - a1), pow2_24a, poly, 24), poly, 24);
+  }
+  block_reduce_xor_or(x, z, red);
+  if (t == 0) {
+    // The 24-bit TB checksum follows the last codeblock's data bits (:764-769).
+    const uint32_t last_c = d.nof_cbs - 1;
+    const uint32_t start = d.tbs_bits - last_c * D;
+    const uint8_t* src = first + (size_t)last_c * d.cb_bits_stride;
+    uint32_t ck = 0;
+    for (uint32_t k = 0; k < 24; ++k) {
+      const uint32_t j = start + k;
+      ck = (ck << 1) | ((src[j >> 3] >> (7 - (j & 7))) & 1u);
+    }
+    status[blockIdx.x] = (x == ck) ? 1 : -1;
+  }
+}
+
 // ---- host-side launchers ------------------------------------------------------------
 size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
 {
@@ -793,6 +909,13 @@ hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned max
 {
   const unsigned chunks = (maxN + 1023) / 1024;
   hipLaunchKernelGGL(k_rate_dematch, dim3(chunks, n), dim3(256), 0, s, d_jobs, cw, rm);
+  return hipGetLastError();
+}
+
+hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
+                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_tb_finalize, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2);
   return hipGetLastError();
 }
 

@@ -165,8 +165,14 @@ class PuschCbDesc(C.Structure):
                 ("crc_poly", C.c_uint32)]
 
 
+class TbDesc(C.Structure):
+    _fields_ = [("tb_offset", C.c_uint64), ("cb_bits_offset", C.c_uint64), ("cb_bits_stride", C.c_uint32),
+                ("tbs_bits", C.c_uint32), ("first_cb", C.c_uint32), ("nof_cbs", C.c_uint32),
+                ("cb_data_bits", C.c_uint32), ("pad", C.c_uint32)]
+
+
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
-            "lh_pusch_decode_batch", "lh_crc_calculate"]
+            "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize"]
 
 
 def lib():
@@ -185,6 +191,8 @@ def lib():
                                       C.c_void_p]
         L.lh_pusch_decode_batch.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_void_p]
+        L.lh_tb_finalize.argtypes = [C.c_void_p, C.POINTER(TbDesc), C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                        C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -230,6 +238,11 @@ class Context:
         _lh(lib().lh_pusch_decode_batch(self.handle, arr, len(arr), d_cw.ptr, d_rm.ptr, d_bits.ptr, d_results.ptr,
                                         int(new_data), int(use_early_stop), nof_ldpc_iterations,
                                         stream.handle if stream else None), "lh_pusch_decode_batch")
+
+    def tb_finalize(self, tbs, d_cb_bits, d_cb_results, d_tb, d_status, stream=None):
+        arr = (TbDesc * len(tbs))(*tbs)
+        _lh(lib().lh_tb_finalize(self.handle, arr, len(tbs), d_cb_bits.ptr, d_cb_results.ptr, d_tb.ptr, d_status.ptr,
+                                 stream.handle if stream else None), "lh_tb_finalize")
 
     def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
         n = len(offsets)

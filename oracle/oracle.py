@@ -107,6 +107,10 @@ def ref():
         R.ref_llr_promotion_sum.argtypes = [C.c_int8, C.c_int8]
         R.ref_llr_quantize.restype = C.c_int8
         R.ref_llr_quantize.argtypes = [C.c_float, C.c_float]
+        R.ref_segment_tx.argtypes = [C.c_int, _u8p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                                     C.POINTER(C.c_uint), _u8p, C.c_uint]
+        R.ref_segment_rx.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                                     C.POINTER(C.c_uint)]
         R.ref_decode_throughput.restype = C.c_double
         R.ref_decode_throughput.argtypes = [C.c_int, C.c_int, _i8p, C.c_uint, C.c_uint, C.c_int, C.c_uint,
                                             C.c_uint, C.c_uint, C.c_uint32, C.c_double,
@@ -240,3 +244,60 @@ def make_codeblocks(n_cb, bg, Z, E, Qm, esn0_db, seed, rv=0, F=0, crc_poly=CRC24
         llrs[c] = awgn_llrs(e, Qm, esn0_db, int(rng.integers(0, 2 ** 63)))
         msgs[c] = pack_bits(m)
     return msgs, llrs
+
+
+SEG_META = ("Z", "segment_length", "full_length", "rm_length", "nof_filler_bits", "cw_offset", "nof_crc_bits",
+            "info_bits")
+
+
+def segment_rx(bg, tbs_bytes, Qm, nof_layers, nof_ch_symbols):
+    """Oracle RX segmentation: list of per-codeblock dicts with the SEG_META keys (info_bits 0)."""
+    seg = _Seg()
+    rc = lib().orc_segment_rx(C.byref(seg), bg, 8 * tbs_bytes, nof_ch_symbols, Qm, nof_layers)
+    if rc != 0:
+        raise ValueError("invalid segmentation")
+    out = []
+    for c in range(seg.nof_segments):
+        out.append(dict(Z=seg.Z, segment_length=seg.segment_length, full_length=seg.full_length,
+                        rm_length=seg.rm_length[c], nof_filler_bits=seg.nof_filler_bits, cw_offset=seg.cw_offset[c],
+                        nof_crc_bits=seg.nof_crc_bits, info_bits=0))
+    return out, seg
+
+
+def tb_concat_crc(cb_bits_packed, stride, tbs_bytes, seg):
+    """Oracle TB reassembly: (TB bytes, received checksum XOR computed CRC24A; 0 = TB CRC ok)."""
+    cb = np.ascontiguousarray(cb_bits_packed, dtype=np.uint8)
+    tb = np.zeros(tbs_bytes, np.uint8)
+    r = lib().orc_tb_concat_crc(_p(tb, _u8p), 8 * tbs_bytes, _p(cb, _u8p), stride, C.byref(seg))
+    return tb, int(r)
+
+
+def _meta_dicts(meta, n):
+    m = np.ctypeslib.as_array(meta)[: 8 * n].reshape(n, 8)
+    return [dict(zip(SEG_META, (int(v) for v in row))) for row in m]
+
+
+def ref_segment_tx(bg, tb_bytes, Qm, nof_layers, nof_ch_symbols, rv=0, with_bits=True):
+    """Reference TX segmentation of a transport block: (metadata dicts, codeblock bits (C, seg_len) uint8)."""
+    R = ref()
+    tb = np.ascontiguousarray(tb_bytes, dtype=np.uint8)
+    meta = (C.c_uint * (8 * 256))()
+    kmax = 22 * 384
+    bits = np.zeros((256, kmax), np.uint8) if with_bits else None
+    n = R.ref_segment_tx(bg, _p(tb, _u8p), tb.size, Qm, nof_layers, nof_ch_symbols, rv, 256, meta,
+                         _p(bits, _u8p) if with_bits else None, kmax)
+    if n < 0:
+        raise ValueError("too many codeblocks")
+    md = _meta_dicts(meta, n)
+    if with_bits:
+        bits = bits[:n, : md[0]["segment_length"]]
+    return md, bits
+
+
+def ref_segment_rx(bg, tbs_bytes, Qm, nof_layers, nof_ch_symbols, rv=0):
+    """Reference RX segmentation metadata (info_bits 0)."""
+    meta = (C.c_uint * (8 * 256))()
+    n = ref().ref_segment_rx(bg, tbs_bytes, Qm, nof_layers, nof_ch_symbols, rv, 256, meta)
+    if n < 0:
+        raise ValueError("too many codeblocks")
+    return _meta_dicts(meta, n)

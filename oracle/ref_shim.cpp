@@ -12,6 +12,8 @@
 #include "ldpc_luts_impl.h"
 #include "ldpc_rate_dematcher_impl.h"
 #include "ldpc_rate_matcher_impl.h"
+#include "ldpc_segmenter_rx_impl.h"
+#include "ldpc_segmenter_tx_impl.h"
 #include "ocudu/adt/bit_buffer.h"
 #include "ocudu/phy/upper/log_likelihood_ratio.h"
 #include <atomic>
@@ -295,6 +297,109 @@ double ref_decode_throughput(int            kind,
   *crc_ok     = ok.load();
   *iters_sum  = its.load();
   return secs;
+}
+
+} // extern "C"
+
+namespace {
+
+// Codeblock metadata as 8 words: lifting size, segment length, full length, rm length,
+// filler bits, cw offset, CRC bits, info bits.
+void put_meta(unsigned* m, const codeblock_metadata& md, unsigned seg_len, unsigned info_bits)
+{
+  m[0] = static_cast<unsigned>(md.tb_common.lifting_size);
+  m[1] = seg_len;
+  m[2] = md.cb_specific.full_length;
+  m[3] = md.cb_specific.rm_length;
+  m[4] = md.cb_specific.nof_filler_bits;
+  m[5] = md.cb_specific.cw_offset;
+  m[6] = md.cb_specific.nof_crc_bits;
+  m[7] = info_bits;
+}
+
+segmenter_config seg_config(int bg, unsigned tbs_bytes, unsigned qm, unsigned nof_layers, unsigned nof_ch_symbols,
+                            unsigned rv)
+{
+  segmenter_config cfg;
+  cfg.transport_block_size = units::bytes(tbs_bytes);
+  cfg.base_graph           = to_bg(bg);
+  cfg.rv                   = rv;
+  cfg.mod                  = to_mod(qm);
+  cfg.Nref                 = 0;
+  cfg.nof_layers           = nof_layers;
+  cfg.nof_ch_symbols       = nof_ch_symbols;
+  return cfg;
+}
+
+} // namespace
+
+extern "C" {
+
+// ldpc_segmenter_tx_impl::new_transmission + read_codeblock on a transport block: returns
+// the number of codeblocks (or -1 when more than max_cbs), writes 8 metadata words per
+// codeblock and, when cb_bits is not null, the codeblock messages (segment length bits
+// each, one bit per byte, cb_stride bytes apart).
+int ref_segment_tx(int            bg,
+                   const uint8_t* tb,
+                   unsigned       tbs_bytes,
+                   unsigned       qm,
+                   unsigned       nof_layers,
+                   unsigned       nof_ch_symbols,
+                   unsigned       rv,
+                   unsigned       max_cbs,
+                   unsigned*      meta,
+                   uint8_t*       cb_bits,
+                   unsigned       cb_stride)
+{
+  ldpc_segmenter_tx_impl::sch_crc crcs{std::make_unique<crc_calculator_generic_impl>(crc_generator_poly::CRC16),
+                                       std::make_unique<crc_calculator_generic_impl>(crc_generator_poly::CRC24A),
+                                       std::make_unique<crc_calculator_generic_impl>(crc_generator_poly::CRC24B)};
+  ldpc_segmenter_tx_impl       seg(crcs);
+  const ldpc_segmenter_buffer& buf = seg.new_transmission(seg_config(bg, tbs_bytes, qm, nof_layers, nof_ch_symbols, rv));
+  unsigned                     C   = buf.get_nof_codeblocks();
+  if (C > max_cbs) {
+    return -1;
+  }
+  unsigned           seg_len = buf.get_segment_length().value();
+  dynamic_bit_buffer cb(seg_len);
+  for (unsigned c = 0; c != C; ++c) {
+    put_meta(meta + 8 * c, buf.get_cb_metadata(c), seg_len, buf.get_cb_info_bits(c).value());
+    if (cb_bits != nullptr) {
+      buf.read_codeblock(cb, span<const uint8_t>(tb, tbs_bytes), c);
+      for (unsigned i = 0; i != seg_len; ++i) {
+        cb_bits[static_cast<size_t>(c) * cb_stride + i] = static_cast<uint8_t>(cb.extract(i, 1));
+      }
+    }
+  }
+  return static_cast<int>(C);
+}
+
+// ldpc_segmenter_rx_impl::segment on a codeword of nof_ch_symbols * qm LLRs: returns the
+// number of codeblocks (or -1 when more than max_cbs) and 8 metadata words per codeblock
+// (the info-bits word is 0).
+int ref_segment_rx(int      bg,
+                   unsigned tbs_bytes,
+                   unsigned qm,
+                   unsigned nof_layers,
+                   unsigned nof_ch_symbols,
+                   unsigned rv,
+                   unsigned max_cbs,
+                   unsigned* meta)
+{
+  std::vector<log_likelihood_ratio> cw(static_cast<size_t>(nof_ch_symbols) * qm);
+  static_vector<described_rx_codeblock, MAX_NOF_SEGMENTS> cbs;
+  ldpc_segmenter_rx_impl seg;
+  seg.segment(cbs, cw, seg_config(bg, tbs_bytes, qm, nof_layers, nof_ch_symbols, rv));
+  if (cbs.size() > max_cbs) {
+    return -1;
+  }
+  unsigned lifting = 0;
+  for (unsigned c = 0; c != cbs.size(); ++c) {
+    const codeblock_metadata& md = cbs[c].second;
+    lifting                      = static_cast<unsigned>(md.tb_common.lifting_size);
+    put_meta(meta + 8 * c, md, (bg == 1 ? 22 : 10) * lifting, 0);
+  }
+  return static_cast<int>(cbs.size());
 }
 
 } // extern "C"
