@@ -151,6 +151,34 @@ typedef struct lh_tb_desc {
 int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits,
                    const int32_t* d_cb_results, uint8_t* d_tb, int32_t* d_tb_status, void* stream);
 
+/* One synthetic codeblock of lh_gen_codeblocks. */
+typedef struct lh_gen_desc {
+  uint64_t llr_offset;  /* its rm_length int8 LLRs in d_llrs */
+  uint64_t msg_offset;  /* its packed message (K_b*Z bits, MSB first) in d_msgs */
+  uint64_t seed_msg;    /* data bits: splitmix64 stream */
+  uint64_t seed_noise;  /* AWGN: splitmix64 stream, Box-Muller */
+  double n0;            /* noise variance per complex symbol with Es = 1 (0 = noise-free) */
+  uint32_t rm_length;   /* E, multiple of modulation_order */
+  uint32_t base_graph;
+  uint32_t lifting_size;
+  uint32_t rv;
+  uint32_t modulation_order; /* 2, 4, 6, 8 */
+  uint32_t Nref;
+  uint32_t nof_filler_bits;
+  uint32_t crc_poly;    /* CRC attached to the data bits: LH_CRC24A/B, LH_CRC16, or 0 */
+} lh_gen_desc;
+
+/* Device data generator: the transmit chain the reference's chain test builds from
+ * ldpc_encoder::encode (include/ocudu/phy/upper/channel_coding/ldpc/ldpc_encoder.h:41),
+ * ldpc_rate_matcher::rate_match (ldpc_rate_matcher.h:29), modulation_mapper::modulate
+ * (include/ocudu/phy/upper/channel_modulation/modulation_mapper.h:34), an AWGN channel and
+ * demodulation_mapper::demodulate_soft (demodulation_mapper.h:48) with the reference's
+ * quantiser.  Codeblock i: message = data bits (seed_msg) + CRC + F zero fillers, written
+ * packed to d_msgs + msg_offset (when d_msgs is not NULL); LLRs of its rate-matched,
+ * mapped, noisy, max-log demapped codeword to d_llrs + llr_offset. */
+int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t* d_llrs, uint8_t* d_msgs,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,8 @@ hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uin
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s);
+hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
+                                 const uint32_t* gtab, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
 } // namespace lh
@@ -675,6 +677,74 @@ int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t
   hipError_t e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), n, d_cb_bits, d_cb_results, d_tb,
                                     d_tb_status, ctx->d_tbcrc, ctx->d_tbcrc + 256, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "TB finalize launch");
+}
+
+int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t* d_llrs, uint8_t* d_msgs, void* stream)
+{
+  if (ctx == nullptr || (n > 0 && (descs == nullptr || d_llrs == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_gen_codeblocks");
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u codeblocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  if (n == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  gen_job* jobs = reinterpret_cast<gen_job*>(s->host);
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_gen_desc& d = descs[i];
+    const int ci = crc_index(d.crc_poly);
+    if (ci < 0) {
+      return fail(LH_ERR_INVALID, "cb %u: unknown CRC polynomial 0x%x", i, d.crc_poly);
+    }
+    if (d.modulation_order != 2 && d.modulation_order != 4 && d.modulation_order != 6 && d.modulation_order != 8) {
+      return fail(LH_ERR_INVALID, "cb %u: the generator maps QPSK to 256QAM, not order %u", i, d.modulation_order);
+    }
+    if (!(d.n0 >= 0.0)) {
+      return fail(LH_ERR_INVALID, "cb %u: invalid noise variance", i);
+    }
+    lh_rm_desc rd{};
+    rd.cw_offset = d.llr_offset;
+    rd.rm_length = d.rm_length;
+    rd.base_graph = d.base_graph;
+    rd.lifting_size = d.lifting_size;
+    rd.rv = d.rv;
+    rd.modulation_order = d.modulation_order;
+    rd.Nref = d.Nref;
+    rd.nof_filler_bits = d.nof_filler_bits;
+    gen_job& j = jobs[i];
+    j = gen_job{};
+    uint64_t mp = 0;
+    int rc = make_rm_params(rd, true, j.rm, mp, i);
+    if (rc != LH_OK) {
+      return rc;
+    }
+    const bg_info b = bginfo(d.base_graph);
+    const unsigned order = ci == 0 ? 0 : (ci == 3 ? 16 : 24);
+    if ((uint64_t)d.nof_filler_bits + order >= (uint64_t)b.K * d.lifting_size) {
+      return fail(LH_ERR_INVALID, "cb %u: no data bits left", i);
+    }
+    j.msg_off = d.msg_offset;
+    j.seed_msg = d.seed_msg;
+    j.seed_noise = d.seed_noise;
+    j.n0 = d.n0;
+    j.gt_off = graph_offset(d.base_graph, d.lifting_size);
+    j.Z = (uint16_t)d.lifting_size;
+    j.bg = (uint8_t)d.base_graph;
+    j.crc_idx = (uint8_t)ci;
+    j.F = (uint16_t)d.nof_filler_bits;
+    j.write_msg = d_msgs != nullptr ? 1 : 0;
+  }
+  int rc = upload(ctx, s, n * sizeof(gen_job), st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = launch_gen_codeblocks(reinterpret_cast<const gen_job*>(s->dev), n, d_llrs, d_msgs, ctx->d_crc_x,
+                                       ctx->d_gtab, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "generator launch");
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,

@@ -1,0 +1,318 @@
+// Synthetic codeblock generator for gfx950: message bits, CRC, LDPC encoding, rate
+// matching, Gray QAM mapping, AWGN and max-log soft demapping to int8 LLRs, one
+// workgroup per codeblock with the whole codeword in LDS.
+//
+// It produces the inputs of the decoder benchmarks and of the learned-decoder training
+// (TS38.212 5.3.2 encoding as in lib/phy/upper/channel_coding/ldpc/ldpc_encoder_generic.cpp,
+// 5.4.2 rate matching as in ldpc_rate_matcher_impl.cpp, TS38.211 5.1 mapping as in
+// modulation_mapper_lut_impl.cpp, quantisation as log_likelihood_ratio::quantize).
+//
+// Random streams are splitmix64 in counter form: output k of the stream seeded with s
+// is mix(s + (k + 1) * 0x9e3779b97f4a7c15), so every thread draws its own outputs
+// without a sequential walk.  Message bit j is bit 63 - (j mod 64) of output j / 64;
+// Gaussian component i (I and Q interleaved) uses outputs 2i and 2i + 1 (Box-Muller).
+#include <hip/hip_runtime.h>
+
+#include "bg_tables.inc"
+#include "lh_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace lh {
+
+namespace {
+
+constexpr uint64_t kGamma = 0x9e3779b97f4a7c15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t stream_at(uint64_t seed, uint64_t k) { return mix64(seed + (k + 1) * kGamma); }
+
+__device__ __forceinline__ double unit_open(uint64_t x) { return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+__device__ __forceinline__ double normal_at(uint64_t seed, uint64_t i)
+{
+  const double u1 = unit_open(stream_at(seed, 2 * i));
+  const double u2 = unit_open(stream_at(seed, 2 * i + 1));
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// Gray PAM amplitude of one component from its bits b_0 (sign), b_1, ... (nb bits, b_k
+// = bit k of the mask).
+__device__ __forceinline__ int pam_level(uint32_t b, int nb)
+{
+  int v = 1;
+  for (int k = nb - 1; k >= 1; --k) {
+    v = (1 << (nb - k)) - (1 - 2 * (int)((b >> k) & 1u)) * v;
+  }
+  return (1 - 2 * (int)(b & 1u)) * v;
+}
+
+__device__ __forceinline__ double qam_norm(int qm)
+{
+  return qm == 2 ? 1.0 / sqrt(2.0) : qm == 4 ? 1.0 / sqrt(10.0) : qm == 6 ? 1.0 / sqrt(42.0) : 1.0 / sqrt(170.0);
+}
+
+__device__ __forceinline__ int8_t quantize(float v, float range)
+{
+  const float c = fabsf(v) > range ? copysignf(range, v) : v;
+  return (int8_t)roundf(c / range * 120.0f);
+}
+
+__device__ __forceinline__ uint32_t block_xor(uint32_t x, uint32_t* red)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    x ^= (uint32_t)__shfl_xor((int)x, o);
+  }
+  const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = x;
+  }
+  __syncthreads();
+  x = 0;
+  for (int i = 0; i < nw; ++i) {
+    x ^= red[i];
+  }
+  return x;
+}
+
+// Shift of the edge of row `row` on column `col`, or -1 (uniform scan of the row).
+__device__ __forceinline__ int edge_shift(const uint32_t* gt, int rs0, int rs1, uint32_t colZ)
+{
+  int s = -1;
+  for (int e = rs0; e < rs1; ++e) {
+    const uint32_t g = gt[e];
+    if ((g & 0xffffu) == colZ) {
+      s = (int)(g >> 16);
+    }
+  }
+  return s;
+}
+
+constexpr int kGenMaxBytes = 68 * kMaxZ; // (K_b + M) * Z of BG1 at Z = 384
+
+} // namespace
+
+template <int BG>
+__device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint8_t* __restrict__ msgs,
+                              const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab, uint8_t* cw,
+                              uint8_t* lam, uint32_t* red)
+{
+  constexpr int K = BG == 1 ? 22 : 10, M = BG == 1 ? 46 : 42;
+  const uint16_t* rs = BG == 1 ? lh_bg1_row_start : lh_bg2_row_start;
+  const int t = (int)threadIdx.x, NT = (int)blockDim.x;
+  const int Z = jb.Z;
+  const int order = jb.crc_idx == 0 ? 0 : (jb.crc_idx == 3 ? 16 : 24);
+  const int kd = K * Z - jb.F - order;
+  const uint32_t* gt = gtab + jb.gt_off;
+
+  // 1. Message: data bits, CRC, zero fillers; the parity part starts zeroed.
+  for (int w = t; 64 * w < kd; w += NT) {
+    const uint64_t x = stream_at(jb.seed_msg, (uint64_t)w);
+    const int n = min(64, kd - 64 * w);
+    for (int b = 0; b < n; ++b) {
+      cw[64 * w + b] = (uint8_t)((x >> (63 - b)) & 1u);
+    }
+  }
+  for (int j = kd + t; j < (K + M) * Z; j += NT) {
+    cw[j] = 0;
+  }
+  __syncthreads();
+  if (order != 0) {
+    const uint32_t* xt = crc_x + (jb.crc_idx - 1) * kCrcXLen + (kd - 1 + order);
+    uint32_t x = 0;
+    for (int j = t; j < kd; j += NT) {
+      x ^= cw[j] ? xt[-j] : 0u;
+    }
+    const uint32_t crc = block_xor(x, red);
+    for (int k = t; k < order; k += NT) {
+      cw[kd + k] = (uint8_t)((crc >> (order - 1 - k)) & 1u);
+    }
+    __syncthreads();
+  }
+
+  // 2. Core parity (rows 0..3, double diagonal): lambda_i = systematic sum of row i.
+  for (int r = t; r < Z; r += NT) {
+    for (int i = 0; i < 4; ++i) {
+      uint8_t acc = 0;
+      for (int e = rs[i]; e < rs[i + 1]; ++e) {
+        const uint32_t g = gt[e];
+        const uint32_t colZ = g & 0xffffu;
+        if (colZ < (uint32_t)(K * Z)) {
+          int j = r + (int)(g >> 16);
+          j = j >= Z ? j - Z : j;
+          acc ^= cw[colZ + j];
+        }
+      }
+      lam[i * kMaxZ + r] = acc;
+    }
+  }
+  // The sum of the four core rows leaves one circulant on column K: its shift is the one
+  // that appears an odd number of times among the column-K entries.
+  int sh[4][4]; // sh[row][c] = shift of core column K + c in core row, or -1
+  for (int i = 0; i < 4; ++i) {
+    for (int c = 0; c < 4; ++c) {
+      sh[i][c] = edge_shift(gt, rs[i], rs[i + 1], (uint32_t)((K + c) * Z));
+    }
+  }
+  int tshift = -1;
+  for (int i = 0; i < 4; ++i) {
+    if (sh[i][0] < 0) {
+      continue;
+    }
+    int dup = 0;
+    for (int i2 = 0; i2 < 4; ++i2) {
+      dup += (i2 != i && sh[i2][0] == sh[i][0]) ? 1 : 0;
+    }
+    if ((dup & 1) == 0) {
+      tshift = sh[i][0];
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < Z; q += NT) {
+    int r = q - tshift;
+    r = r < 0 ? r + Z : r;
+    cw[K * Z + q] = lam[r] ^ lam[kMaxZ + r] ^ lam[2 * kMaxZ + r] ^ lam[3 * kMaxZ + r];
+  }
+  __syncthreads();
+  // Remaining core columns: repeatedly a core row with exactly one unknown column.
+  bool known[4] = {true, false, false, false};
+  for (int pass = 0; pass < 4; ++pass) {
+    for (int i = 0; i < 4; ++i) {
+      int unk = -1, nunk = 0;
+      for (int c = 0; c < 4; ++c) {
+        if (sh[i][c] >= 0 && !known[c]) {
+          unk = c;
+          ++nunk;
+        }
+      }
+      if (nunk != 1) {
+        continue;
+      }
+      for (int r = t; r < Z; r += NT) {
+        uint8_t acc = lam[i * kMaxZ + r];
+        for (int c = 0; c < 4; ++c) {
+          if (sh[i][c] >= 0 && known[c]) {
+            int j = r + sh[i][c];
+            j = j >= Z ? j - Z : j;
+            acc ^= cw[(K + c) * Z + j];
+          }
+        }
+        int j = r + sh[i][unk];
+        j = j >= Z ? j - Z : j;
+        cw[(K + unk) * Z + j] = acc;
+      }
+      known[unk] = true;
+      __syncthreads();
+    }
+  }
+
+  // 3. Extension parity (rows 4..M-1, identity diagonal): independent rows.
+  for (int idx = t; idx < (M - 4) * Z; idx += NT) {
+    const int i = 4 + idx / Z, r = idx - (i - 4) * Z;
+    uint8_t acc = 0;
+    int own = 0;
+    for (int e = rs[i]; e < rs[i + 1]; ++e) {
+      const uint32_t g = gt[e];
+      const uint32_t colZ = g & 0xffffu;
+      if (colZ == (uint32_t)((K + i) * Z)) {
+        own = (int)(g >> 16);
+        continue;
+      }
+      int j = r + (int)(g >> 16);
+      j = j >= Z ? j - Z : j;
+      acc ^= cw[colZ + j];
+    }
+    int j = r + own;
+    j = j >= Z ? j - Z : j;
+    cw[(K + i) * Z + j] = acc;
+  }
+  __syncthreads();
+
+  // 4. Packed message bits (MSB first).
+  if (jb.write_msg) {
+    uint8_t* out = msgs + jb.msg_off;
+    const int nbytes = (K * Z + 7) / 8;
+    for (int b = t; b < nbytes; b += NT) {
+      uint32_t v = 0;
+      for (int k = 0; k < 8; ++k) {
+        const int j = 8 * b + k;
+        v |= (j < K * Z ? (uint32_t)cw[j] : 0u) << (7 - k);
+      }
+      out[b] = (uint8_t)v;
+    }
+  }
+
+  // 5. Rate matching (circular buffer from k0 without fillers, bit interleaving by Qm),
+  //    mapping, AWGN, max-log demapping and quantisation, one symbol per thread step.
+  const rm_params& rp = jb.rm;
+  const int qm = rp.Qm, half = qm / 2;
+  const double nrm = qam_norm(qm);
+  const double sd = sqrt(jb.n0 / 2.0);
+  const float n0f = (float)jb.n0;
+  const float range = qm == 2 ? 24.0f : 20.0f;
+  const uint8_t* d = cw + 2 * Z;
+  int8_t* out = llrs + rp.cw_off;
+  for (uint32_t s = (uint32_t)t; s < rp.Kq; s += (uint32_t)NT) {
+    uint32_t bits = 0; // bit q = rate-matched bit q of the symbol
+    for (int q = 0; q < qm; ++q) {
+      const uint32_t i = (uint32_t)q * rp.Kq + s;
+      const uint32_t rank = (uint32_t)(((uint64_t)rp.start + i) % rp.P);
+      const uint32_t p = rank < rp.info ? rank : rank + (rp.nsys - rp.info);
+      bits |= (uint32_t)d[p] << q;
+    }
+    for (int c = 0; c < 2; ++c) {
+      uint32_t cb = 0; // component bits: I = even symbol bits, Q = odd
+      for (int k = 0; k < half; ++k) {
+        cb |= ((bits >> (2 * k + c)) & 1u) << k;
+      }
+      const float tx = (float)((double)pam_level(cb, half) * nrm);
+      const float x = (float)((double)tx + sd * normal_at(jb.seed_noise, 2 * (uint64_t)s + (uint64_t)c));
+      const double xd = (double)x;
+      for (int k = 0; k < half; ++k) {
+        double d0 = 1e30, d1 = 1e30;
+        for (int pt = 0; pt < (1 << half); ++pt) {
+          const double lev = (double)pam_level((uint32_t)pt, half) * nrm;
+          const double dd = (xd - lev) * (xd - lev);
+          if ((pt >> k) & 1) {
+            d1 = dd < d1 ? dd : d1;
+          } else {
+            d0 = dd < d0 ? dd : d0;
+          }
+        }
+        const float l = (float)((d1 - d0) / (double)n0f);
+        out[s * (uint32_t)qm + 2 * k + c] = quantize(l, range);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(384) k_gen_codeblocks(const gen_job* __restrict__ jobs, int8_t* __restrict__ llrs,
+                                                        uint8_t* __restrict__ msgs, const uint32_t* __restrict__ crc_x,
+                                                        const uint32_t* __restrict__ gtab)
+{
+  __shared__ uint8_t cw[kGenMaxBytes];
+  __shared__ uint8_t lam[4 * kMaxZ];
+  __shared__ uint32_t red[16];
+  const gen_job jb = jobs[blockIdx.x];
+  if (jb.bg == 1) {
+    gen_codeblock<1>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
+  } else {
+    gen_codeblock<2>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
+  }
+}
+
+hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
+                                 const uint32_t* gtab, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gen_codeblocks, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, llrs, msgs, crc_x, gtab);
+  return hipGetLastError();
+}
+
+} // namespace lh

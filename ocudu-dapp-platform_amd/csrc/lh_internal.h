@@ -81,6 +81,25 @@ struct alignas(16) launch_header {
   uint32_t pad[2];
 };
 
+// One synthetic codeblock of the generator (128 bytes): message bits from seed_msg,
+// noise from seed_noise, rate matching by `rm` (cw_off = LLR output offset).
+struct alignas(16) gen_job {
+  uint64_t msg_off;    // packed message bits output (K_b*Z bits)
+  uint64_t seed_msg;
+  uint64_t seed_noise;
+  double n0;           // noise variance per complex symbol (Es = 1)
+  uint32_t gt_off;
+  uint16_t Z;
+  uint8_t bg;
+  uint8_t crc_idx;     // 0 none, 1 CRC24A, 2 CRC24B, 3 CRC16
+  uint16_t F;
+  uint8_t write_msg;
+  uint8_t pad0;
+  uint32_t pad1[5];
+  rm_params rm;
+};
+static_assert(sizeof(gen_job) == 128, "gen_job layout");
+
 // Transport-block descriptor as seen by the device (same layout as lh_tb_desc).
 struct alignas(8) lh_tb_desc_dev {
   uint64_t tb_offset;

@@ -171,8 +171,15 @@ class TbDesc(C.Structure):
                 ("cb_data_bits", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class GenDesc(C.Structure):
+    _fields_ = [("llr_offset", C.c_uint64), ("msg_offset", C.c_uint64), ("seed_msg", C.c_uint64),
+                ("seed_noise", C.c_uint64), ("n0", C.c_double), ("rm_length", C.c_uint32), ("base_graph", C.c_uint32),
+                ("lifting_size", C.c_uint32), ("rv", C.c_uint32), ("modulation_order", C.c_uint32),
+                ("Nref", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("crc_poly", C.c_uint32)]
+
+
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
-            "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize"]
+            "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks"]
 
 
 def lib():
@@ -193,6 +200,8 @@ def lib():
                                             C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_void_p]
         L.lh_tb_finalize.argtypes = [C.c_void_p, C.POINTER(TbDesc), C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]
+        L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                        C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -243,6 +252,11 @@ class Context:
         arr = (TbDesc * len(tbs))(*tbs)
         _lh(lib().lh_tb_finalize(self.handle, arr, len(tbs), d_cb_bits.ptr, d_cb_results.ptr, d_tb.ptr, d_status.ptr,
                                  stream.handle if stream else None), "lh_tb_finalize")
+
+    def gen_codeblocks(self, descs, d_llrs, d_msgs=None, stream=None):
+        arr = (GenDesc * len(descs))(*descs)
+        _lh(lib().lh_gen_codeblocks(self.handle, arr, len(descs), d_llrs.ptr, d_msgs.ptr if d_msgs else None,
+                                    stream.handle if stream else None), "lh_gen_codeblocks")
 
     def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
         n = len(offsets)

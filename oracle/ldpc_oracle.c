@@ -880,6 +880,53 @@ void orc_modulate_awgn_demap(int8_t* llrs, const uint8_t* bits, unsigned nbits, 
   free(iq);
 }
 
+/* Synthetic codeblock, the CPU statement of the device generator: data bits from the
+ * splitmix64 stream seed_msg (bit j = bit 63 - j%64 of output j/64), CRC (crc_poly, or
+ * none when 0), F zero fillers, encoding, rate matching, mapping, AWGN from seed_noise,
+ * max-log demapping and quantisation.  msg_packed: (K_b*Z + 7)/8 bytes.  Returns 0. */
+int orc_gen_codeblock(int bg, unsigned Z, unsigned F, uint32_t crc_poly, unsigned E, unsigned rv, unsigned Qm,
+                      unsigned Nref, double n0, uint64_t seed_msg, uint64_t seed_noise, uint8_t* msg_packed,
+                      int8_t* llrs)
+{
+  const unsigned K = (unsigned)orc_bg_info_nodes(bg) * Z, M = (unsigned)orc_bg_check_nodes(bg) * Z;
+  const unsigned order = crc_poly == 0 ? 0u : (crc_poly == ORC_CRC16 ? 16u : 24u);
+  if (F + order >= K) {
+    return -1;
+  }
+  const unsigned kd = K - F - order;
+  uint8_t* msg = (uint8_t*)calloc(K, 1);
+  uint8_t* cw = (uint8_t*)calloc(K + M, 1);
+  uint8_t* e = (uint8_t*)malloc(E ? E : 1);
+  uint64_t st = seed_msg, w = 0;
+  for (unsigned j = 0; j < kd; ++j) {
+    if (j % 64 == 0) {
+      w = orc_rand_u64(&st);
+    }
+    msg[j] = (uint8_t)((w >> (63 - j % 64)) & 1u);
+  }
+  if (order) {
+    uint32_t c = orc_crc_unpacked(crc_poly, msg, kd);
+    for (unsigned k = 0; k < order; ++k) {
+      msg[kd + k] = (uint8_t)((c >> (order - 1 - k)) & 1u);
+    }
+  }
+  memset(msg_packed, 0, (K + 7) / 8);
+  for (unsigned j = 0; j < K; ++j) {
+    msg_packed[j / 8] |= (uint8_t)(msg[j] << (7 - j % 8));
+  }
+  int rc = orc_ldpc_encode(bg, Z, msg, cw);
+  if (rc == 0) {
+    rc = orc_rate_match(e, E, cw + 2 * Z, K + M - 2 * Z, bg, Z, rv, Qm, Nref, F);
+  }
+  if (rc == 0) {
+    orc_modulate_awgn_demap(llrs, e, E, Qm, n0, seed_noise);
+  }
+  free(msg);
+  free(cw);
+  free(e);
+  return rc;
+}
+
 /* ================================================================================= */
 /* a13: TB concatenation -- pusch_decoder_impl.cpp:734-781                             */
 /* ================================================================================= */

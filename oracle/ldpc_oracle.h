@@ -121,6 +121,11 @@ double orc_rand_normal(uint64_t* state);
 uint32_t orc_tb_concat_crc(uint8_t* tb_packed, unsigned tbs_bits, const uint8_t* cb_bits_packed,
                            unsigned cb_bits_stride_bytes, const orc_segmentation* seg);
 
+/* Synthetic codeblock (the device generator's statement): message packed, E LLRs. */
+int orc_gen_codeblock(int bg, unsigned Z, unsigned F, uint32_t crc_poly, unsigned E, unsigned rv, unsigned Qm,
+                      unsigned Nref, double n0, uint64_t seed_msg, uint64_t seed_noise, uint8_t* msg_packed,
+                      int8_t* llrs);
+
 #ifdef __cplusplus
 }
 #endif

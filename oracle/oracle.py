@@ -73,6 +73,8 @@ def lib():
         L.orc_rate_match.argtypes = [_u8p, C.c_uint, _u8p, C.c_uint, C.c_int, C.c_uint, C.c_uint, C.c_uint,
                                      C.c_uint, C.c_uint]
         L.orc_modulate_awgn_demap.argtypes = [_i8p, _u8p, C.c_uint, C.c_uint, C.c_double, C.c_uint64]
+        L.orc_gen_codeblock.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint32, C.c_uint, C.c_uint, C.c_uint,
+                                        C.c_uint, C.c_double, C.c_uint64, C.c_uint64, _u8p, _i8p]
         L.orc_tb_concat_crc.restype = C.c_uint32
         L.orc_tb_concat_crc.argtypes = [_u8p, C.c_uint, _u8p, C.c_uint, C.POINTER(_Seg)]
         _lib = L
@@ -301,3 +303,14 @@ def ref_segment_rx(bg, tbs_bytes, Qm, nof_layers, nof_ch_symbols, rv=0):
     if n < 0:
         raise ValueError("too many codeblocks")
     return _meta_dicts(meta, n)
+
+
+def gen_codeblock(bg, Z, E, Qm, n0, seed_msg, seed_noise, rv=0, F=0, crc_poly=CRC24B, Nref=0):
+    """The device generator's statement: (packed message, E int8 LLRs)."""
+    msg = np.zeros((K_b(bg) * Z + 7) // 8, np.uint8)
+    llr = np.zeros(E, np.int8)
+    rc = lib().orc_gen_codeblock(bg, Z, F, crc_poly, E, rv, Qm, Nref, n0, seed_msg, seed_noise, _p(msg, _u8p),
+                                 _p(llr, _i8p))
+    if rc != 0:
+        raise ValueError(f"generator failed ({rc})")
+    return msg, llr

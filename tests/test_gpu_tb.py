@@ -105,8 +105,8 @@ def test_tb_finalize_rejects_bad_sizes(ctx):
 def test_tb_decode_and_finalize(ctx):
     print("case tb decode + finalize", flush=True)
     # Two TBs in one decoder batch and one finalize call: 4 CBs of BG1 and 1 CB of BG2.
-    tbspec = [(1, 3752, 4, 1, 9000, 11), (2, 300, 2, 1, 2000, 12)]
-    cbs, cw_parts, tbdescs, tbs_all = [], [], [], []
+    tbspec = [(1, 3752, 4, 1, 12000, 11), (2, 300, 2, 1, 2000, 12)]
+    cbs, cw_parts, tbdescs, tbs_all, expect = [], [], [], [], []
     cw_off = rm_off = bits_off = first = tb_off = 0
     for bg, tbs, qm, nl, ns, seed in tbspec:
         tb, seg, bits = _tb(bg, tbs, qm, nl, ns, seed)
@@ -121,6 +121,8 @@ def test_tb_decode_and_finalize(ctx):
             cw = O.encode(bits[c], bg, Z)
             e = O.rate_match(cw[2 * Z:], E, bg, Z, rv=0, Qm=qm, F=F)
             llr = O.awgn_llrs(e, qm, 12.0, seed * 1000 + c)
+            rm = O.rate_dematch(llr, bg, Z, rv=0, Qm=qm, F=F)
+            expect.append(O.decode(rm, bg, Z, F=F, crc_poly=cb_poly, max_iter=6, crc_bits=md[c]["nof_crc_bits"])[0])
             cbs.append(H.PuschCbDesc(H.RmDesc(cw_off, rm_off, E, bg, Z, 0, qm, 0, F), bits_off + c * stride,
                                      md[c]["nof_crc_bits"], cb_poly))
             cw_parts.append((cw_off, llr))
@@ -145,7 +147,7 @@ def test_tb_decode_and_finalize(ctx):
     d_st = H.DeviceBuffer(4 * len(tbdescs))
     ctx.tb_finalize(tbdescs, d_bits, d_res, d_tb, d_st)
     res = d_res.download(np.int32, len(cbs))
-    assert (res > 0).all(), res
+    assert list(res) == expect and (res > 0).all(), (res, expect)
     st = d_st.download(np.int32, len(tbdescs))
     out = d_tb.download(np.uint8, tb_off)
     for i, (o, tb) in enumerate(tbs_all):
