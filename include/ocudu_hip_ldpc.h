@@ -117,6 +117,26 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
                           uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
                           unsigned nof_ldpc_iterations, void* stream);
 
+/* Learned min-sum normalisation table of one base graph (the lh_set_scale_table of the
+ * hot-path design; the reference's fixed counterpart is the 0.8 scaling factor of
+ * ldpc_decoder_impl.h:185 applied by ldpc_decoder_generic.cpp:52-61).  Entry (t, l),
+ * t < iters, l < layers (46 for BG1, 42 for BG2): the check-to-variable magnitude of
+ * layer l in iteration t is min(120, max(0, floor((alpha_q * m - beta_q + 128) / 256)))
+ * for the min-sum magnitude m, i.e. round(alpha m - beta) with alpha = alpha_q / 256
+ * (alpha_q <= 511) and beta = beta_q / 256.  Iterations from `iters` on use the last row.
+ * Do not call while a learned decode on this context is in flight. */
+int lh_set_scale_table(lh_ctx* ctx, unsigned base_graph, const uint16_t* alpha_q, const int16_t* beta_q,
+                       unsigned iters, unsigned layers);
+
+/* lh_pusch_decode_batch with the learned decoder, then the plain decoder as fallback:
+ * every codeblock is decoded with the learned tables (at most nof_ldpc_iterations); those
+ * whose CRC did not pass are decoded again by the plain (reference-exact) decoder from the
+ * soft bits the first pass wrote to their HARQ buffers (at most nof_fallback_iterations,
+ * 0 = no fallback).  d_results[i] = iterations of the pass that decoded it, or 0. */
+int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw,
+                                  int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                                  unsigned nof_ldpc_iterations, unsigned nof_fallback_iterations, void* stream);
+
 /* Device-side CRC of packed bit buffers: crc_calculator::calculate(bit_buffer)
  * (include/ocudu/phy/upper/channel_coding/crc_calculator.h:59-81, behaviour of
  * lib/phy/upper/channel_coding/crc_calculator_generic_impl.cpp:69-91).

@@ -22,8 +22,8 @@ namespace lh {
 size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage);
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-                         const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
-int decode_occupancy(int nt, size_t lds);
+                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, hipStream_t s);
+int decode_occupancy(int nt, size_t lds, bool learned);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
@@ -130,6 +130,8 @@ struct lh_ctx {
   uint32_t* d_crc_x = nullptr;
   uint32_t* d_gtab = nullptr;
   uint32_t* d_tbcrc = nullptr; // CRC24A byte table (256) + x^(2^k) mod G (32)
+  uint32_t* d_scale = nullptr; // learned normalisation tables (kScaleTableWords)
+  bool scale_set[2] = {false, false};
   slot ring[kRing];
   int next = 0;
 };
@@ -283,7 +285,7 @@ int groups_per_item(int Z, int nt)
 }
 
 int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-             hipStream_t stream, slot* s, size_t off)
+             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr)
 {
   std::vector<dec_job>& jobs = plan.jobs;
   if (jobs.empty()) {
@@ -360,12 +362,12 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
   if (rc != LH_OK) {
     return rc;
   }
-  const int per_cu = std::max(1, decode_occupancy(nt, lds));
+  const int per_cu = std::max(1, decode_occupancy(nt, lds, scale_tab != nullptr));
   const unsigned grid = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu);
   hipError_t e = launch_decode(reinterpret_cast<launch_header*>(s->dev + off),
                                reinterpret_cast<const dec_item*>(s->dev + o_items),
                                reinterpret_cast<const dec_job*>(s->dev + o_jobs), grid, nt, lds, rm, cw, bits, results,
-                               ctx->d_crc_x, ctx->d_gtab, stream);
+                               ctx->d_crc_x, ctx->d_gtab, scale_tab, stream);
   return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
@@ -447,6 +449,12 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
     e = hipMemcpy(c->d_tbcrc, tbc.data(), tbc.size() * 4, hipMemcpyHostToDevice);
   }
   if (e == hipSuccess) {
+    e = hipMalloc(&c->d_scale, kScaleTableWords * 4);
+  }
+  if (e == hipSuccess) {
+    e = hipMemset(c->d_scale, 0, kScaleTableWords * 4);
+  }
+  if (e == hipSuccess) {
     e = hipMalloc(&c->d_gtab, gt.size() * 4);
   }
   if (e == hipSuccess) {
@@ -495,6 +503,9 @@ int lh_ctx_destroy(lh_ctx* c)
   }
   if (c->d_tbcrc) {
     (void)hipFree(c->d_tbcrc);
+  }
+  if (c->d_scale) {
+    (void)hipFree(c->d_scale);
   }
   delete c;
   return LH_OK;
@@ -585,24 +596,17 @@ int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8
   return e == hipSuccess ? LH_OK : hip_fail(e, "dematcher launch");
 }
 
-int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
-                          uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
-                          unsigned nof_ldpc_iterations, void* stream)
+} // extern "C"
+
+namespace {
+
+// Decoder jobs of a PUSCH batch (fused dematching + decoding, HARQ write-back).
+int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data, int use_early_stop,
+               unsigned nof_ldpc_iterations, launch_plan& plan)
 {
-  if (ctx == nullptr ||
-      (n > 0 && (cbs == nullptr || d_cw == nullptr || d_rm == nullptr || d_bits == nullptr || d_results == nullptr))) {
-    return fail(LH_ERR_INVALID, "invalid arguments to lh_pusch_decode_batch");
-  }
-  if (n > ctx->max_cb) {
-    return fail(LH_ERR_CAPACITY, "batch of %u codeblocks exceeds the context capacity %u", n, ctx->max_cb);
-  }
   if (nof_ldpc_iterations == 0 || nof_ldpc_iterations > 255) {
     return fail(LH_ERR_INVALID, "invalid number of LDPC iterations %u", nof_ldpc_iterations);
   }
-  if (n == 0) {
-    return LH_OK;
-  }
-  launch_plan plan;
   for (unsigned i = 0; i < n; ++i) {
     const lh_pusch_cb_desc& d = cbs[i];
     uint64_t mp = 0;
@@ -635,10 +639,117 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
     j.out_idx = i;
     plan.jobs.push_back(j);
   }
+  return LH_OK;
+}
+
+int check_pusch_args(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                     uint8_t* d_bits, int32_t* d_results, const char* what)
+{
+  if (ctx == nullptr ||
+      (n > 0 && (cbs == nullptr || d_cw == nullptr || d_rm == nullptr || d_bits == nullptr || d_results == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to %s", what);
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u codeblocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  return LH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                          uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                          unsigned nof_ldpc_iterations, void* stream)
+{
+  int rc = check_pusch_args(ctx, cbs, n, d_cw, d_rm, d_bits, d_results, "lh_pusch_decode_batch");
+  if (rc != LH_OK) {
+    return rc;
+  }
+  launch_plan plan;
+  rc = pusch_jobs(cbs, n, new_data, use_early_stop, nof_ldpc_iterations, plan);
+  if (rc != LH_OK || n == 0) {
+    return rc;
+  }
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0);
+}
+
+int lh_set_scale_table(lh_ctx* ctx, unsigned base_graph, const uint16_t* alpha_q, const int16_t* beta_q,
+                       unsigned iters, unsigned layers)
+{
+  if (ctx == nullptr || alpha_q == nullptr || beta_q == nullptr || (base_graph != 1 && base_graph != 2)) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_set_scale_table");
+  }
+  const unsigned M = base_graph == 1 ? 46 : 42;
+  if (iters == 0 || iters > (unsigned)kLearnedIters || layers != M) {
+    return fail(LH_ERR_INVALID, "scale table of BG%u must be 1..%d iterations x %u layers", base_graph, kLearnedIters,
+                M);
+  }
+  std::vector<uint32_t> tab((size_t)kLearnedIters * M);
+  for (unsigned t = 0; t < (unsigned)kLearnedIters; ++t) {
+    const unsigned src = std::min(t, iters - 1);
+    for (unsigned l = 0; l < M; ++l) {
+      const unsigned a = alpha_q[src * M + l];
+      if (a > 511) {
+        return fail(LH_ERR_INVALID, "scale table entry (%u, %u): alpha above 511/256", src, l);
+      }
+      tab[t * M + l] = a | ((uint32_t)(uint16_t)beta_q[src * M + l] << 16);
+    }
+  }
+  (void)hipSetDevice(ctx->device);
+  uint32_t* dst = ctx->d_scale + (base_graph == 1 ? 0 : kLearnedIters * 46);
+  hipError_t e = hipMemcpy(dst, tab.data(), tab.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    return hip_fail(e, "scale table upload");
+  }
+  ctx->scale_set[base_graph - 1] = true;
+  return LH_OK;
+}
+
+int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw,
+                                  int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                                  unsigned nof_ldpc_iterations, unsigned nof_fallback_iterations, void* stream)
+{
+  int rc = check_pusch_args(ctx, cbs, n, d_cw, d_rm, d_bits, d_results, "lh_pusch_decode_batch_learned");
+  if (rc != LH_OK) {
+    return rc;
+  }
+  launch_plan plan;
+  rc = pusch_jobs(cbs, n, new_data, use_early_stop, nof_ldpc_iterations, plan);
+  if (rc != LH_OK || n == 0) {
+    return rc;
+  }
+  if (nof_fallback_iterations > 255) {
+    return fail(LH_ERR_INVALID, "invalid number of fallback iterations %u", nof_fallback_iterations);
+  }
+  for (unsigned i = 0; i < n; ++i) {
+    if (!ctx->scale_set[cbs[i].rm.base_graph - 1]) {
+      return fail(LH_ERR_INVALID, "no learned scale table for BG%u", cbs[i].rm.base_graph);
+    }
+  }
+  // Fallback jobs: the plain decoder on the HARQ buffers the learned pass wrote back,
+  // for the codeblocks whose CRC the learned pass did not pass.
+  launch_plan fb;
+  if (nof_fallback_iterations > 0) {
+    fb.jobs = plan.jobs;
+    for (dec_job& j : fb.jobs) {
+      j.flags = (uint8_t)((j.flags & kFlagCrcAfter) | kFlagSkipPassed);
+      j.max_iter = (uint8_t)nof_fallback_iterations;
+    }
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  rc = run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0, ctx->d_scale);
+  if (rc != LH_OK || fb.jobs.empty()) {
+    return rc;
+  }
+  s = acquire_slot(ctx);
+  return run_plan(ctx, fb, d_rm, d_cw, d_bits, d_results, st, s, 0);
 }
 
 int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits, const int32_t* d_cb_results,

@@ -20,6 +20,12 @@ constexpr uint8_t kFlagCrcAfter = 4;   // PUSCH without early stop: syndrome, th
 constexpr uint8_t kFlagNewData = 8;    // rate dematch: copy instead of combine
 constexpr uint8_t kFlagFused = 16;     // decoder dematches its own input from the codeword LLRs
 constexpr uint8_t kFlagHarqWrite = 32; // fused: write the combined soft bits back (HARQ buffer)
+constexpr uint8_t kFlagSkipPassed = 64; // fallback pass: leave codeblocks whose result is > 0
+
+// Learned normalisation tables: kLearnedIters rows of M words (A | B << 16) per base
+// graph, BG1 (46 layers) first, then BG2 (42).
+constexpr int kLearnedIters = 8;
+constexpr int kScaleTableWords = kLearnedIters * (46 + 42);
 
 // Rate-dematching parameters of one codeblock (64 bytes).
 struct alignas(16) rm_params {

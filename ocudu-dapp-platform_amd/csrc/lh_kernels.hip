@@ -67,6 +67,14 @@ __device__ __forceinline__ int med3i(int a, int lo, int hi) { return min(max(a, 
 __device__ __forceinline__ int isqrt_sq(uint32_t q) { return (int)(__fsqrt_rn((float)q) + 0.5f); }
 // round(0.8 * m) for 0 <= m <= 127 (0.8m never ends in .5).
 __device__ __forceinline__ int scale08(int m) { return (4 * m + 2) / 5; }
+// Learned normalisation and offset of one (iteration, layer): ab = A | B << 16 with
+// A = round(256 alpha) in [0, 511] and B = round(256 beta) a signed 16-bit value;
+// magnitude min(120, max(0, floor((A m - B + 128) / 256))) = round(alpha m - beta)
+// clamped to the finite LLR range.  A = 205, B = -1 gives round(0.8 m) exactly.
+__device__ __forceinline__ int learned_mag(int m, uint32_t ab)
+{
+  return med3i(((int)(ab & 0xffffu) * m - ((int)ab >> 16) + 128) >> 8, 0, 120);
+}
 
 // v2c from a soft bit and the previous check-to-variable message: saturated difference
 // for a finite soft bit; an infinite soft bit (|sv| = 127) passes through as +-(269..358),
@@ -85,9 +93,10 @@ __device__ __forceinline__ int v2c_of(int sv, int c)
 //  * otherwise one word: bits 0-6 scaled min1, 7-13 scaled min2, 14-18 argmin edge,
 //    19-31 the message signs (extension layers, degree <= 13, in LDS).
 // A zero state reproduces the reference's first visit (v2c = soft).
-template <int DEG, bool RT, bool PACKED>
+template <int DEG, bool RT, bool PACKED, bool LEARNED>
 __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff, const uint32_t* __restrict__ gt,
-                                             int deg, unsigned r, unsigned Z, uint32_t* __restrict__ state)
+                                             int deg, unsigned r, unsigned Z, uint32_t* __restrict__ state,
+                                             uint32_t ab)
 {
   constexpr int NW = (DEG + 3) / 4;
   int addr[DEG];
@@ -136,7 +145,14 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
     }
   }
   const int amin = (int)(m1 & 31u);
-  int s1 = scale08(isqrt_sq(m1 >> 5)), s2 = scale08(isqrt_sq(m2 >> 5));
+  int s1, s2;
+  if constexpr (LEARNED) {
+    s1 = learned_mag(isqrt_sq(m1 >> 5), ab);
+    s2 = learned_mag(isqrt_sq(m2 >> 5), ab);
+  } else {
+    s1 = scale08(isqrt_sq(m1 >> 5));
+    s2 = scale08(isqrt_sq(m2 >> 5));
+  }
   // Materialise the two scaled magnitudes once (keeps the per-edge selects cheap).
   asm volatile("" : "+v"(s1), "+v"(s2));
   uint32_t ns = 0;
@@ -169,13 +185,13 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
   }
 }
 
-template <int BG, int ROW>
+template <int BG, int ROW, bool LEARNED>
 __device__ __forceinline__ void core_layer(int8_t* soft, int goff, const uint32_t* gt, unsigned r, unsigned Z,
-                                           uint32_t* cpk)
+                                           uint32_t* cpk, const uint32_t* abt)
 {
   constexpr int E0 = bgt<BG>::rs(ROW);
   constexpr int DEG = bgt<BG>::rs(ROW + 1) - E0;
-  layer_update<DEG, false, true>(soft, goff, gt + E0, DEG, r, Z, cpk);
+  layer_update<DEG, false, true, LEARNED>(soft, goff, gt + E0, DEG, r, Z, cpk, LEARNED ? abt[ROW] : 0u);
 }
 
 // Wave64 + workgroup reductions.  `red` holds 16 words.
@@ -400,11 +416,11 @@ constexpr int kCtlBytes = (int)sizeof(grp_ctl);
 // kFlagFused the group first rate-dematches its codeblock (writing the HARQ buffer when
 // asked) instead of reading a dematched buffer.  Thread t works on row t mod Z of group
 // t / Z; the threads beyond G*Z only take part in barriers.
-template <int BG>
+template <int BG, bool LEARNED>
 __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs, int8_t* __restrict__ rm_base,
                             const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
                             int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x,
-                            const uint32_t* __restrict__ gtab, uint8_t* lds)
+                            const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ scale_tab, uint8_t* lds)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
@@ -418,6 +434,8 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   const unsigned r = (unsigned)(t - g * Z);
   const dec_job jb = jobs[item.first_job + gg];
   const uint32_t* __restrict__ gt = gtab + jb.gt_off;
+  // Fallback pass: codeblocks that an earlier pass decoded are left alone.
+  const bool skip = active && (jb.flags & kFlagSkipPassed) && results[jb.out_idx] > 0;
 
   // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
   const int soft_stride = ((K + lmax) * Z + 15) & ~15;
@@ -530,7 +548,9 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   bool done = !active;
   int L = 4;
   if (active) {
-    if (input_size < msg && !(jb.flags & kFlagForce)) {
+    if (skip) {
+      done = true;
+    } else if (input_size < msg && !(jb.flags & kFlagForce)) {
       // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
       if (jb.crc_idx == 0 || (jb.flags & kFlagCrcAfter)) {
         write_hard_bits(soft, out, msg, true, (int)r, Z);
@@ -589,27 +609,29 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     }
     const bool run = !done;
     const int par = it & 1;
+    // Learned tables: one row of M (A, B) words per iteration, the last row repeated.
+    const uint32_t* abt = LEARNED ? scale_tab + min(it, kLearnedIters - 1) * T::M : nullptr;
     if (run) {
-      core_layer<BG, 0>(soft0, goff, gt, r, (unsigned)Z, c0);
+      core_layer<BG, 0, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c0, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 1>(soft0, goff, gt, r, (unsigned)Z, c1);
+      core_layer<BG, 1, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c1, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 2>(soft0, goff, gt, r, (unsigned)Z, c2);
+      core_layer<BG, 2, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c2, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 3>(soft0, goff, gt, r, (unsigned)Z, c3);
+      core_layer<BG, 3, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c3, abt);
     }
     __syncthreads();
     for (int l = 4; l < Litem; ++l) {
       if (run && l < L) {
         const int e0 = T::rs(l);
-        layer_update<T::DEG_EXT, true, false>(soft0, goff, gt + e0, T::rs(l + 1) - e0, r, (unsigned)Z,
-                                              &ext[(l - 4) * Z + (int)r]);
+        layer_update<T::DEG_EXT, true, false, LEARNED>(soft0, goff, gt + e0, T::rs(l + 1) - e0, r, (unsigned)Z,
+                                                       &ext[(l - 4) * Z + (int)r], LEARNED ? abt[l] : 0u);
       }
       __syncthreads();
     }
@@ -677,7 +699,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     __syncthreads();
   }
 
-  if (active) {
+  if (active && !skip) {
     if (result >= 0 && !(input_size < msg && !(jb.flags & kFlagForce))) {
       write_hard_bits(soft, out, msg, false, (int)r, Z);
     }
@@ -691,6 +713,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
 // Persistent decoder: workgroups pull work items from the (BG, Z)-sorted queue until it
 // is empty.  Items were sorted largest first by the host, so the dynamic pull balances
 // uneven codeblocks.
+template <bool LEARNED>
 __global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_ldpc_decode(launch_header* __restrict__ hdr,
                                                              const dec_item* __restrict__ items,
@@ -700,7 +723,8 @@ k_ldpc_decode(launch_header* __restrict__ hdr,
                                                              uint8_t* __restrict__ bits_base,
                                                              int32_t* __restrict__ results,
                                                              const uint32_t* __restrict__ crc_x,
-                                                             const uint32_t* __restrict__ gtab)
+                                                             const uint32_t* __restrict__ gtab,
+                                                             const uint32_t* __restrict__ scale_tab)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t item_id;
@@ -717,9 +741,10 @@ k_ldpc_decode(launch_header* __restrict__ hdr,
     }
     const dec_item item = items[id];
     if (item.bg == 1) {
-      decode_item<1>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, lds);
+      decode_item<1, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab, lds);
     } else {
-      decode_item<2>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, lds);
+      decode_item<2, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
+                              LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, lds);
     }
   }
 }
@@ -884,21 +909,28 @@ size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
 
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-                         const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s)
+                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, hipStream_t s)
 {
-  hipError_t e = hipFuncSetAttribute((const void*)k_ldpc_decode, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void* fn = scale_tab ? (const void*)k_ldpc_decode<true> : (const void*)k_ldpc_decode<false>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
     return e;
   }
-  hipLaunchKernelGGL(k_ldpc_decode, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, results,
-                     crc_x, gtab);
+  if (scale_tab) {
+    hipLaunchKernelGGL(k_ldpc_decode<true>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
+                       results, crc_x, gtab, scale_tab);
+  } else {
+    hipLaunchKernelGGL(k_ldpc_decode<false>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
+                       results, crc_x, gtab, scale_tab);
+  }
   return hipGetLastError();
 }
 
-int decode_occupancy(int nt, size_t lds)
+int decode_occupancy(int nt, size_t lds, bool learned)
 {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_ldpc_decode, nt, lds) != hipSuccess) {
+  const void* fn = learned ? (const void*)k_ldpc_decode<true> : (const void*)k_ldpc_decode<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, lds) != hipSuccess) {
     return 1;
   }
   return n;

@@ -179,7 +179,8 @@ class GenDesc(C.Structure):
 
 
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
-            "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks"]
+            "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
+            "lh_set_scale_table", "lh_pusch_decode_batch_learned"]
 
 
 def lib():
@@ -200,6 +201,11 @@ def lib():
                                             C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_void_p]
         L.lh_tb_finalize.argtypes = [C.c_void_p, C.POINTER(TbDesc), C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]
+        L.lh_set_scale_table.argtypes = [C.c_void_p, C.c_uint, C.POINTER(C.c_uint16), C.POINTER(C.c_int16),
+                                         C.c_uint, C.c_uint]
+        L.lh_pusch_decode_batch_learned.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint,
+                                                    C.c_uint, C.c_void_p]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -247,6 +253,24 @@ class Context:
         _lh(lib().lh_pusch_decode_batch(self.handle, arr, len(arr), d_cw.ptr, d_rm.ptr, d_bits.ptr, d_results.ptr,
                                         int(new_data), int(use_early_stop), nof_ldpc_iterations,
                                         stream.handle if stream else None), "lh_pusch_decode_batch")
+
+    def set_scale_table(self, bg, alpha_q, beta_q):
+        """alpha_q, beta_q: (iters, layers) arrays in 1/256 units."""
+        a = np.ascontiguousarray(alpha_q, dtype=np.uint16)
+        b = np.ascontiguousarray(beta_q, dtype=np.int16)
+        assert a.shape == b.shape and a.ndim == 2
+        _lh(lib().lh_set_scale_table(self.handle, bg, a.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                     b.ctypes.data_as(C.POINTER(C.c_int16)), a.shape[0], a.shape[1]),
+            "lh_set_scale_table")
+
+    def pusch_decode_batch_learned(self, cbs, d_cw, d_rm, d_bits, d_results, new_data=True, use_early_stop=True,
+                                   nof_ldpc_iterations=6, nof_fallback_iterations=6, stream=None, arr=None):
+        if arr is None:
+            arr = (PuschCbDesc * len(cbs))(*cbs)
+        _lh(lib().lh_pusch_decode_batch_learned(self.handle, arr, len(arr), d_cw.ptr, d_rm.ptr, d_bits.ptr,
+                                                d_results.ptr, int(new_data), int(use_early_stop),
+                                                nof_ldpc_iterations, nof_fallback_iterations,
+                                                stream.handle if stream else None), "lh_pusch_decode_batch_learned")
 
     def tb_finalize(self, tbs, d_cb_bits, d_cb_results, d_tb, d_status, stream=None):
         arr = (TbDesc * len(tbs))(*tbs)

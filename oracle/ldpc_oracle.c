@@ -201,6 +201,8 @@ typedef struct {
   int8_t v2c[MAX_SLOTS][MAXZ];
   int8_t c2v[MAX_M][MAX_SLOTS][MAXZ];
   int c2v_init[MAX_M];
+  const uint32_t* tab; /* learned tables [8][M] (A | B << 16), or NULL for the 0.8 scaling */
+  unsigned it;         /* current iteration */
 } dec_state;
 
 /* scale_llr (ldpc_decoder_generic.cpp:52-61) with scaling factor 0.8
@@ -211,6 +213,16 @@ static int8_t scale_llr(int8_t m)
     return m;
   }
   return (int8_t)roundf((float)m * 0.8f);
+}
+
+/* Learned normalisation of a min-sum magnitude: min(120, max(0, floor((A m - B + 128) /
+ * 256))) with A = ab & 0xffff and B the signed upper half. */
+static int8_t learned_mag(int8_t m, uint32_t ab)
+{
+  int b = (int)(int16_t)(uint16_t)(ab >> 16);
+  int v = (int)(ab & 0xffffu) * m - b + 128;
+  int q = v <= 0 ? 0 : v / 256;
+  return (int8_t)(q > 120 ? 120 : q);
 }
 
 static void layer_update(dec_state* d, unsigned l)
@@ -266,7 +278,7 @@ static void layer_update(dec_state* d, unsigned l)
     for (unsigned j = 0; j < Z; ++j) {
       unsigned r = (j + Z - sh) % Z;
       int8_t m = (e != amin[r]) ? min1[r] : min2[r];
-      m = scale_llr(m);
+      m = d->tab ? learned_mag(m, d->tab[(d->it < 8 ? d->it : 7) * d->M + l]) : scale_llr(m);
       unsigned neg = sgn[r] ^ ((d->v2c[s][j] >= 0) ? 0u : 1u);
       int8_t am = (int8_t)abs(m);
       d->c2v[l][s][j] = neg ? (int8_t)-am : am;
@@ -332,13 +344,14 @@ static int check_syndrome(const dec_state* d)
   return 1;
 }
 
-unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_stop_syndrome, uint32_t crc_poly,
-                         const int8_t* in, unsigned n_in, uint8_t* out)
+static unsigned ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_stop_syndrome, uint32_t crc_poly,
+                            const int8_t* in, unsigned n_in, uint8_t* out, const uint32_t* tab)
 {
   dec_state* d = (dec_state*)calloc(1, sizeof(dec_state));
   if (!d) {
     return 0;
   }
+  d->tab = tab;
   /* init (ldpc_decoder_impl.cpp:17-39) */
   d->bg = cfg->bg;
   d->Z = cfg->Z;
@@ -393,6 +406,7 @@ unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_s
   d->L = cb_len / Z - d->K;
 
   for (unsigned it = 0; it < cfg->max_iterations; ++it) {
+    d->it = it;
     for (unsigned l = 0; l < d->L; ++l) {
       layer_update(d, l);
     }
@@ -421,6 +435,18 @@ unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_s
 done:
   free(d);
   return result;
+}
+
+unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_stop_syndrome, uint32_t crc_poly,
+                         const int8_t* in, unsigned n_in, uint8_t* out)
+{
+  return ldpc_decode(cfg, force_decoding, early_stop_syndrome, crc_poly, in, n_in, out, NULL);
+}
+
+unsigned orc_ldpc_decode_learned(const orc_dec_cfg* cfg, const uint32_t* tab, uint32_t crc_poly, const int8_t* in,
+                                 unsigned n_in, uint8_t* out)
+{
+  return ldpc_decode(cfg, 0, 0, crc_poly, in, n_in, out, tab);
 }
 
 /* ================================================================================= */

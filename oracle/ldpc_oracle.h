@@ -63,6 +63,11 @@ typedef struct {
  * If stats_iters != NULL it receives the number of iterations actually run. */
 unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_stop_syndrome, uint32_t crc_poly,
                          const int8_t* llrs, unsigned nof_llrs, uint8_t* out_packed);
+/* The same decoder with learned per-(iteration, layer) normalisation: tab holds 8 rows of
+ * M words A | B << 16, B signed (row min(it, 7) serves iteration it); the 0.8 scaling
+ * becomes min(120, max(0, floor((A m - B + 128) / 256))). */
+unsigned orc_ldpc_decode_learned(const orc_dec_cfg* cfg, const uint32_t* tab, uint32_t crc_poly, const int8_t* llrs,
+                                 unsigned nof_llrs, uint8_t* out_packed);
 
 /* ---- a3: rate dematcher ---------------------------------------------------------- */
 typedef struct {

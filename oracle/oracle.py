@@ -73,6 +73,9 @@ def lib():
         L.orc_rate_match.argtypes = [_u8p, C.c_uint, _u8p, C.c_uint, C.c_int, C.c_uint, C.c_uint, C.c_uint,
                                      C.c_uint, C.c_uint]
         L.orc_modulate_awgn_demap.argtypes = [_i8p, _u8p, C.c_uint, C.c_uint, C.c_double, C.c_uint64]
+        L.orc_ldpc_decode_learned.restype = C.c_uint
+        L.orc_ldpc_decode_learned.argtypes = [C.POINTER(_DecCfg), C.POINTER(C.c_uint32), C.c_uint32, _i8p, C.c_uint,
+                                              _u8p]
         L.orc_gen_codeblock.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint32, C.c_uint, C.c_uint, C.c_uint,
                                         C.c_uint, C.c_double, C.c_uint64, C.c_uint64, _u8p, _i8p]
         L.orc_tb_concat_crc.restype = C.c_uint32
@@ -314,3 +317,23 @@ def gen_codeblock(bg, Z, E, Qm, n0, seed_msg, seed_noise, rv=0, F=0, crc_poly=CR
     if rc != 0:
         raise ValueError(f"generator failed ({rc})")
     return msg, llr
+
+
+def scale_table_words(alpha_q, beta_q):
+    """(iters, M) alpha/beta in 1/256 units -> the 8 x M words A | B << 16 (last row repeated)."""
+    a = np.asarray(alpha_q, np.int64)
+    b = np.asarray(beta_q, np.int64)
+    rows = [np.minimum(t, a.shape[0] - 1) for t in range(8)]
+    w = (a[rows] & 0xffff) | ((b[rows] & 0xffff) << 16)
+    return np.ascontiguousarray(w.astype(np.uint32))
+
+
+def decode_learned(llrs, bg, Z, alpha_q, beta_q, F=0, crc_poly=CRC24B, max_iter=6, crc_bits=24):
+    """Learned min-sum decode (CRC early stop): (iterations or 0, packed bits)."""
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    tab = scale_table_words(alpha_q, beta_q)
+    out = np.zeros((K_b(bg) * Z + 7) // 8, np.uint8)
+    cfg = _DecCfg(bg, Z, F, crc_bits, max_iter)
+    r = lib().orc_ldpc_decode_learned(C.byref(cfg), tab.ctypes.data_as(C.POINTER(C.c_uint32)), crc_poly,
+                                      _p(llrs, _i8p), llrs.size, _p(out, _u8p))
+    return int(r), out
