@@ -137,6 +137,41 @@ int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
                                   int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
                                   unsigned nof_ldpc_iterations, unsigned nof_fallback_iterations, void* stream);
 
+/* ---- Training of the learned decoder (no counterpart in the reference: its decoders
+ * use a fixed scaling factor).  The model is the layered normalised-offset min-sum
+ * decoder unrolled over `iters` iterations in FP32, with per-(iteration, layer) scale
+ * alpha and offset beta; the loss is sum_t w[t] * mean BCE of the message bits with
+ * logits tau * soft after iteration t. ---- */
+
+/* One codeblock of a training batch (one base graph per batch). */
+typedef struct lh_train_desc {
+  uint64_t llr_offset;   /* int8 rate-dematched soft bits in d_llrs (nof_llrs of them) */
+  uint64_t msg_offset;   /* packed message bits (the targets) in d_msgs */
+  uint32_t nof_llrs;
+  uint32_t lifting_size;
+  uint32_t nof_filler_bits;
+  uint32_t nof_layers;   /* layers decoded, 4..46 (BG1) / 4..42 (BG2) */
+} lh_train_desc;
+
+/* Workspace bytes lh_nms_train_step needs for this batch. */
+size_t lh_nms_workspace_bytes(unsigned base_graph, const lh_train_desc* descs, unsigned n, unsigned iters);
+
+/* Forward (and, with backward != 0, backward) pass over a batch: d_loss[i] = loss of
+ * codeblock i; with backward, d_grad_alpha / d_grad_beta ([iters][M] floats) receive the
+ * gradient of the summed loss added to their contents.  d_alpha / d_beta: [iters][M]
+ * device floats (M = 46 / 42); w: iters host floats; d_workspace of
+ * lh_nms_workspace_bytes bytes. */
+int lh_nms_train_step(lh_ctx* ctx, unsigned base_graph, const lh_train_desc* descs, unsigned n, unsigned iters,
+                      const float* d_alpha, const float* d_beta, const float* w, float tau, const int8_t* d_llrs,
+                      const uint8_t* d_msgs, void* d_workspace, size_t workspace_bytes, float* d_loss,
+                      float* d_grad_alpha, float* d_grad_beta, int backward, void* stream);
+
+/* One Adam step on d_params (n floats: n_alpha scales, then offsets) with gradients
+ * d_grads and moments d_m, d_v; step counts from 1.  Scales are kept in [0, 511/256],
+ * offsets in [0, 64] (a non-negative offset keeps the model continuous in its inputs). */
+int lh_nms_adam(lh_ctx* ctx, float* d_params, const float* d_grads, float* d_m, float* d_v, unsigned n,
+                unsigned n_alpha, float lr, float beta1, float beta2, float eps, unsigned step, void* stream);
+
 /* Device-side CRC of packed bit buffers: crc_calculator::calculate(bit_buffer)
  * (include/ocudu/phy/upper/channel_coding/crc_calculator.h:59-81, behaviour of
  * lib/phy/upper/channel_coding/crc_calculator_generic_impl.cpp:69-91).

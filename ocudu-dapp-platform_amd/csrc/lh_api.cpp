@@ -30,6 +30,13 @@ hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned max
                                hipStream_t s);
 hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
                                  const uint32_t* gtab, hipStream_t s);
+size_t nms_train_lds_bytes(int bg, int Z, int L);
+hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt, size_t lds, int T,
+                            const float* alpha, const float* beta, const float* wt, float tau, const int8_t* in,
+                            const uint8_t* tgt, float* ws, float* loss, float* galpha, float* gbeta,
+                            const uint32_t* gtab, int backward, hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, int n_alpha, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
 } // namespace lh
@@ -856,6 +863,132 @@ int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t*
   hipError_t e = launch_gen_codeblocks(reinterpret_cast<const gen_job*>(s->dev), n, d_llrs, d_msgs, ctx->d_crc_x,
                                        ctx->d_gtab, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "generator launch");
+}
+
+} // extern "C"
+
+namespace {
+
+// Workspace floats of one training codeblock: hist[T][E_L][Z] + gl[T][K Z] + gnext[E_L][Z],
+// rounded up to 64 floats.
+size_t train_ws_floats(unsigned bg, unsigned Z, unsigned L, unsigned T)
+{
+  const uint16_t* rs = bg == 1 ? lh_bg1_row_start : lh_bg2_row_start;
+  const size_t EL = rs[L], K = bg == 1 ? 22 : 10;
+  const size_t f = (size_t)Z * (T * EL + T * K + EL);
+  return (f + 63) & ~(size_t)63;
+}
+
+int check_train_desc(unsigned bg, const lh_train_desc& d, unsigned i)
+{
+  if (lifting_index(d.lifting_size) < 0) {
+    return fail(LH_ERR_INVALID, "cb %u: invalid lifting size %u", i, d.lifting_size);
+  }
+  const bg_info b = bginfo(bg);
+  if (d.nof_layers < 4 || d.nof_layers > (unsigned)b.M) {
+    return fail(LH_ERR_INVALID, "cb %u: invalid number of layers %u", i, d.nof_layers);
+  }
+  if (d.nof_llrs > (unsigned)b.nshort * d.lifting_size) {
+    return fail(LH_ERR_INVALID, "cb %u: %u soft bits exceed the codeword", i, d.nof_llrs);
+  }
+  if (d.nof_filler_bits >= (unsigned)(b.K - 2) * d.lifting_size) {
+    return fail(LH_ERR_INVALID, "cb %u: invalid number of filler bits", i);
+  }
+  return LH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+size_t lh_nms_workspace_bytes(unsigned base_graph, const lh_train_desc* descs, unsigned n, unsigned iters)
+{
+  if ((base_graph != 1 && base_graph != 2) || descs == nullptr) {
+    return 0;
+  }
+  size_t f = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    f += train_ws_floats(base_graph, descs[i].lifting_size, descs[i].nof_layers, iters);
+  }
+  return 4 * f;
+}
+
+int lh_nms_train_step(lh_ctx* ctx, unsigned base_graph, const lh_train_desc* descs, unsigned n, unsigned iters,
+                      const float* d_alpha, const float* d_beta, const float* w, float tau, const int8_t* d_llrs,
+                      const uint8_t* d_msgs, void* d_workspace, size_t workspace_bytes, float* d_loss,
+                      float* d_grad_alpha, float* d_grad_beta, int backward, void* stream)
+{
+  if (ctx == nullptr || (base_graph != 1 && base_graph != 2) || w == nullptr ||
+      (n > 0 && (descs == nullptr || d_alpha == nullptr || d_beta == nullptr || d_llrs == nullptr ||
+                 d_msgs == nullptr || d_workspace == nullptr || d_loss == nullptr)) ||
+      (backward && (d_grad_alpha == nullptr || d_grad_beta == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_nms_train_step");
+  }
+  if (iters == 0 || iters > (unsigned)kLearnedIters) {
+    return fail(LH_ERR_INVALID, "invalid number of iterations %u (1..%d)", iters, kLearnedIters);
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u codeblocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  if (n == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  train_job* jobs = reinterpret_cast<train_job*>(s->host);
+  size_t ws = 0;
+  int maxZ = 0;
+  size_t lds = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_train_desc& d = descs[i];
+    int rc = check_train_desc(base_graph, d, i);
+    if (rc != LH_OK) {
+      return rc;
+    }
+    train_job& j = jobs[i];
+    j = train_job{};
+    j.in_off = d.llr_offset;
+    j.tgt_off = d.msg_offset;
+    j.ws_off = ws;
+    j.n_in = d.nof_llrs;
+    j.gt_off = graph_offset(base_graph, d.lifting_size);
+    j.Z = (uint16_t)d.lifting_size;
+    j.L = (uint16_t)d.nof_layers;
+    j.F = (uint16_t)d.nof_filler_bits;
+    ws += train_ws_floats(base_graph, d.lifting_size, d.nof_layers, iters);
+    maxZ = std::max<int>(maxZ, (int)d.lifting_size);
+    lds = std::max(lds, nms_train_lds_bytes((int)base_graph, (int)d.lifting_size, (int)d.nof_layers));
+  }
+  if (4 * ws > workspace_bytes) {
+    return fail(LH_ERR_CAPACITY, "workspace of %zu bytes, %zu needed", workspace_bytes, 4 * ws);
+  }
+  const size_t o_w = ((size_t)n * sizeof(train_job) + 15) & ~(size_t)15;
+  std::memcpy(s->host + o_w, w, iters * sizeof(float));
+  int rc = upload(ctx, s, o_w + iters * sizeof(float), st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  const int nt = std::max(64, ((maxZ + 63) / 64) * 64);
+  hipError_t e = launch_nms_train((int)base_graph, reinterpret_cast<const train_job*>(s->dev), n, nt, lds, (int)iters,
+                                  d_alpha, d_beta, reinterpret_cast<const float*>(s->dev + o_w), tau, d_llrs, d_msgs,
+                                  static_cast<float*>(d_workspace), d_loss, d_grad_alpha, d_grad_beta, ctx->d_gtab,
+                                  backward, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "training launch");
+}
+
+int lh_nms_adam(lh_ctx* ctx, float* d_params, const float* d_grads, float* d_m, float* d_v, unsigned n,
+                unsigned n_alpha, float lr, float beta1, float beta2, float eps, unsigned step, void* stream)
+{
+  if (ctx == nullptr || d_params == nullptr || d_grads == nullptr || d_m == nullptr || d_v == nullptr || step == 0 ||
+      n_alpha > n) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_nms_adam");
+  }
+  (void)hipSetDevice(ctx->device);
+  const float c1 = 1.0f - std::pow(beta1, (float)step), c2 = 1.0f - std::pow(beta2, (float)step);
+  hipError_t e = launch_adam(d_params, d_grads, d_m, d_v, (int)n, (int)n_alpha, lr, beta1, beta2, eps, c1, c2,
+                             reinterpret_cast<hipStream_t>(stream));
+  return e == hipSuccess ? LH_OK : hip_fail(e, "adam launch");
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,

@@ -106,6 +106,21 @@ struct alignas(16) gen_job {
 };
 static_assert(sizeof(gen_job) == 128, "gen_job layout");
 
+// One codeblock of a learned-decoder training batch (48 bytes).
+struct alignas(16) train_job {
+  uint64_t in_off;  // int8 rate-dematched soft bits (n_in of them)
+  uint64_t tgt_off; // packed message bits (the targets)
+  uint64_t ws_off;  // workspace of this codeblock, in floats
+  uint32_t n_in;
+  uint32_t gt_off;
+  uint16_t Z;
+  uint16_t L;       // layers decoded
+  uint16_t F;
+  uint16_t pad0;
+  uint32_t pad1[2];
+};
+static_assert(sizeof(train_job) == 48, "train_job layout");
+
 // Transport-block descriptor as seen by the device (same layout as lh_tb_desc).
 struct alignas(8) lh_tb_desc_dev {
   uint64_t tb_offset;

@@ -178,9 +178,15 @@ class GenDesc(C.Structure):
                 ("Nref", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("crc_poly", C.c_uint32)]
 
 
+class TrainDesc(C.Structure):
+    _fields_ = [("llr_offset", C.c_uint64), ("msg_offset", C.c_uint64), ("nof_llrs", C.c_uint32),
+                ("lifting_size", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("nof_layers", C.c_uint32)]
+
+
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
-            "lh_set_scale_table", "lh_pusch_decode_batch_learned"]
+            "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
+            "lh_nms_adam"]
 
 
 def lib():
@@ -206,6 +212,14 @@ def lib():
         L.lh_pusch_decode_batch_learned.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p,
                                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint,
                                                     C.c_uint, C.c_void_p]
+        L.lh_nms_workspace_bytes.restype = C.c_size_t
+        L.lh_nms_workspace_bytes.argtypes = [C.c_uint, C.POINTER(TrainDesc), C.c_uint, C.c_uint]
+        L.lh_nms_train_step.argtypes = [C.c_void_p, C.c_uint, C.POINTER(TrainDesc), C.c_uint, C.c_uint, C.c_void_p,
+                                        C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_void_p]
+        L.lh_nms_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
+                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint, C.c_void_p]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -281,6 +295,27 @@ class Context:
         arr = (GenDesc * len(descs))(*descs)
         _lh(lib().lh_gen_codeblocks(self.handle, arr, len(descs), d_llrs.ptr, d_msgs.ptr if d_msgs else None,
                                     stream.handle if stream else None), "lh_gen_codeblocks")
+
+    @staticmethod
+    def nms_workspace_bytes(bg, descs, iters):
+        arr = (TrainDesc * len(descs))(*descs)
+        return int(lib().lh_nms_workspace_bytes(bg, arr, len(descs), iters))
+
+    def nms_train_step(self, bg, descs, iters, d_alpha, d_beta, w, tau, d_llrs, d_msgs, d_ws, d_loss,
+                       d_galpha=None, d_gbeta=None, backward=True, stream=None, arr=None):
+        if arr is None:
+            arr = (TrainDesc * len(descs))(*descs)
+        wv = (C.c_float * iters)(*[float(x) for x in w])
+        _lh(lib().lh_nms_train_step(self.handle, bg, arr, len(arr), iters, d_alpha.ptr, d_beta.ptr, wv, tau,
+                                    d_llrs.ptr, d_msgs.ptr, d_ws.ptr, d_ws.nbytes, d_loss.ptr,
+                                    d_galpha.ptr if d_galpha else None, d_gbeta.ptr if d_gbeta else None,
+                                    int(backward), stream.handle if stream else None), "lh_nms_train_step")
+
+    def nms_adam(self, d_params, d_grads, d_m, d_v, n, n_alpha, lr, step, beta1=0.9, beta2=0.999, eps=1e-8,
+                 stream=None, offset=0):
+        _lh(lib().lh_nms_adam(self.handle, d_params.ptr + offset, d_grads.ptr + offset, d_m.ptr + offset,
+                              d_v.ptr + offset, n, n_alpha, lr, beta1, beta2, eps, step,
+                              stream.handle if stream else None), "lh_nms_adam")
 
     def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
         n = len(offsets)

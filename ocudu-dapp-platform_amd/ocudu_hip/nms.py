@@ -1,0 +1,260 @@
+"""Learned (neural) min-sum decoder: training, quantisation and evaluation.
+
+The model is the layered normalised-offset min-sum decoder with one scale alpha[t][l] and
+one offset beta[t][l] per iteration t and layer l (the reference decoders use the fixed
+0.8 scaling of lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.h:185).  Training
+runs entirely on the GPU: codeblocks come from the device generator (encoder, rate
+matcher, QAM mapper, AWGN, max-log demapper, int8 quantiser), are rate dematched by the
+device dematcher, and the FP32 forward/backward kernels (lh_nms_train_step) and the Adam
+kernel (lh_nms_adam) update the tables.  The trained tables are quantised to the int8
+decoder's format (alpha, beta in 1/256) and deployed with lh_set_scale_table; the plain
+decoder remains the fallback on CRC failure (lh_pusch_decode_batch_learned).
+
+    python -m ocudu_hip.nms train --bg 2 --Z 208 --cbs 1048576 --out tables/nms_bg2.json
+"""
+import argparse
+import json
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import (LH_CRC24B, Context, DeviceBuffer, GenDesc, PuschCbDesc, RmDesc, TrainDesc, k_b, msg_bytes, n_short)
+
+M_LAYERS = {1: 46, 2: 42}
+ITERS = 8
+TABLE_DIR = Path(__file__).resolve().parent.parent / "tables"
+
+
+class _View:
+    """A sub-range of a device buffer."""
+
+    def __init__(self, buf, off_bytes, nbytes):
+        self.ptr = buf.ptr + off_bytes
+        self.nbytes = nbytes
+
+
+def rm_length(bg, Z, rate, qm):
+    """E for code rate K_b*Z / E, rounded up to a multiple of Qm."""
+    return int(math.ceil(k_b(bg) * Z / rate / qm)) * qm
+
+
+def layers_for(bg, Z, E):
+    """Layers the decoder runs for a rv0 codeblock without fillers (ldpc_decoder_impl.cpp:85-99)."""
+    N = n_short(bg) * Z
+    nsys = (k_b(bg) - 2) * Z
+    mp = max(min(E, N), nsys)
+    cbl = max(mp + 2 * Z, (k_b(bg) + 4) * Z)
+    L = -(-cbl // Z) - k_b(bg)
+    return int(min(max(L, 4), M_LAYERS[bg]))
+
+
+def n0_of(esn0_db):
+    return float(10.0 ** (-esn0_db / 10.0))
+
+
+class DeviceBatch:
+    """Generated codeblocks of one base graph and lifting size resident on the device:
+    codeword LLRs, rate-dematched soft bits, messages, and the descriptors of the decoder
+    and of the training step."""
+
+    def __init__(self, ctx, bg, Z, cbs, seed):
+        """cbs: list of (E, Qm, Es/N0 dB)."""
+        self.bg, self.Z, self.n = bg, Z, len(cbs)
+        N = n_short(bg) * Z
+        mb = msg_bytes(bg, Z)
+        self.mb = mb
+        gen, rms, self.pusch, self.train = [], [], [], []
+        cw_off = 0
+        rs = np.random.default_rng(seed)
+        seeds = rs.integers(0, 2 ** 63, (len(cbs), 2))
+        for i, (E, qm, snr) in enumerate(cbs):
+            gen.append(GenDesc(cw_off, i * mb, int(seeds[i, 0]), int(seeds[i, 1]), n0_of(snr), E, bg, Z, 0, qm, 0, 0,
+                               LH_CRC24B))
+            rm = RmDesc(cw_off, i * N, E, bg, Z, 0, qm, 0, 0)
+            rms.append(rm)
+            self.pusch.append(PuschCbDesc(rm, i * mb, 24, LH_CRC24B))
+            self.train.append(TrainDesc(i * N, i * mb, N, Z, 0, layers_for(bg, Z, E)))
+            cw_off = (cw_off + E + 15) // 16 * 16
+        self.d_cw = DeviceBuffer(cw_off)
+        self.d_rm = DeviceBuffer(N * len(cbs))
+        self.d_msg = DeviceBuffer(mb * len(cbs))
+        self.d_bits = DeviceBuffer(mb * len(cbs))
+        self.d_res = DeviceBuffer(4 * len(cbs))
+        for c0 in range(0, len(cbs), 4096):
+            ctx.gen_codeblocks(gen[c0:c0 + 4096], self.d_cw, self.d_msg)
+            ctx.rate_dematch(rms[c0:c0 + 4096], self.d_cw, self.d_rm, new_data=True)
+        self.info_bits = np.array([k_b(bg) * Z - 24] * len(cbs))
+
+    def decode(self, ctx, learned=False, iters=ITERS, fb_iters=0, stream=None):
+        self.d_bits.zero()
+        if learned:
+            ctx.pusch_decode_batch_learned(self.pusch, self.d_cw, self.d_rm, self.d_bits, self.d_res,
+                                           nof_ldpc_iterations=iters, nof_fallback_iterations=fb_iters, stream=stream)
+        else:
+            ctx.pusch_decode_batch(self.pusch, self.d_cw, self.d_rm, self.d_bits, self.d_res,
+                                   nof_ldpc_iterations=iters, stream=stream)
+
+    def outcome(self):
+        """(block error flags, iterations) of the last decode: an error is a failed CRC or
+        a passed CRC with wrong bits."""
+        res = self.d_res.download(np.int32, self.n)
+        bits = self.d_bits.download(np.uint8, self.n * self.mb).reshape(self.n, self.mb)
+        msg = self.d_msg.download(np.uint8, self.n * self.mb).reshape(self.n, self.mb)
+        wrong = (bits != msg).any(axis=1)
+        return (res <= 0) | wrong, res
+
+
+class Trainer:
+    """Adam on the FP32 (alpha, beta) tables of one base graph."""
+
+    def __init__(self, ctx, bg, iters=ITERS, tau=0.125, lr=0.01, alpha0=0.8, beta0=0.0, weights=None):
+        self.ctx, self.bg, self.T, self.tau, self.lr = ctx, bg, iters, tau, lr
+        self.M = M_LAYERS[bg]
+        self.n = 2 * self.T * self.M
+        p = np.concatenate([np.full(self.T * self.M, alpha0), np.full(self.T * self.M, beta0)]).astype(np.float32)
+        self.d_par = DeviceBuffer.from_array(p)
+        self.d_grad, self.d_m, self.d_v = DeviceBuffer(4 * self.n), DeviceBuffer(4 * self.n), DeviceBuffer(4 * self.n)
+        self.d_m.zero()
+        self.d_v.zero()
+        self.w = list(weights) if weights is not None else [1.0] * self.T
+        self.step_no = 0
+        self.d_ws = None
+        self.d_loss = None
+
+    def _bufs(self, batch):
+        need = Context.nms_workspace_bytes(self.bg, batch.train, self.T)
+        if self.d_ws is None or self.d_ws.nbytes < need:
+            self.d_ws = DeviceBuffer(need)
+        if self.d_loss is None or self.d_loss.nbytes < 4 * batch.n:
+            self.d_loss = DeviceBuffer(4 * batch.n)
+
+    def loss(self, batch):
+        self._bufs(batch)
+        tm = 4 * self.T * self.M
+        self.ctx.nms_train_step(self.bg, batch.train, self.T, _View(self.d_par, 0, tm), _View(self.d_par, tm, tm),
+                                self.w, self.tau, batch.d_rm, batch.d_msg, self.d_ws, self.d_loss, backward=False)
+        return float(self.d_loss.download(np.float32, batch.n).sum())
+
+    def step(self, batch):
+        """One forward/backward pass over the batch and one Adam update; returns the mean loss."""
+        self._bufs(batch)
+        tm = 4 * self.T * self.M
+        self.d_grad.zero()
+        self.ctx.nms_train_step(self.bg, batch.train, self.T, _View(self.d_par, 0, tm), _View(self.d_par, tm, tm),
+                                self.w, self.tau, batch.d_rm, batch.d_msg, self.d_ws, self.d_loss,
+                                _View(self.d_grad, 0, tm), _View(self.d_grad, tm, tm), backward=True)
+        self.step_no += 1
+        self.ctx.nms_adam(self.d_par, self.d_grad, self.d_m, self.d_v, self.n, self.T * self.M, self.lr,
+                          self.step_no)
+        return float(self.d_loss.download(np.float32, batch.n).sum()) / batch.n
+
+    def tables(self):
+        p = self.d_par.download(np.float32, self.n)
+        return p[:self.T * self.M].reshape(self.T, self.M), p[self.T * self.M:].reshape(self.T, self.M)
+
+    def quantized(self):
+        a, b = self.tables()
+        aq = np.clip(np.rint(a * 256.0), 0, 511).astype(np.uint16)
+        bq = np.clip(np.rint(b * 256.0), -32768, 32767).astype(np.int16)
+        return aq, bq
+
+
+def save_table(path, bg, aq, bq, meta):
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"base_graph": bg, "iterations": int(aq.shape[0]), "layers": int(aq.shape[1]),
+                   "alpha_q": aq.astype(int).tolist(), "beta_q": bq.astype(int).tolist(), "meta": meta}, f)
+
+
+def load_table(path):
+    with open(path) as f:
+        d = json.load(f)
+    return d["base_graph"], np.array(d["alpha_q"], np.uint16), np.array(d["beta_q"], np.int16), d.get("meta", {})
+
+
+def bler(ctx, bg, Z, E, qm, snr, n, seed, learned=False, iters=ITERS, fb_iters=0):
+    """(BLER, mean iterations) of n codeblocks at one SNR."""
+    b = DeviceBatch(ctx, bg, Z, [(E, qm, snr)] * n, seed)
+    b.decode(ctx, learned=learned, iters=iters, fb_iters=fb_iters)
+    err, res = b.outcome()
+    its = np.where(res > 0, res, iters)
+    return float(err.mean()), float(its.mean())
+
+
+def calibrate(ctx, bg, Z, E, qm, n=512, seed=1, iters=ITERS, lo=-4.0, hi=30.0, targets=(0.30, 0.01)):
+    """SNRs (dB) at which the plain decoder's BLER crosses each target (bisection)."""
+    out = []
+    for tgt in targets:
+        a, b = lo, hi
+        for _ in range(14):
+            m = 0.5 * (a + b)
+            e, _ = bler(ctx, bg, Z, E, qm, m, n, seed)
+            if e > tgt:
+                a = m
+            else:
+                b = m
+        out.append(0.5 * (a + b))
+    return out
+
+
+def train(ctx, bg, Z, configs, total_cbs, batch, seed=0, iters=ITERS, lr=0.01, log=print):
+    """configs: list of (rate, Qm, snr_lo, snr_hi); each codeblock draws a configuration
+    and an SNR uniformly.  Returns the trainer."""
+    tr = Trainer(ctx, bg, iters=iters, lr=lr)
+    rng = np.random.default_rng(seed)
+    steps = max(1, total_cbs // batch)
+    t0 = time.time()
+    for s in range(steps):
+        pick = rng.integers(0, len(configs), batch)
+        cbs = []
+        for k in pick:
+            rate, qm, lo, hi = configs[k]
+            cbs.append((rm_length(bg, Z, rate, qm), qm, float(rng.uniform(lo, hi))))
+        b = DeviceBatch(ctx, bg, Z, cbs, seed=int(rng.integers(0, 2 ** 62)))
+        loss = tr.step(b)
+        if s % max(1, steps // 20) == 0 or s == steps - 1:
+            log(f"step {s + 1}/{steps} loss {loss:.4f} ({time.time() - t0:.1f} s)")
+    return tr
+
+
+def _main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--bg", type=int, default=2)
+    t.add_argument("--Z", type=int, default=208)
+    t.add_argument("--rates", default="0.3,0.5,0.7")
+    t.add_argument("--qms", default="2,4")
+    t.add_argument("--cbs", type=int, default=1 << 20)
+    t.add_argument("--batch", type=int, default=1024)
+    t.add_argument("--lr", type=float, default=0.01)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    ctx = Context(0, max_codeblocks=max(4096, args.batch))
+
+    def log(m):
+        print(m, flush=True)
+
+    rates = [float(x) for x in args.rates.split(",")]
+    qms = [int(x) for x in args.qms.split(",")]
+    configs = []
+    for r in rates:
+        for q in qms:
+            E = rm_length(args.bg, args.Z, r, q)
+            s30, s1 = calibrate(ctx, args.bg, args.Z, E, q)
+            log(f"rate {r} Qm {q} E {E}: plain BLER 30% at {s30:.2f} dB, 1% at {s1:.2f} dB")
+            configs.append((r, q, s30, s1))
+    tr = train(ctx, args.bg, args.Z, configs, args.cbs, args.batch, seed=args.seed, lr=args.lr, log=log)
+    aq, bq = tr.quantized()
+    save_table(args.out, args.bg, aq, bq, {"Z": args.Z, "configs": configs, "codeblocks": args.cbs,
+                                          "batch": args.batch, "lr": args.lr})
+    log(f"table written to {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

@@ -450,6 +450,84 @@ unsigned orc_ldpc_decode_learned(const orc_dec_cfg* cfg, const uint32_t* tab, ui
 }
 
 /* ================================================================================= */
+/* Learned min-sum decoder, floating-point training model                            */
+/* ================================================================================= */
+
+/* Forward pass of the unrolled normalised-offset min-sum decoder (the training model of
+ * the learned decoder), in REAL arithmetic: soft values start at the clamped channel
+ * values with two punctured nodes at 0; layer l of iteration t computes, per check row,
+ * v_e = s_e - c_e(t-1), the two smallest |v| (first occurrence wins ties), and
+ * c_e(t) = sgn_e * max(alpha[t][l] * mag_e - beta[t][l], 0), then s_e = v_e + c_e(t).  The
+ * loss is sum_t w[t] * mean over the message bits of softplus(-(1 - 2 b) * tau * s). */
+#define NMS_FORWARD(NAME, REAL, FABS, FMAX, LOG1P, EXP)                                                              \
+  double NAME(int bg, unsigned Z, unsigned F, unsigned L, unsigned T, const int8_t* in, unsigned n_in,               \
+              const uint8_t* msg, const double* alpha, const double* beta, const double* w, double tau)               \
+  {                                                                                                                  \
+    const unsigned K = (unsigned)orc_bg_info_nodes(bg), M = (unsigned)orc_bg_check_nodes(bg);                        \
+    const unsigned ils = (unsigned)orc_lifting_index(Z);                                                             \
+    const uint16_t* rs = bg == 1 ? lh_bg1_row_start : lh_bg2_row_start;                                             \
+    const unsigned EL = rs[L], nbits = K * Z - F;                                                                    \
+    REAL* soft = (REAL*)calloc((size_t)(K + L) * Z, sizeof(REAL));                                                   \
+    REAL* c2v = (REAL*)calloc((size_t)EL * Z, sizeof(REAL));                                                         \
+    for (unsigned i = 2 * Z; i < (K + L) * Z; ++i) {                                                                 \
+      if (i - 2 * Z < n_in) {                                                                                        \
+        int x = in[i - 2 * Z];                                                                                       \
+        soft[i] = (REAL)(x > 64 ? 64 : (x < -64 ? -64 : x));                                                         \
+      }                                                                                                              \
+    }                                                                                                                \
+    double loss = 0.0;                                                                                               \
+    for (unsigned t = 0; t < T; ++t) {                                                                               \
+      for (unsigned l = 0; l < L; ++l) {                                                                             \
+        const uint8_t* cols;                                                                                         \
+        unsigned deg, base;                                                                                          \
+        row_edges(bg, l, &cols, NULL, &deg, &base);                                                                  \
+        const REAL a = (REAL)alpha[t * M + l], b = (REAL)beta[t * M + l];                                            \
+        for (unsigned r = 0; r < Z; ++r) {                                                                           \
+          REAL v[19];                                                                                                \
+          unsigned addr[19];                                                                                         \
+          REAL m1 = (REAL)3.0e38f, m2 = (REAL)3.0e38f;                                                               \
+          unsigned a1 = 0, neg = 0;                                                                                  \
+          for (unsigned e = 0; e < deg; ++e) {                                                                       \
+            addr[e] = cols[e] * Z + (r + edge_shift(bg, ils, base + e, Z)) % Z;                                      \
+            v[e] = soft[addr[e]] - (t > 0 ? c2v[(size_t)(base + e) * Z + r] : (REAL)0);                             \
+            REAL m = FABS(v[e]);                                                                                     \
+            if (m < m1) {                                                                                            \
+              m2 = m1;                                                                                               \
+              m1 = m;                                                                                                \
+              a1 = e;                                                                                                \
+            } else if (m < m2) {                                                                                     \
+              m2 = m;                                                                                                \
+            }                                                                                                        \
+            neg ^= v[e] < 0 ? 1u : 0u;                                                                               \
+          }                                                                                                          \
+          for (unsigned e = 0; e < deg; ++e) {                                                                       \
+            REAL mag = e == a1 ? m2 : m1;                                                                            \
+            REAL c = FMAX(a * mag - b, (REAL)0);                                                                     \
+            if (neg ^ (v[e] < 0 ? 1u : 0u)) {                                                                        \
+              c = -c;                                                                                                \
+            }                                                                                                        \
+            c2v[(size_t)(base + e) * Z + r] = c;                                                                     \
+            soft[addr[e]] = v[e] + c;                                                                                \
+          }                                                                                                          \
+        }                                                                                                            \
+      }                                                                                                              \
+      double lt = 0.0;                                                                                               \
+      for (unsigned j = 0; j < nbits; ++j) {                                                                         \
+        REAL sg = ((msg[j / 8] >> (7 - j % 8)) & 1u) ? (REAL)-1 : (REAL)1;                                           \
+        REAL z = sg * soft[j] * (REAL)tau;                                                                           \
+        lt += (double)(FMAX(-z, (REAL)0) + LOG1P(EXP(-FABS(z))));                                                   \
+      }                                                                                                              \
+      loss += (double)w[t] * lt / (double)nbits;                                                                     \
+    }                                                                                                                \
+    free(soft);                                                                                                      \
+    free(c2v);                                                                                                       \
+    return loss;                                                                                                     \
+  }
+
+NMS_FORWARD(orc_nms_forward_f, float, fabsf, fmaxf, log1pf, expf)
+NMS_FORWARD(orc_nms_forward_d, double, fabs, fmax, log1p, exp)
+
+/* ================================================================================= */
 /* a3: rate dematcher -- ldpc_rate_dematcher_impl.cpp:29-250                          */
 /* ================================================================================= */
 

@@ -69,6 +69,14 @@ unsigned orc_ldpc_decode(const orc_dec_cfg* cfg, int force_decoding, int early_s
 unsigned orc_ldpc_decode_learned(const orc_dec_cfg* cfg, const uint32_t* tab, uint32_t crc_poly, const int8_t* llrs,
                                  unsigned nof_llrs, uint8_t* out_packed);
 
+/* Loss of the learned decoder's floating-point training model (float and double
+ * arithmetic): T iterations of L layers, tables alpha/beta [T][M], iteration weights w,
+ * logit scale tau; msg = packed message bits (targets). */
+double orc_nms_forward_f(int bg, unsigned Z, unsigned F, unsigned L, unsigned T, const int8_t* in, unsigned n_in,
+                         const uint8_t* msg, const double* alpha, const double* beta, const double* w, double tau);
+double orc_nms_forward_d(int bg, unsigned Z, unsigned F, unsigned L, unsigned T, const int8_t* in, unsigned n_in,
+                         const uint8_t* msg, const double* alpha, const double* beta, const double* w, double tau);
+
 /* ---- a3: rate dematcher ---------------------------------------------------------- */
 typedef struct {
   unsigned rv;        /* 0..3 */

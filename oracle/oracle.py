@@ -76,6 +76,10 @@ def lib():
         L.orc_ldpc_decode_learned.restype = C.c_uint
         L.orc_ldpc_decode_learned.argtypes = [C.POINTER(_DecCfg), C.POINTER(C.c_uint32), C.c_uint32, _i8p, C.c_uint,
                                               _u8p]
+        for f in (L.orc_nms_forward_f, L.orc_nms_forward_d):
+            f.restype = C.c_double
+            f.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, _i8p, C.c_uint, _u8p,
+                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double]
         L.orc_gen_codeblock.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint32, C.c_uint, C.c_uint, C.c_uint,
                                         C.c_uint, C.c_double, C.c_uint64, C.c_uint64, _u8p, _i8p]
         L.orc_tb_concat_crc.restype = C.c_uint32
@@ -337,3 +341,17 @@ def decode_learned(llrs, bg, Z, alpha_q, beta_q, F=0, crc_poly=CRC24B, max_iter=
     r = lib().orc_ldpc_decode_learned(C.byref(cfg), tab.ctypes.data_as(C.POINTER(C.c_uint32)), crc_poly,
                                       _p(llrs, _i8p), llrs.size, _p(out, _u8p))
     return int(r), out
+
+
+def nms_forward(llrs, msg_packed, bg, Z, L, alpha, beta, w, tau, F=0, double=True):
+    """Loss of the learned decoder's floating-point training model; alpha, beta: (T, M)."""
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    msg = np.ascontiguousarray(msg_packed, dtype=np.uint8)
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    wv = np.ascontiguousarray(w, dtype=np.float64)
+    T = a.shape[0]
+    dp = C.POINTER(C.c_double)
+    f = lib().orc_nms_forward_d if double else lib().orc_nms_forward_f
+    return float(f(bg, Z, F, L, T, _p(llrs, _i8p), llrs.size, _p(msg, _u8p), a.ctypes.data_as(dp),
+                   b.ctypes.data_as(dp), wv.ctypes.data_as(dp), tau))

@@ -11,7 +11,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "ocudu_hip_ldpc.h"
 
 def declared():
     txt = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"^(?:int|const char\*|void)\s+(lh_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char\*|void|size_t)\s+(lh_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_declares_entry_points():
@@ -32,6 +32,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(H.DecodeDesc) == 8 + 8 + 4 + 4 + 20 + 4  # padded to 8
     assert ctypes.sizeof(H.RmDesc) == 8 + 8 + 7 * 4 + 4
     assert ctypes.sizeof(H.PuschCbDesc) == ctypes.sizeof(H.RmDesc) + 8 + 8
+    assert ctypes.sizeof(H.TbDesc) == 40
+    assert ctypes.sizeof(H.GenDesc) == 5 * 8 + 8 * 4
+    assert ctypes.sizeof(H.TrainDesc) == 2 * 8 + 4 * 4
 
 
 def test_invalid_arguments_fail_without_gpu():
