@@ -22,7 +22,8 @@ namespace lh {
 size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage);
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, hipStream_t s);
+                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
+                         hipStream_t s);
 int decode_occupancy(int nt, size_t lds, bool learned);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
@@ -291,8 +292,10 @@ int groups_per_item(int Z, int nt)
   return std::max(1, std::min(kMaxGroups, nt / Z));
 }
 
+// Packs, uploads and launches a plan; with scale_tab the learned decoder runs, and with
+// fb_iters > 0 the plain decoder then runs as its fallback over the same descriptors.
 int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr)
+             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr, unsigned fb_iters = 0)
 {
   std::vector<dec_job>& jobs = plan.jobs;
   if (jobs.empty()) {
@@ -352,29 +355,35 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
     items.push_back(it);
     i = k;
   }
-  // Device block: header | items | jobs.
-  launch_header hdr{};
-  hdr.next_item = 0;
-  hdr.n_items = (uint32_t)items.size();
-  const size_t o_items = off + sizeof(launch_header);
+  // Device block: header (pass 1) | header (fallback pass) | items | jobs.
+  launch_header hdr[2]{};
+  hdr[0].n_items = hdr[1].n_items = (uint32_t)items.size();
+  const size_t o_items = off + sizeof(hdr);
   const size_t o_jobs = o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
   const size_t end = o_jobs + jobs.size() * sizeof(dec_job);
   if (end > ctx->slot_bytes) {
     return fail(LH_ERR_CAPACITY, "descriptor block exceeds the staging capacity");
   }
-  std::memcpy(s->host + off, &hdr, sizeof(hdr));
+  std::memcpy(s->host + off, hdr, sizeof(hdr));
   std::memcpy(s->host + o_items, items.data(), items.size() * sizeof(dec_item));
   std::memcpy(s->host + o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
   int rc = upload(ctx, s, end, stream);
   if (rc != LH_OK) {
     return rc;
   }
+  launch_header* d_hdr = reinterpret_cast<launch_header*>(s->dev + off);
+  const dec_item* d_items = reinterpret_cast<const dec_item*>(s->dev + o_items);
+  const dec_job* d_jobs = reinterpret_cast<const dec_job*>(s->dev + o_jobs);
   const int per_cu = std::max(1, decode_occupancy(nt, lds, scale_tab != nullptr));
   const unsigned grid = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu);
-  hipError_t e = launch_decode(reinterpret_cast<launch_header*>(s->dev + off),
-                               reinterpret_cast<const dec_item*>(s->dev + o_items),
-                               reinterpret_cast<const dec_job*>(s->dev + o_jobs), grid, nt, lds, rm, cw, bits, results,
-                               ctx->d_crc_x, ctx->d_gtab, scale_tab, stream);
+  hipError_t e = launch_decode(d_hdr, d_items, d_jobs, grid, nt, lds, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
+                               scale_tab, 0, stream);
+  if (e == hipSuccess && fb_iters > 0) {
+    const int per_cu_fb = std::max(1, decode_occupancy(nt, lds, false));
+    const unsigned grid_fb = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu_fb);
+    e = launch_decode(d_hdr + 1, d_items, d_jobs, grid_fb, nt, lds, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
+                      nullptr, (int)fb_iters, stream);
+  }
   return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
@@ -738,25 +747,10 @@ int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
       return fail(LH_ERR_INVALID, "no learned scale table for BG%u", cbs[i].rm.base_graph);
     }
   }
-  // Fallback jobs: the plain decoder on the HARQ buffers the learned pass wrote back,
-  // for the codeblocks whose CRC the learned pass did not pass.
-  launch_plan fb;
-  if (nof_fallback_iterations > 0) {
-    fb.jobs = plan.jobs;
-    for (dec_job& j : fb.jobs) {
-      j.flags = (uint8_t)((j.flags & kFlagCrcAfter) | kFlagSkipPassed);
-      j.max_iter = (uint8_t)nof_fallback_iterations;
-    }
-  }
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
-  rc = run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0, ctx->d_scale);
-  if (rc != LH_OK || fb.jobs.empty()) {
-    return rc;
-  }
-  s = acquire_slot(ctx);
-  return run_plan(ctx, fb, d_rm, d_cw, d_bits, d_results, st, s, 0);
+  return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0, ctx->d_scale, nof_fallback_iterations);
 }
 
 int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits, const int32_t* d_cb_results,

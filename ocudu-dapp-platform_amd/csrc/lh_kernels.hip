@@ -420,7 +420,8 @@ template <int BG, bool LEARNED>
 __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs, int8_t* __restrict__ rm_base,
                             const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
                             int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x,
-                            const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ scale_tab, uint8_t* lds)
+                            const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ scale_tab, int fb_iters,
+                            uint8_t* lds)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
@@ -432,10 +433,19 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   const bool active = g < G;
   const int gg = active ? g : 0;
   const unsigned r = (unsigned)(t - g * Z);
-  const dec_job jb = jobs[item.first_job + gg];
+  dec_job jb = jobs[item.first_job + gg];
+  if (fb_iters > 0) {
+    // Fallback pass over the jobs of a fused PUSCH pass: the plain decoder on the HARQ
+    // buffer that pass wrote, for the codeblocks it did not decode.
+    jb.flags = (uint8_t)((jb.flags & kFlagCrcAfter) | kFlagSkipPassed);
+    jb.max_iter = (uint8_t)fb_iters;
+    jb.in_off = jb.rm.rm_off;
+    jb.n_in = jb.rm.N;
+  }
   const uint32_t* __restrict__ gt = gtab + jb.gt_off;
   // Fallback pass: codeblocks that an earlier pass decoded are left alone.
   const bool skip = active && (jb.flags & kFlagSkipPassed) && results[jb.out_idx] > 0;
+  const bool work = active && !skip; // groups that load and decode
 
   // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
   const int soft_stride = ((K + lmax) * Z + 15) & ~15;
@@ -475,7 +485,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     // Stage the E codeword LLRs in LDS with aligned 16-byte loads (a 16-byte block that
     // holds a byte of the codeblock lies in that byte's page).
     const bool staged = stage_stride > 0;
-    if (staged && active) {
+    if (staged && work) {
       const uintptr_t a = reinterpret_cast<uintptr_t>(in);
       const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
       const int lead = (int)(a & 15);
@@ -495,14 +505,14 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     if (staged) {
       __syncthreads();
     }
-    if (active) {
+    if (work) {
       if (staged) {
         fused_positions(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
       } else {
         fused_positions(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
       }
     }
-  } else if (active) {
+  } else if (work) {
     // Aligned 16-byte loads of the dematched soft bits.
     const int8_t* in = rm_base + jb.in_off;
     const uintptr_t a = reinterpret_cast<uintptr_t>(in);
@@ -525,7 +535,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
       }
     }
   }
-  if (active) {
+  if (work) {
     for (int i = (int)r; i < 2 * Z; i += Z) {
       soft[i] = 0;
     }
@@ -533,7 +543,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
       soft[2 * Z + i] = 0;
     }
   }
-  group_max(last, g, active, ctl->last);
+  group_max(last, g, work, ctl->last);
   __syncthreads();
   const int input_size = ctl->last[gg] + 1;
 
@@ -724,7 +734,7 @@ k_ldpc_decode(launch_header* __restrict__ hdr,
                                                              int32_t* __restrict__ results,
                                                              const uint32_t* __restrict__ crc_x,
                                                              const uint32_t* __restrict__ gtab,
-                                                             const uint32_t* __restrict__ scale_tab)
+                                                             const uint32_t* __restrict__ scale_tab, int fb_iters)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t item_id;
@@ -741,10 +751,11 @@ k_ldpc_decode(launch_header* __restrict__ hdr,
     }
     const dec_item item = items[id];
     if (item.bg == 1) {
-      decode_item<1, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab, lds);
+      decode_item<1, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab, fb_iters,
+                              lds);
     } else {
       decode_item<2, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
-                              LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, lds);
+                              LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
     }
   }
 }
@@ -909,7 +920,8 @@ size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
 
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, hipStream_t s)
+                         const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
+                         hipStream_t s)
 {
   const void* fn = scale_tab ? (const void*)k_ldpc_decode<true> : (const void*)k_ldpc_decode<false>;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -918,10 +930,10 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
   }
   if (scale_tab) {
     hipLaunchKernelGGL(k_ldpc_decode<true>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
-                       results, crc_x, gtab, scale_tab);
+                       results, crc_x, gtab, scale_tab, fb_iters);
   } else {
     hipLaunchKernelGGL(k_ldpc_decode<false>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
-                       results, crc_x, gtab, scale_tab);
+                       results, crc_x, gtab, scale_tab, fb_iters);
   }
   return hipGetLastError();
 }

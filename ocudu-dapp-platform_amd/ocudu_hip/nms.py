@@ -21,7 +21,8 @@ from pathlib import Path
 
 import numpy as np
 
-from . import (LH_CRC24B, Context, DeviceBuffer, GenDesc, PuschCbDesc, RmDesc, TrainDesc, k_b, msg_bytes, n_short)
+from . import (LH_CRC24B, Context, DeviceBuffer, Event, GenDesc, PuschCbDesc, RmDesc, TrainDesc, k_b, msg_bytes,
+               n_short)
 
 M_LAYERS = {1: 46, 2: 42}
 ITERS = 8
@@ -86,16 +87,30 @@ class DeviceBatch:
         for c0 in range(0, len(cbs), 4096):
             ctx.gen_codeblocks(gen[c0:c0 + 4096], self.d_cw, self.d_msg)
             ctx.rate_dematch(rms[c0:c0 + 4096], self.d_cw, self.d_rm, new_data=True)
-        self.info_bits = np.array([k_b(bg) * Z - 24] * len(cbs))
+        self.info_bits = k_b(bg) * Z - 24
+        self.arr = (PuschCbDesc * len(self.pusch))(*self.pusch)
 
     def decode(self, ctx, learned=False, iters=ITERS, fb_iters=0, stream=None):
-        self.d_bits.zero()
         if learned:
-            ctx.pusch_decode_batch_learned(self.pusch, self.d_cw, self.d_rm, self.d_bits, self.d_res,
-                                           nof_ldpc_iterations=iters, nof_fallback_iterations=fb_iters, stream=stream)
+            ctx.pusch_decode_batch_learned(None, self.d_cw, self.d_rm, self.d_bits, self.d_res,
+                                           nof_ldpc_iterations=iters, nof_fallback_iterations=fb_iters, stream=stream,
+                                           arr=self.arr)
         else:
-            ctx.pusch_decode_batch(self.pusch, self.d_cw, self.d_rm, self.d_bits, self.d_res,
-                                   nof_ldpc_iterations=iters, stream=stream)
+            ctx.pusch_decode_batch(None, self.d_cw, self.d_rm, self.d_bits, self.d_res, nof_ldpc_iterations=iters,
+                                   stream=stream, arr=self.arr)
+
+    def timed_decode(self, ctx, learned=False, iters=ITERS, fb_iters=0, reps=3):
+        """Best event-timed duration (ms) of `reps` decodes of the batch (descriptor upload and
+        kernels; inputs resident)."""
+        e0, e1 = Event(), Event()
+        best = float("inf")
+        for _ in range(reps):
+            e0.record()
+            self.decode(ctx, learned, iters, fb_iters)
+            e1.record()
+            e1.sync()
+            best = min(best, e0.elapsed_ms(e1))
+        return best
 
     def outcome(self):
         """(block error flags, iterations) of the last decode: an error is a failed CRC or
@@ -184,6 +199,28 @@ def bler(ctx, bg, Z, E, qm, snr, n, seed, learned=False, iters=ITERS, fb_iters=0
     return float(err.mean()), float(its.mean())
 
 
+def measure(ctx, batch, learned, iters=ITERS, fb_iters=0, reps=3):
+    """BLER, mean iterations, decode time (ms) and info Gbit/s (correct codeblocks only) of
+    one decoder on a resident batch."""
+    ms = batch.timed_decode(ctx, learned, iters, fb_iters, reps)
+    err, res = batch.outcome()
+    its = np.where(res > 0, res, iters)
+    good = int((~err).sum())
+    return {"bler": float(err.mean()), "iters": float(its.mean()), "ms": ms,
+            "gbps": good * batch.info_bits / (ms * 1e-3) / 1e9}
+
+
+def operating_point(ctx, bg, Z, E, qm, lo, hi, n=4096, seed=5, step=0.05, target=0.10, iters=ITERS):
+    """Lowest SNR on a `step` dB grid from `lo` at which the plain decoder's BLER <= target."""
+    snr = lo
+    while snr <= hi + 1e-9:
+        e, _ = bler(ctx, bg, Z, E, qm, snr, n, seed, iters=iters)
+        if e <= target:
+            return round(snr, 4)
+        snr += step
+    return None
+
+
 def calibrate(ctx, bg, Z, E, qm, n=512, seed=1, iters=ITERS, lo=-4.0, hi=30.0, targets=(0.30, 0.01)):
     """SNRs (dB) at which the plain decoder's BLER crosses each target (bisection)."""
     out = []
@@ -220,6 +257,43 @@ def train(ctx, bg, Z, configs, total_cbs, batch, seed=0, iters=ITERS, lr=0.01, l
     return tr
 
 
+def curves(ctx, bg, Z, configs, table, n=8192, points=12, iters=ITERS, seed=2024, log=print):
+    """BLER curves of the plain and the learned int8 decoders: for each (rate, Qm, snr_30,
+    snr_1) configuration, `points` SNRs from snr_30 - 0.3 dB to snr_1 + 0.5 dB.  Returns rows
+    (rate, Qm, snr, plain dict, learned dict, learned + fallback dict)."""
+    ctx.set_scale_table(bg, *table)
+    rows = []
+    for rate, qm, lo, hi in configs:
+        E = rm_length(bg, Z, rate, qm)
+        for snr in np.linspace(lo - 0.3, hi + 0.5, points):
+            b = DeviceBatch(ctx, bg, Z, [(E, qm, float(snr))] * n, seed)
+            p = measure(ctx, b, False, iters)
+            l = measure(ctx, b, True, iters)
+            lf = measure(ctx, b, True, iters, fb_iters=iters)
+            rows.append((rate, qm, float(snr), p, l, lf))
+            log(f"rate {rate} Qm {qm} {snr:6.2f} dB: plain {p['bler']:.4f} ({p['iters']:.2f} it, {p['gbps']:.2f} Gb/s)"
+                f" learned {l['bler']:.4f} ({l['iters']:.2f} it, {l['gbps']:.2f} Gb/s)"
+                f" +fallback {lf['bler']:.4f} ({lf['gbps']:.2f} Gb/s)")
+    return rows
+
+
+def curves_markdown(bg, Z, rows, n, command):
+    out = [f"# BLER of the plain and learned int8 decoders, BG{bg} Z={Z}", "",
+           f"{n} codeblocks per point from the device generator (CRC24B, rv0, AWGN, max-log "
+           f"demapping), max {ITERS} iterations, CRC early stop; a block error is a failed CRC or wrong bits.",
+           "Gbit/s = info bits of correct codeblocks / event-timed decode (descriptor upload + kernels).", "",
+           f"Command: `{command}`", "",
+           "| rate | Qm | Es/N0 dB | plain BLER | plain it | plain Gb/s | learned BLER | learned it | learned Gb/s "
+           "| ratio | learned+fallback BLER | +fallback Gb/s |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for rate, qm, snr, p, l, lf in rows:
+        ratio = l["gbps"] / p["gbps"] if p["gbps"] > 0 else float("nan")
+        out.append(f"| {rate} | {qm} | {snr:.2f} | {p['bler']:.4f} | {p['iters']:.2f} | {p['gbps']:.2f} | "
+                   f"{l['bler']:.4f} | {l['iters']:.2f} | {l['gbps']:.2f} | {ratio:.2f} | {lf['bler']:.4f} | "
+                   f"{lf['gbps']:.2f} |")
+    return "\n".join(out) + "\n"
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -233,8 +307,9 @@ def _main(argv=None):
     t.add_argument("--lr", type=float, default=0.01)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", required=True)
+    t.add_argument("--curves", help="also write BLER curves (markdown) of the trained table here")
     args = ap.parse_args(argv)
-    ctx = Context(0, max_codeblocks=max(4096, args.batch))
+    ctx = Context(0, max_codeblocks=max(8192, args.batch))
 
     def log(m):
         print(m, flush=True)
@@ -253,6 +328,12 @@ def _main(argv=None):
     save_table(args.out, args.bg, aq, bq, {"Z": args.Z, "configs": configs, "codeblocks": args.cbs,
                                           "batch": args.batch, "lr": args.lr})
     log(f"table written to {args.out}")
+    if args.curves:
+        n = 8192
+        rows = curves(ctx, args.bg, args.Z, configs, (aq, bq), n=n, log=log)
+        cmd = "python -m ocudu_hip.nms " + " ".join(argv if argv is not None else sys.argv[1:])
+        Path(args.curves).write_text(curves_markdown(args.bg, args.Z, rows, n, cmd))
+        log(f"curves written to {args.curves}")
     return 0
 
 
