@@ -70,6 +70,7 @@ def main():
     stream = H.Stream()
     d_rm = H.DeviceBuffer(n_cb * N)
     d_bits = H.DeviceBuffer(n_cb * msg_b)
+    d_bits.zero()
     d_res = H.DeviceBuffer(4 * n_cb)
     cbs = (H.PuschCbDesc * n_cb)()
     for i in range(n_cb):

@@ -206,6 +206,28 @@ typedef struct lh_tb_desc {
 int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits,
                    const int32_t* d_cb_results, uint8_t* d_tb, int32_t* d_tb_status, void* stream);
 
+/* ---- Slot graphs: one slot's batch captured once as a HIP graph and replayed. ----
+ * The descriptors of the batch are packed and uploaded once; the graph rewinds the work
+ * queue, runs the persistent decoder (learned + plain fallback when `learned`), and
+ * reassembles the transport blocks (lh_tb_finalize) when n_tbs > 0.  Each launch decodes
+ * whatever LLRs d_cw holds at that time into the same buffers, so a serving loop refills
+ * d_cw per slot and replays (the hipGraph per slot of the hot-path design; the
+ * reference's CUDA backend launches its kernels per call,
+ * lib/phy/upper/channel_coding/ldpc/cuda/pusch_codeblock_decoder_cuda_batch.h). */
+typedef struct lh_slot_graph lh_slot_graph;
+int lh_slot_graph_build(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                        uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                        unsigned nof_ldpc_iterations, int learned, unsigned nof_fallback_iterations,
+                        const lh_tb_desc* tbs, unsigned n_tbs, uint8_t* d_tb, int32_t* d_tb_status,
+                        lh_slot_graph** graph);
+int lh_slot_graph_launch(lh_slot_graph* graph, void* stream);
+int lh_slot_graph_destroy(lh_slot_graph* graph);
+
+/* Launch tuning of one (base graph, lifting size), as an autotuner found it: codeblocks
+ * packed per workgroup (0 = as many as fit the workgroup) and LDS staging of the codeword
+ * LLRs (0 = read them from HBM).  Applies to later decode calls and graph builds. */
+int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigned groups, int stage_llrs);
+
 /* One synthetic codeblock of lh_gen_codeblocks. */
 typedef struct lh_gen_desc {
   uint64_t llr_offset;  /* its rm_length int8 LLRs in d_llrs */

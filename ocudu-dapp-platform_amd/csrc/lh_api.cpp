@@ -38,6 +38,7 @@ hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt,
                             const uint32_t* gtab, int backward, hipStream_t s);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, int n_alpha, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_reset_headers(launch_header* hdr, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
 } // namespace lh
@@ -121,6 +122,13 @@ void crc_x_table(uint32_t poly, uint32_t* out)
 
 constexpr int kRing = 4;
 
+// Tuned launch parameters of one (BG, Z): codeblocks per workgroup (0 = as many as fit)
+// and LDS staging of the codeword LLRs (1 = when it fits, 0 = never).
+struct tune_entry {
+  uint16_t groups = 0;
+  uint8_t stage = 1;
+};
+
 struct slot {
   uint8_t* host = nullptr;
   uint8_t* dev = nullptr;
@@ -139,6 +147,7 @@ struct lh_ctx {
   uint32_t* d_gtab = nullptr;
   uint32_t* d_tbcrc = nullptr; // CRC24A byte table (256) + x^(2^k) mod G (32)
   uint32_t* d_scale = nullptr; // learned normalisation tables (kScaleTableWords)
+  tune_entry tune[2][kNofLiftingSizes] = {}; // per (BG, Z): 0 = default
   bool scale_set[2] = {false, false};
   slot ring[kRing];
   int next = 0;
@@ -292,22 +301,35 @@ int groups_per_item(int Z, int nt)
   return std::max(1, std::min(kMaxGroups, nt / Z));
 }
 
-// Packs, uploads and launches a plan; with scale_tab the learned decoder runs, and with
-// fb_iters > 0 the plain decoder then runs as its fallback over the same descriptors.
-int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr, unsigned fb_iters = 0)
+// A packed decoder launch: device block header (pass 1) | header (fallback pass) | items
+// | jobs, and the launch shape.
+struct plan_block {
+  std::vector<uint8_t> bytes;
+  size_t o_items = 0, o_jobs = 0;
+  size_t n_items = 0;
+  int nt = 64;
+  size_t lds = 0;
+};
+
+// Sorts the jobs largest first (longest-processing-time order for the dynamic queue; equal
+// (BG, Z) stay adjacent so they pack into shared items), packs items of up to G codeblocks
+// of one (BG, Z) -- G from the tuning table, else as many as fit the workgroup -- and
+// decides LDS staging of the codeword LLRs per item.
+int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
 {
   std::vector<dec_job>& jobs = plan.jobs;
-  if (jobs.empty()) {
-    return LH_OK;
-  }
-  int maxZ = 0;
+  // Workgroup size: the largest lifting size, or a tuned packing of G codeblocks of one
+  // (BG, Z), rounded up to whole waves.
+  int nt = 64;
   for (const dec_job& j : jobs) {
-    maxZ = std::max<int>(maxZ, j.Z);
+    const int bg = j.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
+    const tune_entry& te = ctx->tune[bg - 1][lifting_position(j.Z)];
+    nt = std::max(nt, ((j.Z + 63) / 64) * 64);
+    if (te.groups > 0) {
+      nt = std::max(nt, ((te.groups * j.Z + 63) / 64) * 64);
+    }
   }
-  const int nt = ((maxZ + 63) / 64) * 64;
-  // Largest work first (longest-processing-time order for the dynamic queue); equal
-  // (BG, Z) stay adjacent so they pack into shared items.
+  nt = std::min(nt, kMaxThreads);
   std::stable_sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
     const int ka = (int)a.Z * (a.lbound + 22), kb = (int)b.Z * (b.lbound + 22);
     const int bga = a.gt_off < kNofLiftingSizes * 316 ? 1 : 2, bgb = b.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
@@ -324,7 +346,12 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
   for (size_t i = 0; i < jobs.size();) {
     const int bg = jobs[i].gt_off < kNofLiftingSizes * 316 ? 1 : 2;
     const int Z = jobs[i].Z;
-    const int G = groups_per_item(Z, nt);
+    const tune_entry& te = ctx->tune[bg - 1][lifting_position((unsigned)Z)];
+    int G = te.groups > 0 ? std::min<int>(te.groups, groups_per_item(Z, nt)) : groups_per_item(Z, nt);
+    // Fewer codeblocks per item when their soft bits would not fit the LDS.
+    while (G > 1 && decode_item_lds_bytes(bg, Z, jobs[i].lbound, G, 0) > kLdsBudget) {
+      --G;
+    }
     dec_item it{};
     it.first_job = (uint32_t)i;
     it.bg = (uint8_t)bg;
@@ -346,7 +373,7 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
     if (need > kLdsBudget) {
       return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", Z, lmax);
     }
-    const int stage = fused ? ((maxE + 15) & ~15) : 0;
+    const int stage = (fused && te.stage != 0) ? ((maxE + 15) & ~15) : 0;
     if (stage > 0 && decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage) <= kLdsBudget) {
       it.stage = (uint32_t)stage;
       need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage);
@@ -355,35 +382,63 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
     items.push_back(it);
     i = k;
   }
-  // Device block: header (pass 1) | header (fallback pass) | items | jobs.
   launch_header hdr[2]{};
   hdr[0].n_items = hdr[1].n_items = (uint32_t)items.size();
-  const size_t o_items = off + sizeof(hdr);
-  const size_t o_jobs = o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
-  const size_t end = o_jobs + jobs.size() * sizeof(dec_job);
-  if (end > ctx->slot_bytes) {
-    return fail(LH_ERR_CAPACITY, "descriptor block exceeds the staging capacity");
+  pb.o_items = sizeof(hdr);
+  pb.o_jobs = pb.o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
+  pb.bytes.assign(pb.o_jobs + jobs.size() * sizeof(dec_job), 0);
+  std::memcpy(pb.bytes.data(), hdr, sizeof(hdr));
+  std::memcpy(pb.bytes.data() + pb.o_items, items.data(), items.size() * sizeof(dec_item));
+  std::memcpy(pb.bytes.data() + pb.o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
+  pb.n_items = items.size();
+  pb.nt = nt;
+  pb.lds = lds;
+  return LH_OK;
+}
+
+// Launches a packed block resident at d_block: the decoder (learned with scale_tab), then
+// with fb_iters > 0 the plain decoder as its fallback over the same descriptors.
+hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int8_t* rm, const int8_t* cw,
+                        uint8_t* bits, int32_t* results, hipStream_t stream, const uint32_t* scale_tab,
+                        unsigned fb_iters)
+{
+  launch_header* d_hdr = reinterpret_cast<launch_header*>(d_block);
+  const dec_item* d_items = reinterpret_cast<const dec_item*>(d_block + pb.o_items);
+  const dec_job* d_jobs = reinterpret_cast<const dec_job*>(d_block + pb.o_jobs);
+  const int per_cu = std::max(1, decode_occupancy(pb.nt, pb.lds, scale_tab != nullptr));
+  const unsigned grid = (unsigned)std::min<size_t>(pb.n_items, (size_t)ctx->nof_cus * per_cu);
+  hipError_t e = launch_decode(d_hdr, d_items, d_jobs, grid, pb.nt, pb.lds, rm, cw, bits, results, ctx->d_crc_x,
+                               ctx->d_gtab, scale_tab, 0, stream);
+  if (e == hipSuccess && fb_iters > 0) {
+    const int per_cu_fb = std::max(1, decode_occupancy(pb.nt, pb.lds, false));
+    const unsigned grid_fb = (unsigned)std::min<size_t>(pb.n_items, (size_t)ctx->nof_cus * per_cu_fb);
+    e = launch_decode(d_hdr + 1, d_items, d_jobs, grid_fb, pb.nt, pb.lds, rm, cw, bits, results, ctx->d_crc_x,
+                      ctx->d_gtab, nullptr, (int)fb_iters, stream);
   }
-  std::memcpy(s->host + off, hdr, sizeof(hdr));
-  std::memcpy(s->host + o_items, items.data(), items.size() * sizeof(dec_item));
-  std::memcpy(s->host + o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
-  int rc = upload(ctx, s, end, stream);
+  return e;
+}
+
+// Packs, uploads (staging ring) and launches a plan.
+int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
+             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr, unsigned fb_iters = 0)
+{
+  if (plan.jobs.empty()) {
+    return LH_OK;
+  }
+  plan_block pb;
+  int rc = build_block(ctx, plan, pb);
   if (rc != LH_OK) {
     return rc;
   }
-  launch_header* d_hdr = reinterpret_cast<launch_header*>(s->dev + off);
-  const dec_item* d_items = reinterpret_cast<const dec_item*>(s->dev + o_items);
-  const dec_job* d_jobs = reinterpret_cast<const dec_job*>(s->dev + o_jobs);
-  const int per_cu = std::max(1, decode_occupancy(nt, lds, scale_tab != nullptr));
-  const unsigned grid = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu);
-  hipError_t e = launch_decode(d_hdr, d_items, d_jobs, grid, nt, lds, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
-                               scale_tab, 0, stream);
-  if (e == hipSuccess && fb_iters > 0) {
-    const int per_cu_fb = std::max(1, decode_occupancy(nt, lds, false));
-    const unsigned grid_fb = (unsigned)std::min<size_t>(items.size(), (size_t)ctx->nof_cus * per_cu_fb);
-    e = launch_decode(d_hdr + 1, d_items, d_jobs, grid_fb, nt, lds, rm, cw, bits, results, ctx->d_crc_x, ctx->d_gtab,
-                      nullptr, (int)fb_iters, stream);
+  if (off + pb.bytes.size() > ctx->slot_bytes) {
+    return fail(LH_ERR_CAPACITY, "descriptor block exceeds the staging capacity");
   }
+  std::memcpy(s->host + off, pb.bytes.data(), pb.bytes.size());
+  rc = upload(ctx, s, off + pb.bytes.size(), stream);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = launch_block(ctx, pb, s->dev + off, rm, cw, bits, results, stream, scale_tab, fb_iters);
   return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
@@ -753,6 +808,174 @@ int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0, ctx->d_scale, nof_fallback_iterations);
 }
 
+} // extern "C"
+
+struct lh_slot_graph {
+  lh_ctx* ctx = nullptr;
+  uint8_t* d_block = nullptr; // decoder descriptors (resident)
+  uint8_t* d_tbs = nullptr;   // TB descriptors (resident)
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int validate_tbs(const lh_tb_desc* tbs, unsigned n)
+{
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_tb_desc& d = tbs[i];
+    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || d.nof_cbs == 0) {
+      return fail(LH_ERR_INVALID, "tb %u: invalid size", i);
+    }
+    if (d.nof_cbs > 1 && (d.cb_data_bits == 0 || (uint64_t)d.cb_data_bits * d.nof_cbs < (uint64_t)d.tbs_bits + 24 ||
+                          (uint64_t)d.cb_data_bits * (d.nof_cbs - 1) > d.tbs_bits)) {
+      return fail(LH_ERR_INVALID, "tb %u: codeblock data bits do not cover the TB and its CRC", i);
+    }
+  }
+  return LH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int lh_slot_graph_build(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw, int8_t* d_rm,
+                        uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                        unsigned nof_ldpc_iterations, int learned, unsigned nof_fallback_iterations,
+                        const lh_tb_desc* tbs, unsigned n_tbs, uint8_t* d_tb, int32_t* d_tb_status,
+                        lh_slot_graph** out)
+{
+  if (out == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_slot_graph_build");
+  }
+  *out = nullptr;
+  int rc = check_pusch_args(ctx, cbs, n, d_cw, d_rm, d_bits, d_results, "lh_slot_graph_build");
+  if (rc != LH_OK) {
+    return rc;
+  }
+  if (n == 0 || (n_tbs > 0 && (tbs == nullptr || d_tb == nullptr || d_tb_status == nullptr)) || n_tbs > ctx->max_cb ||
+      nof_fallback_iterations > 255) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_slot_graph_build");
+  }
+  rc = validate_tbs(tbs, n_tbs);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  if (learned) {
+    for (unsigned i = 0; i < n; ++i) {
+      if (!ctx->scale_set[cbs[i].rm.base_graph - 1]) {
+        return fail(LH_ERR_INVALID, "no learned scale table for BG%u", cbs[i].rm.base_graph);
+      }
+    }
+  }
+  launch_plan plan;
+  rc = pusch_jobs(cbs, n, new_data, use_early_stop, nof_ldpc_iterations, plan);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  plan_block pb;
+  rc = build_block(ctx, plan, pb);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  (void)hipSetDevice(ctx->device);
+  lh_slot_graph* g = new lh_slot_graph();
+  g->ctx = ctx;
+  hipError_t e = hipMalloc(&g->d_block, pb.bytes.size());
+  if (e == hipSuccess) {
+    e = hipMemcpy(g->d_block, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && n_tbs > 0) {
+    e = hipMalloc(&g->d_tbs, n_tbs * sizeof(lh_tb_desc));
+    if (e == hipSuccess) {
+      e = hipMemcpy(g->d_tbs, tbs, n_tbs * sizeof(lh_tb_desc), hipMemcpyHostToDevice);
+    }
+  }
+  if (e == hipSuccess) {
+    e = hipStreamCreate(&g->cap);
+  }
+  // Graph: rewind the work queues, decode (+ fallback), reassemble the TBs.
+  if (e == hipSuccess) {
+    e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeRelaxed);
+  }
+  if (e == hipSuccess) {
+    hipError_t le = launch_reset_headers(reinterpret_cast<launch_header*>(g->d_block), g->cap);
+    if (le == hipSuccess) {
+      le = launch_block(ctx, pb, g->d_block, d_rm, d_cw, d_bits, d_results, g->cap, learned ? ctx->d_scale : nullptr,
+                        learned ? nof_fallback_iterations : 0);
+    }
+    if (le == hipSuccess && n_tbs > 0) {
+      le = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(g->d_tbs), n_tbs, d_bits, d_results, d_tb,
+                              d_tb_status, ctx->d_tbcrc, ctx->d_tbcrc + 256, g->cap);
+    }
+    e = hipStreamEndCapture(g->cap, &g->graph);
+    if (le != hipSuccess) {
+      e = le;
+    }
+  }
+  if (e == hipSuccess) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  }
+  if (e != hipSuccess) {
+    lh_slot_graph_destroy(g);
+    return hip_fail(e, "slot graph build");
+  }
+  *out = g;
+  return LH_OK;
+}
+
+int lh_slot_graph_launch(lh_slot_graph* g, void* stream)
+{
+  if (g == nullptr || g->exec == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid slot graph");
+  }
+  (void)hipSetDevice(g->ctx->device);
+  hipError_t e = hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream));
+  return e == hipSuccess ? LH_OK : hip_fail(e, "slot graph launch");
+}
+
+int lh_slot_graph_destroy(lh_slot_graph* g)
+{
+  if (g == nullptr) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(g->ctx->device);
+  if (g->cap) {
+    (void)hipStreamSynchronize(g->cap);
+  }
+  (void)hipDeviceSynchronize(); // a replay may still read the descriptors
+  if (g->exec) {
+    (void)hipGraphExecDestroy(g->exec);
+  }
+  if (g->graph) {
+    (void)hipGraphDestroy(g->graph);
+  }
+  if (g->cap) {
+    (void)hipStreamDestroy(g->cap);
+  }
+  if (g->d_block) {
+    (void)hipFree(g->d_block);
+  }
+  if (g->d_tbs) {
+    (void)hipFree(g->d_tbs);
+  }
+  delete g;
+  return LH_OK;
+}
+
+int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigned groups, int stage_llrs)
+{
+  if (ctx == nullptr || (base_graph != 1 && base_graph != 2) || lifting_position(lifting_size) < 0 ||
+      groups > (unsigned)kMaxGroups) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_tune_set");
+  }
+  tune_entry& t = ctx->tune[base_graph - 1][lifting_position(lifting_size)];
+  t.groups = (uint16_t)groups;
+  t.stage = stage_llrs ? 1 : 0;
+  return LH_OK;
+}
+
 int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t* d_cb_bits, const int32_t* d_cb_results,
                    uint8_t* d_tb, int32_t* d_tb_status, void* stream)
 {
@@ -765,15 +988,9 @@ int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t
   if (n > ctx->max_cb) {
     return fail(LH_ERR_CAPACITY, "batch of %u transport blocks exceeds the context capacity %u", n, ctx->max_cb);
   }
-  for (unsigned i = 0; i < n; ++i) {
-    const lh_tb_desc& d = tbs[i];
-    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || d.nof_cbs == 0) {
-      return fail(LH_ERR_INVALID, "tb %u: invalid size", i);
-    }
-    if (d.nof_cbs > 1 && (d.cb_data_bits == 0 || (uint64_t)d.cb_data_bits * d.nof_cbs < (uint64_t)d.tbs_bits + 24 ||
-                          (uint64_t)d.cb_data_bits * (d.nof_cbs - 1) > d.tbs_bits)) {
-      return fail(LH_ERR_INVALID, "tb %u: codeblock data bits do not cover the TB and its CRC", i);
-    }
+  int rc0 = validate_tbs(tbs, n);
+  if (rc0 != LH_OK) {
+    return rc0;
   }
   if (n == 0) {
     return LH_OK;

@@ -948,6 +948,20 @@ int decode_occupancy(int nt, size_t lds, bool learned)
   return n;
 }
 
+// Rewinds the work-queue counters of a resident descriptor block (graph replays).
+__global__ void k_reset_headers(launch_header* hdr)
+{
+  if (threadIdx.x < 2) {
+    hdr[threadIdx.x].next_item = 0;
+  }
+}
+
+hipError_t launch_reset_headers(launch_header* hdr, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset_headers, dim3(1), dim3(64), 0, s, hdr);
+  return hipGetLastError();
+}
+
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s)
 {

@@ -186,7 +186,7 @@ class TrainDesc(C.Structure):
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
-            "lh_nms_adam"]
+            "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set"]
 
 
 def lib():
@@ -220,6 +220,13 @@ def lib():
                                         C.c_void_p]
         L.lh_nms_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint, C.c_void_p]
+        L.lh_slot_graph_build.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_int, C.c_uint,
+                                          C.POINTER(TbDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_void_p)]
+        L.lh_slot_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+        L.lh_slot_graph_destroy.argtypes = [C.c_void_p]
+        L.lh_tune_set.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_int]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -317,12 +324,50 @@ class Context:
                               d_v.ptr + offset, n, n_alpha, lr, beta1, beta2, eps, step,
                               stream.handle if stream else None), "lh_nms_adam")
 
+    def slot_graph(self, cbs, d_cw, d_rm, d_bits, d_results, new_data=True, use_early_stop=True,
+                   nof_ldpc_iterations=8, learned=False, nof_fallback_iterations=0, tbs=(), d_tb=None,
+                   d_tb_status=None):
+        return SlotGraph(self, cbs, d_cw, d_rm, d_bits, d_results, new_data, use_early_stop, nof_ldpc_iterations,
+                         learned, nof_fallback_iterations, tbs, d_tb, d_tb_status)
+
+    def tune_set(self, bg, Z, groups, stage_llrs=True):
+        _lh(lib().lh_tune_set(self.handle, bg, Z, groups, int(stage_llrs)), "lh_tune_set")
+
     def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
         n = len(offsets)
         offs = (C.c_uint64 * n)(*offsets)
         nb = (C.c_uint32 * n)(*nbits)
         _lh(lib().lh_crc_calculate(self.handle, poly, offs, nb, n, d_bits.ptr, d_out.ptr,
                                    stream.handle if stream else None), "lh_crc_calculate")
+
+
+class SlotGraph:
+    """A slot's batch captured as a HIP graph (lh_slot_graph_*)."""
+
+    def __init__(self, ctx, cbs, d_cw, d_rm, d_bits, d_results, new_data, use_early_stop, iters, learned, fb_iters,
+                 tbs, d_tb, d_tb_status):
+        arr = (PuschCbDesc * len(cbs))(*cbs)
+        tarr = (TbDesc * max(1, len(tbs)))(*tbs) if len(tbs) else None
+        h = C.c_void_p()
+        _lh(lib().lh_slot_graph_build(ctx.handle, arr, len(cbs), d_cw.ptr, d_rm.ptr, d_bits.ptr, d_results.ptr,
+                                      int(new_data), int(use_early_stop), iters, int(learned), fb_iters, tarr,
+                                      len(tbs), d_tb.ptr if d_tb else None, d_tb_status.ptr if d_tb_status else None,
+                                      C.byref(h)), "lh_slot_graph_build")
+        self.handle = h.value
+
+    def launch(self, stream=None):
+        _lh(lib().lh_slot_graph_launch(self.handle, stream.handle if stream else None), "lh_slot_graph_launch")
+
+    def close(self):
+        if self.handle:
+            lib().lh_slot_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def k_b(bg):

@@ -21,7 +21,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import (LH_CRC24B, Context, DeviceBuffer, Event, GenDesc, PuschCbDesc, RmDesc, TrainDesc, k_b, msg_bytes,
+from . import (LH_CRC16, LH_CRC24B, Context, DeviceBuffer, Event, GenDesc, PuschCbDesc, RmDesc, TrainDesc, k_b, msg_bytes,
                n_short)
 
 M_LAYERS = {1: 46, 2: 42}
@@ -56,38 +56,50 @@ def n0_of(esn0_db):
     return float(10.0 ** (-esn0_db / 10.0))
 
 
-class DeviceBatch:
-    """Generated codeblocks of one base graph and lifting size resident on the device:
-    codeword LLRs, rate-dematched soft bits, messages, and the descriptors of the decoder
-    and of the training step."""
+def cb_crc(bg, Z):
+    """CRC attached to a synthetic codeblock: CRC24B, or CRC16 for codeblocks of at most 48 bits."""
+    return (LH_CRC16, 16) if k_b(bg) * Z <= 48 else (LH_CRC24B, 24)
 
-    def __init__(self, ctx, bg, Z, cbs, seed):
-        """cbs: list of (E, Qm, Es/N0 dB)."""
-        self.bg, self.Z, self.n = bg, Z, len(cbs)
-        N = n_short(bg) * Z
-        mb = msg_bytes(bg, Z)
-        self.mb = mb
-        gen, rms, self.pusch, self.train = [], [], [], []
-        cw_off = 0
-        rs = np.random.default_rng(seed)
-        seeds = rs.integers(0, 2 ** 63, (len(cbs), 2))
-        for i, (E, qm, snr) in enumerate(cbs):
-            gen.append(GenDesc(cw_off, i * mb, int(seeds[i, 0]), int(seeds[i, 1]), n0_of(snr), E, bg, Z, 0, qm, 0, 0,
-                               LH_CRC24B))
-            rm = RmDesc(cw_off, i * N, E, bg, Z, 0, qm, 0, 0)
+
+class MixedBatch:
+    """Generated codeblocks of any (BG, Z) resident on the device: codeword LLRs,
+    rate-dematched soft bits (HARQ buffers), messages and the decoder descriptors."""
+
+    def __init__(self, ctx, cbs, seed, seeds=None):
+        """cbs: list of (bg, Z, E, Qm, Es/N0 dB); the generator seeds of codeblock i are
+        seeds[i] (message, noise), drawn from `seed` when not given."""
+        self.n = len(cbs)
+        gen, rms, self.pusch = [], [], []
+        self.msg_off, self.mbs, self.info = [], [], []
+        cw_off = rm_off = msg_off = 0
+        if seeds is None:
+            seeds = np.random.default_rng(seed).integers(0, 2 ** 63, (len(cbs), 2))
+        for i, (bg, Z, E, qm, snr) in enumerate(cbs):
+            poly, order = cb_crc(bg, Z)
+            mb = msg_bytes(bg, Z)
+            gen.append(GenDesc(cw_off, msg_off, int(seeds[i, 0]), int(seeds[i, 1]), n0_of(snr), E, bg, Z, 0, qm, 0, 0,
+                               poly))
+            rm = RmDesc(cw_off, rm_off, E, bg, Z, 0, qm, 0, 0)
             rms.append(rm)
-            self.pusch.append(PuschCbDesc(rm, i * mb, 24, LH_CRC24B))
-            self.train.append(TrainDesc(i * N, i * mb, N, Z, 0, layers_for(bg, Z, E)))
+            self.pusch.append(PuschCbDesc(rm, msg_off, order, poly))
+            self.msg_off.append(msg_off)
+            self.mbs.append(mb)
+            self.info.append(k_b(bg) * Z - order)
             cw_off = (cw_off + E + 15) // 16 * 16
+            rm_off += n_short(bg) * Z
+            msg_off += mb
         self.d_cw = DeviceBuffer(cw_off)
-        self.d_rm = DeviceBuffer(N * len(cbs))
-        self.d_msg = DeviceBuffer(mb * len(cbs))
-        self.d_bits = DeviceBuffer(mb * len(cbs))
+        self.d_rm = DeviceBuffer(rm_off)
+        self.d_msg = DeviceBuffer(msg_off)
+        self.d_bits = DeviceBuffer(msg_off)
+        # The decoder leaves the bits past K_b*Z in a last partial byte untouched.
+        self.d_bits.zero()
         self.d_res = DeviceBuffer(4 * len(cbs))
+        self.msg_bytes = msg_off
         for c0 in range(0, len(cbs), 4096):
             ctx.gen_codeblocks(gen[c0:c0 + 4096], self.d_cw, self.d_msg)
             ctx.rate_dematch(rms[c0:c0 + 4096], self.d_cw, self.d_rm, new_data=True)
-        self.info_bits = k_b(bg) * Z - 24
+        self.info = np.array(self.info)
         self.arr = (PuschCbDesc * len(self.pusch))(*self.pusch)
 
     def decode(self, ctx, learned=False, iters=ITERS, fb_iters=0, stream=None):
@@ -116,10 +128,28 @@ class DeviceBatch:
         """(block error flags, iterations) of the last decode: an error is a failed CRC or
         a passed CRC with wrong bits."""
         res = self.d_res.download(np.int32, self.n)
-        bits = self.d_bits.download(np.uint8, self.n * self.mb).reshape(self.n, self.mb)
-        msg = self.d_msg.download(np.uint8, self.n * self.mb).reshape(self.n, self.mb)
-        wrong = (bits != msg).any(axis=1)
+        bits = self.d_bits.download(np.uint8, self.msg_bytes)
+        msg = self.d_msg.download(np.uint8, self.msg_bytes)
+        wrong = np.array([not np.array_equal(bits[o:o + m], msg[o:o + m]) for o, m in zip(self.msg_off, self.mbs)])
         return (res <= 0) | wrong, res
+
+    def good_info_bits(self, err):
+        return int(self.info[~err].sum())
+
+
+class DeviceBatch(MixedBatch):
+    """Codeblocks of one base graph and lifting size, with the descriptors of the
+    training step as well."""
+
+    def __init__(self, ctx, bg, Z, cbs, seed):
+        """cbs: list of (E, Qm, Es/N0 dB)."""
+        super().__init__(ctx, [(bg, Z, E, qm, snr) for E, qm, snr in cbs], seed)
+        self.bg, self.Z = bg, Z
+        N = n_short(bg) * Z
+        mb = msg_bytes(bg, Z)
+        self.mb = mb
+        self.train = [TrainDesc(i * N, i * mb, N, Z, 0, layers_for(bg, Z, E)) for i, (E, qm, snr) in enumerate(cbs)]
+        self.info_bits = int(self.info[0]) if self.n else 0
 
 
 class Trainer:
@@ -205,9 +235,8 @@ def measure(ctx, batch, learned, iters=ITERS, fb_iters=0, reps=3):
     ms = batch.timed_decode(ctx, learned, iters, fb_iters, reps)
     err, res = batch.outcome()
     its = np.where(res > 0, res, iters)
-    good = int((~err).sum())
     return {"bler": float(err.mean()), "iters": float(its.mean()), "ms": ms,
-            "gbps": good * batch.info_bits / (ms * 1e-3) / 1e9}
+            "gbps": batch.good_info_bits(err) / (ms * 1e-3) / 1e9}
 
 
 def operating_point(ctx, bg, Z, E, qm, lo, hi, n=4096, seed=5, step=0.05, target=0.10, iters=ITERS):
