@@ -243,7 +243,11 @@ typedef struct lh_gen_desc {
   uint32_t Nref;
   uint32_t nof_filler_bits;
   uint32_t crc_poly;    /* CRC attached to the data bits: LH_CRC24A/B, LH_CRC16, or 0 */
+  uint32_t flags;       /* LH_GEN_MSG_INPUT: take the whole message from d_msgs + msg_offset */
+  uint32_t pad;
 } lh_gen_desc;
+
+#define LH_GEN_MSG_INPUT 1u
 
 /* Device data generator: the transmit chain the reference's chain test builds from
  * ldpc_encoder::encode (include/ocudu/phy/upper/channel_coding/ldpc/ldpc_encoder.h:41),
@@ -255,6 +259,47 @@ typedef struct lh_gen_desc {
  * mapped, noisy, max-log demapped codeword to d_llrs + llr_offset. */
 int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t* d_llrs, uint8_t* d_msgs,
                       void* stream);
+
+/* One transport block of lh_gen_transport_blocks. */
+typedef struct lh_gen_tb_desc {
+  uint64_t tb_offset;      /* TB bytes (tbs_bits / 8) written to d_tb */
+  uint64_t msg_offset;     /* message of its first codeblock in d_msgs (packed, MSB first) */
+  uint64_t seed;           /* TB data: splitmix64 stream */
+  uint32_t msg_stride;     /* bytes between consecutive codeblock messages */
+  uint32_t tbs_bits;       /* multiple of 8 */
+  uint32_t nof_cbs;        /* segmentation (lh_segment_rx) */
+  uint32_t segment_length; /* K_b * Z */
+  uint32_t nof_filler_bits;
+  uint32_t pad;
+} lh_gen_tb_desc;
+
+/* Device transport-block generator: random TB bytes, the TB CRC (CRC16 up to 3824 bits,
+ * else CRC24A) and the codeblock messages of the TX segmentation
+ * (ldpc_segmenter_tx::new_transmission + ldpc_segmenter_buffer::read_codeblock,
+ * include/ocudu/phy/upper/channel_coding/ldpc/ldpc_segmenter_tx.h:31,
+ * ldpc_segmenter_buffer.h), ready for lh_gen_codeblocks with LH_GEN_MSG_INPUT. */
+int lh_gen_transport_blocks(lh_ctx* ctx, const lh_gen_tb_desc* descs, unsigned n, uint8_t* d_tb, uint8_t* d_msgs,
+                            void* stream);
+
+/* Codeblock segmentation of a received transport block (host only, no context):
+ * ldpc_segmenter_rx::segment (lib/phy/upper/channel_coding/ldpc/ldpc_segmenter_rx_impl.cpp:33-85)
+ * with compute_nof_codeblocks / compute_lifting_size (include/ocudu/ran/sch/sch_segmentation.h:52-110)
+ * and compute_rm_length (ldpc_segmenter_helpers.h:63-76). */
+typedef struct lh_segmentation {
+  uint32_t nof_cbs;
+  uint32_t lifting_size;
+  uint32_t segment_length;  /* K_b * Z */
+  uint32_t nof_filler_bits;
+  uint32_t nof_tb_crc_bits; /* 16 or 24 */
+  uint32_t nof_cb_crc_bits; /* the codeblock CRC: 24 when C > 1, else the TB CRC's */
+  uint32_t cb_data_bits;    /* data bits per codeblock (lh_tb_desc.cb_data_bits) */
+  uint32_t zero_pad;
+} lh_segmentation;
+
+/* rm_lengths / cw_offsets receive nof_cbs entries (at most max_cbs). */
+int lh_segment_rx(unsigned base_graph, unsigned tbs_bits, unsigned modulation_order, unsigned nof_layers,
+                  unsigned nof_ch_symbols, lh_segmentation* seg, uint32_t* rm_lengths, uint32_t* cw_offsets,
+                  unsigned max_cbs);
 
 #ifdef __cplusplus
 }

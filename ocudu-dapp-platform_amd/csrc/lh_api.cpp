@@ -39,6 +39,8 @@ hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt,
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, int n_alpha, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t s);
 hipError_t launch_reset_headers(launch_header* hdr, hipStream_t s);
+hipError_t launch_gen_tb(const gen_tb_job* d_jobs, unsigned n, uint8_t* tb, uint8_t* msgs, const uint32_t* crc_x,
+                         const uint32_t* lut24a, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
 } // namespace lh
@@ -1066,6 +1068,10 @@ int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t*
     j.crc_idx = (uint8_t)ci;
     j.F = (uint16_t)d.nof_filler_bits;
     j.write_msg = d_msgs != nullptr ? 1 : 0;
+    j.msg_input = (d.flags & LH_GEN_MSG_INPUT) ? 1 : 0;
+    if (j.msg_input && d_msgs == nullptr) {
+      return fail(LH_ERR_INVALID, "cb %u: message input without a message buffer", i);
+    }
   }
   int rc = upload(ctx, s, n * sizeof(gen_job), st);
   if (rc != LH_OK) {
@@ -1200,6 +1206,45 @@ int lh_nms_adam(lh_ctx* ctx, float* d_params, const float* d_grads, float* d_m, 
   hipError_t e = launch_adam(d_params, d_grads, d_m, d_v, (int)n, (int)n_alpha, lr, beta1, beta2, eps, c1, c2,
                              reinterpret_cast<hipStream_t>(stream));
   return e == hipSuccess ? LH_OK : hip_fail(e, "adam launch");
+}
+
+int lh_gen_transport_blocks(lh_ctx* ctx, const lh_gen_tb_desc* descs, unsigned n, uint8_t* d_tb, uint8_t* d_msgs,
+                            void* stream)
+{
+  static_assert(sizeof(lh_gen_tb_desc) == sizeof(gen_tb_job), "TB generator descriptor layout");
+  if (ctx == nullptr || (n > 0 && (descs == nullptr || d_tb == nullptr || d_msgs == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_gen_transport_blocks");
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u transport blocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_gen_tb_desc& d = descs[i];
+    const unsigned C = d.nof_cbs, K = d.segment_length, F = d.nof_filler_bits;
+    const unsigned L = d.tbs_bits <= 3824 ? 16 : 24;
+    const bool kok = (K % 22 == 0 && lifting_position(K / 22) >= 0) || (K % 10 == 0 && lifting_position(K / 10) >= 0);
+    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || C == 0 || !kok || F >= K || d.msg_stride < (K + 7) / 8) {
+      return fail(LH_ERR_INVALID, "tb %u: invalid generator descriptor", i);
+    }
+    const uint64_t D = C > 1 ? (uint64_t)K - F - 24 : (uint64_t)K - F;
+    if (K - F <= (C > 1 ? 24u : 0u) || D * C < (uint64_t)d.tbs_bits + L || (C == 1 && D != (uint64_t)d.tbs_bits + L)) {
+      return fail(LH_ERR_INVALID, "tb %u: segmentation does not hold the TB and its CRC", i);
+    }
+  }
+  if (n == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  std::memcpy(s->host, descs, n * sizeof(lh_gen_tb_desc));
+  int rc = upload(ctx, s, n * sizeof(lh_gen_tb_desc), st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = launch_gen_tb(reinterpret_cast<const gen_tb_job*>(s->dev), n, d_tb, d_msgs, ctx->d_crc_x,
+                               ctx->d_tbcrc, ctx->d_tbcrc + 256, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "TB generator launch");
 }
 
 int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const uint32_t* nbits, unsigned n,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bg_tables.inc"
+#include "lh_gf.h"
 #include "lh_internal.h"
 
 #pragma clang fp contract(off)
@@ -112,19 +113,27 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   const int kd = K * Z - jb.F - order;
   const uint32_t* gt = gtab + jb.gt_off;
 
-  // 1. Message: data bits, CRC, zero fillers; the parity part starts zeroed.
-  for (int w = t; 64 * w < kd; w += NT) {
+  // 1. Message: data bits, CRC, zero fillers; the parity part starts zeroed.  With
+  //    msg_input the whole message (data, CRC, fillers) comes from msgs + msg_off.
+  if (jb.msg_input) {
+    const uint8_t* src = msgs + jb.msg_off;
+    for (int j = t; j < (K + M) * Z; j += NT) {
+      cw[j] = j < K * Z ? (uint8_t)((src[j >> 3] >> (7 - (j & 7))) & 1u) : (uint8_t)0;
+    }
+    __syncthreads();
+  }
+  for (int w = t; !jb.msg_input && 64 * w < kd; w += NT) {
     const uint64_t x = stream_at(jb.seed_msg, (uint64_t)w);
     const int n = min(64, kd - 64 * w);
     for (int b = 0; b < n; ++b) {
       cw[64 * w + b] = (uint8_t)((x >> (63 - b)) & 1u);
     }
   }
-  for (int j = kd + t; j < (K + M) * Z; j += NT) {
+  for (int j = kd + t; !jb.msg_input && j < (K + M) * Z; j += NT) {
     cw[j] = 0;
   }
   __syncthreads();
-  if (order != 0) {
+  if (order != 0 && !jb.msg_input) {
     const uint32_t* xt = crc_x + (jb.crc_idx - 1) * kCrcXLen + (kd - 1 + order);
     uint32_t x = 0;
     for (int j = t; j < kd; j += NT) {
@@ -236,7 +245,7 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   __syncthreads();
 
   // 4. Packed message bits (MSB first).
-  if (jb.write_msg) {
+  if (jb.write_msg && !jb.msg_input) {
     uint8_t* out = msgs + jb.msg_off;
     const int nbytes = (K * Z + 7) / 8;
     for (int b = t; b < nbytes; b += NT) {
@@ -306,6 +315,101 @@ __global__ void __launch_bounds__(384) k_gen_codeblocks(const gen_job* __restric
   } else {
     gen_codeblock<2>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
   }
+}
+
+// Transport-block generator: TB bytes from a splitmix64 stream (byte b = byte b mod 8,
+// most significant first, of output b / 8), the TB CRC (CRC16 up to 3824 bits, else
+// CRC24A), and the codeblock messages of the TX segmentation (TS38.212 5.2.2 as in
+// lib/phy/upper/channel_coding/ldpc/ldpc_segmenter_tx_impl.cpp:124-198): codeblock c
+// carries bits [c D, c D + D) of TB || TB CRC || zero padding, then its CRC24B when there
+// are several codeblocks, then F zero fillers.  One workgroup (256 threads) per TB; the TB
+// bytes go to tb (required: the codeblock bits are read back from there).
+__device__ __forceinline__ uint32_t tb_bit(const uint8_t* tb, uint32_t tbs, uint32_t crc, uint32_t L, uint32_t p)
+{
+  if (p < tbs) {
+    return (tb[p >> 3] >> (7 - (p & 7))) & 1u;
+  }
+  p -= tbs;
+  return p < L ? (crc >> (L - 1 - p)) & 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_gen_tb(const gen_tb_job* __restrict__ jobs, uint8_t* __restrict__ tb_base,
+                                                uint8_t* __restrict__ msgs, const uint32_t* __restrict__ crc_x,
+                                                const uint32_t* __restrict__ lut24a, const uint32_t* __restrict__ pow2)
+{
+  __shared__ uint32_t red[16];
+  __shared__ uint32_t lut[256];
+  const gen_tb_job jb = jobs[blockIdx.x];
+  const int t = (int)threadIdx.x, NT = (int)blockDim.x;
+  uint8_t* tb = tb_base + jb.tb_off;
+  const uint32_t nbytes = jb.tbs_bits / 8;
+  lut[t] = lut24a[t];
+  for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)NT) {
+    const uint64_t w = stream_at(jb.seed, b >> 3);
+    tb[b] = (uint8_t)(w >> (56 - 8 * (b & 7)));
+  }
+  __threadfence_block();
+  __syncthreads();
+  // TB CRC.
+  const bool crc16 = jb.tbs_bits <= 3824;
+  const uint32_t L = crc16 ? 16u : 24u;
+  uint32_t x = 0;
+  if (crc16) {
+    const uint32_t* xt = crc_x + 2 * kCrcXLen + (jb.tbs_bits - 1 + 16);
+    for (uint32_t i = (uint32_t)t; i < jb.tbs_bits; i += (uint32_t)NT) {
+      x ^= ((tb[i >> 3] >> (7 - (i & 7))) & 1u) ? xt[-(int)i] : 0u;
+    }
+  } else {
+    const uint32_t chunk = (nbytes + (uint32_t)NT - 1) / (uint32_t)NT;
+    const uint32_t a0 = (uint32_t)t * chunk, a1 = min(nbytes, a0 + chunk);
+    uint32_t rem = 0;
+    for (uint32_t b = a0; b < a1; ++b) {
+      rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ tb[b]) & 0xffu];
+    }
+    if (a0 < a1) {
+      x = gf_mulmod(rem, gf_xpow(8u * (nbytes - a1), pow2, 0x1864cfbu, 24), 0x1864cfbu, 24);
+    }
+  }
+  const uint32_t tbcrc = block_xor(x, red);
+  // Codeblock messages.
+  const uint32_t C = jb.nof_cbs, K = jb.segment_length, F = jb.nof_filler_bits;
+  const uint32_t D = C > 1 ? K - F - 24 : K - F;
+  const uint32_t kbytes = (K + 7) / 8;
+  const uint32_t* xb = crc_x + kCrcXLen + (D - 1 + 24); // CRC24B weights
+  for (uint32_t c = 0; c < C; ++c) {
+    uint8_t* m = msgs + jb.msg_off + (size_t)c * jb.msg_stride;
+    uint32_t cbcrc = 0;
+    if (C > 1) {
+      uint32_t y = 0;
+      for (uint32_t i = (uint32_t)t; i < D; i += (uint32_t)NT) {
+        y ^= tb_bit(tb, jb.tbs_bits, tbcrc, L, c * D + i) ? xb[-(int)i] : 0u;
+      }
+      __syncthreads(); // red is reused
+      cbcrc = block_xor(y, red);
+    }
+    for (uint32_t b = (uint32_t)t; b < kbytes; b += (uint32_t)NT) {
+      uint32_t v = 0;
+      for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t i = 8 * b + k;
+        uint32_t bit = 0;
+        if (i < D) {
+          bit = tb_bit(tb, jb.tbs_bits, tbcrc, L, c * D + i);
+        } else if (C > 1 && i < D + 24) {
+          bit = (cbcrc >> (23 - (i - D))) & 1u;
+        }
+        v |= bit << (7 - k);
+      }
+      m[b] = (uint8_t)v;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_gen_tb(const gen_tb_job* d_jobs, unsigned n, uint8_t* tb, uint8_t* msgs, const uint32_t* crc_x,
+                         const uint32_t* lut24a, const uint32_t* pow2, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gen_tb, dim3(n), dim3(256), 0, s, d_jobs, tb, msgs, crc_x, lut24a, pow2);
+  return hipGetLastError();
 }
 
 hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,

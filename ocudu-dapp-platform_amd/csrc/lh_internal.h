@@ -100,11 +100,25 @@ struct alignas(16) gen_job {
   uint8_t crc_idx;     // 0 none, 1 CRC24A, 2 CRC24B, 3 CRC16
   uint16_t F;
   uint8_t write_msg;
-  uint8_t pad0;
+  uint8_t msg_input; // the message is read from msgs + msg_off (data, CRC and fillers)
   uint32_t pad1[5];
   rm_params rm;
 };
 static_assert(sizeof(gen_job) == 128, "gen_job layout");
+
+// One transport block of the TB generator (same layout as lh_gen_tb_desc).
+struct alignas(8) gen_tb_job {
+  uint64_t tb_off;
+  uint64_t msg_off;
+  uint64_t seed;
+  uint32_t msg_stride;
+  uint32_t tbs_bits;
+  uint32_t nof_cbs;
+  uint32_t segment_length;
+  uint32_t nof_filler_bits;
+  uint32_t pad;
+};
+static_assert(sizeof(gen_tb_job) == 48, "gen_tb_job layout");
 
 // One codeblock of a learned-decoder training batch (48 bytes).
 struct alignas(16) train_job {

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bg_tables.inc"
+#include "lh_gf.h"
 #include "lh_internal.h"
 
 namespace lh {
@@ -796,34 +797,6 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 }
 
 // GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
-__device__ __forceinline__ uint32_t gf_mulx(uint32_t v, uint32_t poly, int order)
-{
-  v <<= 1;
-  return (v >> order) & 1u ? v ^ poly : v;
-}
-__device__ uint32_t gf_mulmod(uint32_t a, uint32_t b, uint32_t poly, int order)
-{
-  uint32_t res = 0;
-  for (int i = order - 1; i >= 0; --i) {
-    res = gf_mulx(res, poly, order);
-    if ((b >> i) & 1u) {
-      res ^= a;
-    }
-  }
-  return res;
-}
-// x^n mod G from the table of x^(2^k) mod G.
-__device__ uint32_t gf_xpow(uint32_t n, const uint32_t* pow2, uint32_t poly, int order)
-{
-  uint32_t res = 1;
-  for (int k = 0; n != 0; ++k, n >>= 1) {
-    if (n & 1u) {
-      res = gf_mulmod(res, pow2[k], poly, order);
-    }
-  }
-  return res;
-}
-
 // TB reassembly + TB CRC24A, one workgroup per transport block (pusch_decoder_impl.cpp:612-781).
 __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
                                                      const uint8_t* __restrict__ cb_bits,

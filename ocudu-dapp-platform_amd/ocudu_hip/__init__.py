@@ -175,7 +175,22 @@ class GenDesc(C.Structure):
     _fields_ = [("llr_offset", C.c_uint64), ("msg_offset", C.c_uint64), ("seed_msg", C.c_uint64),
                 ("seed_noise", C.c_uint64), ("n0", C.c_double), ("rm_length", C.c_uint32), ("base_graph", C.c_uint32),
                 ("lifting_size", C.c_uint32), ("rv", C.c_uint32), ("modulation_order", C.c_uint32),
-                ("Nref", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("crc_poly", C.c_uint32)]
+                ("Nref", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("crc_poly", C.c_uint32),
+                ("flags", C.c_uint32), ("pad", C.c_uint32)]
+
+
+LH_GEN_MSG_INPUT = 1
+
+
+class GenTbDesc(C.Structure):
+    _fields_ = [("tb_offset", C.c_uint64), ("msg_offset", C.c_uint64), ("seed", C.c_uint64),
+                ("msg_stride", C.c_uint32), ("tbs_bits", C.c_uint32), ("nof_cbs", C.c_uint32),
+                ("segment_length", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class Segmentation(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("nof_cbs", "lifting_size", "segment_length", "nof_filler_bits",
+                                           "nof_tb_crc_bits", "nof_cb_crc_bits", "cb_data_bits", "zero_pad")]
 
 
 class TrainDesc(C.Structure):
@@ -186,7 +201,8 @@ class TrainDesc(C.Structure):
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
-            "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set"]
+            "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
+            "lh_gen_transport_blocks", "lh_segment_rx"]
 
 
 def lib():
@@ -227,6 +243,10 @@ def lib():
         L.lh_slot_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
         L.lh_slot_graph_destroy.argtypes = [C.c_void_p]
         L.lh_tune_set.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_int]
+        L.lh_gen_transport_blocks.argtypes = [C.c_void_p, C.POINTER(GenTbDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
+        L.lh_segment_rx.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(Segmentation),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -333,6 +353,11 @@ class Context:
     def tune_set(self, bg, Z, groups, stage_llrs=True):
         _lh(lib().lh_tune_set(self.handle, bg, Z, groups, int(stage_llrs)), "lh_tune_set")
 
+    def gen_transport_blocks(self, descs, d_tb, d_msgs, stream=None):
+        arr = (GenTbDesc * len(descs))(*descs)
+        _lh(lib().lh_gen_transport_blocks(self.handle, arr, len(descs), d_tb.ptr, d_msgs.ptr,
+                                          stream.handle if stream else None), "lh_gen_transport_blocks")
+
     def crc_calculate(self, poly, offsets, nbits, d_bits, d_out, stream=None):
         n = len(offsets)
         offs = (C.c_uint64 * n)(*offsets)
@@ -368,6 +393,16 @@ class SlotGraph:
             self.close()
         except Exception:
             pass
+
+
+def segment_rx(bg, tbs_bits, qm, nof_layers, nof_ch_symbols, max_cbs=512):
+    """(Segmentation, rm lengths, cw offsets) of a received transport block (lh_segment_rx)."""
+    seg = Segmentation()
+    rm = (C.c_uint32 * max_cbs)()
+    off = (C.c_uint32 * max_cbs)()
+    _lh(lib().lh_segment_rx(bg, tbs_bits, qm, nof_layers, nof_ch_symbols, C.byref(seg), rm, off, max_cbs),
+        "lh_segment_rx")
+    return seg, list(rm[:seg.nof_cbs]), list(off[:seg.nof_cbs])
 
 
 def k_b(bg):

@@ -33,7 +33,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(H.RmDesc) == 8 + 8 + 7 * 4 + 4
     assert ctypes.sizeof(H.PuschCbDesc) == ctypes.sizeof(H.RmDesc) + 8 + 8
     assert ctypes.sizeof(H.TbDesc) == 40
-    assert ctypes.sizeof(H.GenDesc) == 5 * 8 + 8 * 4
+    assert ctypes.sizeof(H.GenDesc) == 5 * 8 + 10 * 4
+    assert ctypes.sizeof(H.GenTbDesc) == 48
+    assert ctypes.sizeof(H.Segmentation) == 32
     assert ctypes.sizeof(H.TrainDesc) == 2 * 8 + 4 * 4
 
 

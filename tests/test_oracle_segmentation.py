@@ -74,3 +74,29 @@ def test_tb_concat_inverts_reference_tx(ref, case):
         pb = np.stack([np.packbits(b) for b in bad])
         _, r = O.tb_concat_crc(pb.reshape(-1), stride, tbs, seg)
         assert r != 0, (c, j)
+
+
+def test_product_segmentation_matches_oracle():
+    """lh_segment_rx (the C-ABI's host segmentation) against the oracle's restatement over
+    a sweep of sizes, layers and modulations (the oracle is pinned to the reference above)."""
+    import ocudu_hip as H
+    rng = np.random.default_rng(17)
+    cases = list(CASES)
+    for _ in range(300):
+        bg = int(rng.integers(1, 3))
+        tbs = int(rng.integers(1, 159000))
+        qm = int(rng.choice([2, 4, 6, 8]))
+        nl = int(rng.integers(1, 5))
+        ns = nl * int(rng.integers(1, 60000))
+        cases.append((bg, tbs, qm, nl, ns))
+    for bg, tbs, qm, nl, ns in cases:
+        seg, rm, off = H.segment_rx(bg, 8 * tbs, qm, nl, ns)
+        if seg.nof_cbs > 256:  # beyond the oracle's arrays (BG2 at the largest sizes)
+            continue
+        mo, oseg = O.segment_rx(bg, tbs, qm, nl, ns)
+        assert seg.nof_cbs == len(mo) and seg.lifting_size == oseg.Z, (bg, tbs)
+        assert seg.segment_length == oseg.segment_length and seg.nof_filler_bits == oseg.nof_filler_bits
+        assert seg.nof_cb_crc_bits == oseg.nof_crc_bits and seg.nof_tb_crc_bits == oseg.nof_tb_crc_bits
+        assert rm == [m["rm_length"] for m in mo] and off == [m["cw_offset"] for m in mo]
+        if seg.nof_cbs > 1:
+            assert seg.cb_data_bits == cb_data_bits(oseg)
