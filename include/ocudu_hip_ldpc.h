@@ -301,6 +301,76 @@ int lh_segment_rx(unsigned base_graph, unsigned tbs_bits, unsigned modulation_or
                   unsigned nof_ch_symbols, lh_segmentation* seg, uint32_t* rm_lengths, uint32_t* cw_offsets,
                   unsigned max_cbs);
 
+/* ---- Slot-deadline serving runtime (the resident batch path of the reference:
+ * pusch_resident_codeblock_decoder::decode_resident_softbits,
+ * include/ocudu/phy/upper/channel_processors/pusch/pusch_resident_codeblock_decoder.h:125-149,
+ * HARQ soft buffers kept on the device as rx_buffer_pool does on the host). ----
+ * Transport-block requests are admitted when their HARQ buffers fit the HBM budget (a
+ * retransmission combines into its process's buffers), batched continuously into slot
+ * launches (decode + TB reassembly) on up to max_slots_in_flight streams whose work
+ * overlaps on the device, and completed asynchronously.  A TB whose CRC passes releases its
+ * HARQ buffers; a failed one keeps them for a retransmission (lh_server_harq_release drops
+ * them).  Single-threaded: one caller drives submit / step / poll. */
+typedef struct lh_server lh_server;
+
+typedef struct lh_server_config {
+  size_t harq_budget_bytes;        /* device memory for HARQ soft buffers (N_short*Z per codeblock) */
+  unsigned max_cbs_per_slot;       /* codeblocks per slot launch (<= the context capacity) */
+  unsigned max_slots_in_flight;    /* 1..8 */
+  unsigned nof_ldpc_iterations;
+  unsigned nof_fallback_iterations;/* learned decoder: plain fallback iterations (0 = none) */
+  int learned;                     /* decode with the learned tables */
+} lh_server_config;
+
+typedef struct lh_tb_request {
+  uint64_t id;              /* caller's tag */
+  const int8_t* d_llrs;     /* codeword LLRs on the device: nof_ch_symbols * modulation_order */
+  uint32_t harq_process;    /* HARQ buffer key */
+  int32_t new_data;         /* 1: new transmission; 0: retransmission combined into the buffers */
+  uint32_t base_graph;
+  uint32_t tbs_bits;
+  uint32_t modulation_order;
+  uint32_t nof_layers;
+  uint32_t nof_ch_symbols;
+  uint32_t rv;
+  uint32_t Nref;
+  uint32_t pad;
+} lh_tb_request;
+
+#define LH_TB_REJECTED (-2) /* a retransmission without matching HARQ buffers */
+
+typedef struct lh_tb_result {
+  uint64_t id;
+  uint32_t harq_process;
+  int32_t tb_crc_ok;        /* LH_TB_CRC_OK / LH_TB_CB_FAILED / LH_TB_CRC_FAILED / LH_TB_REJECTED */
+  uint32_t nof_cbs;
+  uint32_t nof_cbs_ok;
+  uint32_t max_iterations;  /* over the codeblocks that passed */
+  uint32_t pad;
+  double latency_us;        /* submit -> completion seen by poll */
+} lh_tb_result;
+
+typedef struct lh_server_stats_t {
+  uint64_t harq_free_bytes;
+  uint32_t harq_processes;
+  uint32_t pending;         /* waiting for HARQ memory or for their process */
+  uint32_t ready;           /* admitted, not launched */
+  uint32_t slots_in_flight;
+} lh_server_stats_t;
+
+int lh_server_create(lh_ctx* ctx, const lh_server_config* cfg, lh_server** server);
+int lh_server_destroy(lh_server* server);
+/* Queues a request (its LLRs must stay valid until its result is polled); tb_out (host,
+ * tbs_bits / 8 bytes, may be NULL) receives the TB when its CRC passes. */
+int lh_server_submit(lh_server* server, const lh_tb_request* req, uint8_t* tb_out);
+/* Launches one slot of the admitted requests when a slot is free (non-blocking). */
+int lh_server_step(lh_server* server);
+/* Retires completed slots and returns up to max results; wait != 0 blocks until one
+ * in-flight slot completes when no result is ready. */
+int lh_server_poll(lh_server* server, lh_tb_result* out, unsigned max, unsigned* n_out, int wait);
+int lh_server_harq_release(lh_server* server, uint32_t harq_process);
+int lh_server_stats(lh_server* server, lh_server_stats_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

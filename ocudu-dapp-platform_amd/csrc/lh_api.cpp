@@ -8,6 +8,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <unordered_map>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -122,7 +126,7 @@ void crc_x_table(uint32_t poly, uint32_t* out)
   }
 }
 
-constexpr int kRing = 4;
+constexpr int kRing = 16;
 
 // Tuned launch parameters of one (BG, Z): codeblocks per workgroup (0 = as many as fit)
 // and LDS staging of the codeword LLRs (1 = when it fits, 0 = never).
@@ -166,6 +170,13 @@ slot* acquire_slot(lh_ctx* ctx)
     (void)hipEventSynchronize(s->done);
   }
   return s;
+}
+
+// Marks the slot busy until the work enqueued on `stream` so far (the kernels that read
+// its descriptors) has completed.
+void retire(slot* s, hipStream_t stream)
+{
+  (void)hipEventRecord(s->done, stream);
 }
 
 int upload(lh_ctx* ctx, slot* s, size_t bytes, hipStream_t stream)
@@ -441,6 +452,7 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
     return rc;
   }
   hipError_t e = launch_block(ctx, pb, s->dev + off, rm, cw, bits, results, stream, scale_tab, fb_iters);
+  retire(s, stream);
   return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
@@ -666,6 +678,7 @@ int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8
     return rc;
   }
   hipError_t e = launch_rate_dematch(reinterpret_cast<const rm_params*>(s->dev), n, maxN, d_cw, d_rm, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "dematcher launch");
 }
 
@@ -674,14 +687,16 @@ int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8
 namespace {
 
 // Decoder jobs of a PUSCH batch (fused dematching + decoding, HARQ write-back).
-int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data, int use_early_stop,
-               unsigned nof_ldpc_iterations, launch_plan& plan)
+// new_data_per_cb (optional) overrides new_data per codeblock.
+int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data_all, int use_early_stop,
+               unsigned nof_ldpc_iterations, launch_plan& plan, const uint8_t* new_data_per_cb = nullptr)
 {
   if (nof_ldpc_iterations == 0 || nof_ldpc_iterations > 255) {
     return fail(LH_ERR_INVALID, "invalid number of LDPC iterations %u", nof_ldpc_iterations);
   }
   for (unsigned i = 0; i < n; ++i) {
     const lh_pusch_cb_desc& d = cbs[i];
+    const int new_data = new_data_per_cb ? new_data_per_cb[i] : new_data_all;
     uint64_t mp = 0;
     dec_job j{};
     int rc = make_rm_params(d.rm, new_data != 0, j.rm, mp, i);
@@ -1007,6 +1022,7 @@ int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t
   }
   hipError_t e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), n, d_cb_bits, d_cb_results, d_tb,
                                     d_tb_status, ctx->d_tbcrc, ctx->d_tbcrc + 256, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "TB finalize launch");
 }
 
@@ -1079,6 +1095,7 @@ int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t*
   }
   hipError_t e = launch_gen_codeblocks(reinterpret_cast<const gen_job*>(s->dev), n, d_llrs, d_msgs, ctx->d_crc_x,
                                        ctx->d_gtab, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "generator launch");
 }
 
@@ -1191,6 +1208,7 @@ int lh_nms_train_step(lh_ctx* ctx, unsigned base_graph, const lh_train_desc* des
                                   d_alpha, d_beta, reinterpret_cast<const float*>(s->dev + o_w), tau, d_llrs, d_msgs,
                                   static_cast<float*>(d_workspace), d_loss, d_grad_alpha, d_grad_beta, ctx->d_gtab,
                                   backward, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "training launch");
 }
 
@@ -1244,6 +1262,7 @@ int lh_gen_transport_blocks(lh_ctx* ctx, const lh_gen_tb_desc* descs, unsigned n
   }
   hipError_t e = launch_gen_tb(reinterpret_cast<const gen_tb_job*>(s->dev), n, d_tb, d_msgs, ctx->d_crc_x,
                                ctx->d_tbcrc, ctx->d_tbcrc + 256, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "TB generator launch");
 }
 
@@ -1278,7 +1297,484 @@ int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const 
   const int order = ci == 3 ? 16 : 24;
   hipError_t e = launch_crc(reinterpret_cast<const uint64_t*>(s->dev), reinterpret_cast<const uint32_t*>(s->dev + n * 8),
                             n, d_bits, ctx->d_crc_x + (ci - 1) * kCrcXLen, order, d_out, st);
+  retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "CRC launch");
+}
+
+} // extern "C"
+
+// ===================================================================================
+// Slot-deadline serving runtime
+// ===================================================================================
+
+namespace {
+
+double now_us()
+{
+  return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct harq_entry {
+  uint64_t off = 0, bytes = 0;
+  uint32_t nof_cbs = 0, Z = 0, bg = 0, tbs = 0;
+  bool busy = false;
+};
+
+struct srv_request {
+  lh_tb_request req{};
+  lh_segmentation seg{};
+  std::vector<uint32_t> rm, off;
+  uint64_t harq_off = 0;
+  double t_submit = 0;
+};
+
+struct srv_slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  uint8_t* d_bits = nullptr;
+  int32_t* d_res = nullptr;
+  uint8_t* d_tb = nullptr;
+  int32_t* d_st = nullptr;
+  int32_t* h_res = nullptr;
+  int32_t* h_st = nullptr;
+  uint8_t* h_tb = nullptr;
+  std::vector<srv_request> reqs;
+  std::vector<uint32_t> first_cb, tb_off;
+  bool busy = false;
+};
+
+// First-fit allocator over the HARQ arena (offsets), coalescing on free.
+struct arena_alloc {
+  std::map<uint64_t, uint64_t> free_; // offset -> bytes
+  explicit arena_alloc(uint64_t bytes = 0) { if (bytes) free_[0] = bytes; }
+  bool alloc(uint64_t bytes, uint64_t& off)
+  {
+    bytes = (bytes + 255) & ~(uint64_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        off = it->first;
+        const uint64_t rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) {
+          free_[off + bytes] = rest;
+        }
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t bytes)
+  {
+    bytes = (bytes + 255) & ~(uint64_t)255;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  uint64_t free_bytes() const
+  {
+    uint64_t f = 0;
+    for (auto& kv : free_) {
+      f += kv.second;
+    }
+    return f;
+  }
+};
+
+constexpr unsigned kMaxMsgBytes = (22 * 384) / 8;
+
+} // namespace
+
+struct lh_server {
+  lh_ctx* ctx = nullptr;
+  lh_server_config cfg{};
+  int8_t* d_arena = nullptr;
+  arena_alloc arena;
+  std::unordered_map<uint32_t, harq_entry> harq;
+  std::deque<srv_request> pending;
+  std::deque<srv_request> ready;
+  std::vector<srv_slot> slots;
+  std::deque<lh_tb_result> results;
+  std::unordered_map<uint64_t, uint8_t*> tb_dest; // host TB destinations by request id
+  unsigned next_slot = 0;
+};
+
+namespace {
+
+bool harq_matches(const harq_entry& h, const srv_request& r)
+{
+  return h.nof_cbs == r.seg.nof_cbs && h.Z == r.seg.lifting_size && h.bg == r.req.base_graph &&
+         h.tbs == r.req.tbs_bits;
+}
+
+void push_result(lh_server* sv, const srv_request& r, int32_t status, unsigned ok, unsigned iters)
+{
+  lh_tb_result res{};
+  res.id = r.req.id;
+  res.harq_process = r.req.harq_process;
+  res.tb_crc_ok = status;
+  res.nof_cbs = r.seg.nof_cbs;
+  res.nof_cbs_ok = ok;
+  res.max_iterations = iters;
+  res.latency_us = now_us() - r.t_submit;
+  sv->results.push_back(res);
+}
+
+// Admits pending requests whose HARQ buffers can be placed (FIFO; a request whose process
+// is in flight waits, later requests may pass it).
+void admit(lh_server* sv)
+{
+  for (auto it = sv->pending.begin(); it != sv->pending.end();) {
+    srv_request& r = *it;
+    auto h = sv->harq.find(r.req.harq_process);
+    const uint64_t bytes = (uint64_t)r.seg.nof_cbs * (r.req.base_graph == 1 ? 66u : 50u) * r.seg.lifting_size;
+    if (h != sv->harq.end() && h->second.busy) {
+      ++it;
+      continue;
+    }
+    if (!r.req.new_data) {
+      if (h == sv->harq.end() || !harq_matches(h->second, r)) {
+        push_result(sv, r, LH_TB_REJECTED, 0, 0);
+        it = sv->pending.erase(it);
+        continue;
+      }
+    } else if (h != sv->harq.end() && !harq_matches(h->second, r)) {
+      sv->arena.release(h->second.off, h->second.bytes);
+      sv->harq.erase(h);
+      h = sv->harq.end();
+    }
+    if (h == sv->harq.end()) {
+      uint64_t off = 0;
+      if (!sv->arena.alloc(bytes, off)) {
+        ++it; // waits for memory
+        continue;
+      }
+      harq_entry e;
+      e.off = off;
+      e.bytes = bytes;
+      e.nof_cbs = r.seg.nof_cbs;
+      e.Z = r.seg.lifting_size;
+      e.bg = r.req.base_graph;
+      e.tbs = r.req.tbs_bits;
+      h = sv->harq.emplace(r.req.harq_process, e).first;
+    }
+    h->second.busy = true;
+    r.harq_off = h->second.off;
+    sv->ready.push_back(std::move(r));
+    it = sv->pending.erase(it);
+  }
+}
+
+// Retires a completed slot: results, HARQ release on TB CRC pass, TB bytes to the caller.
+void retire_slot(lh_server* sv, srv_slot& sl)
+{
+  for (size_t k = 0; k < sl.reqs.size(); ++k) {
+    const srv_request& r = sl.reqs[k];
+    const int32_t st = sl.h_st[k];
+    unsigned ok = 0, its = 0;
+    for (uint32_t c = 0; c < r.seg.nof_cbs; ++c) {
+      const int32_t x = sl.h_res[sl.first_cb[k] + c];
+      ok += x > 0 ? 1u : 0u;
+      its = std::max<unsigned>(its, x > 0 ? (unsigned)x : 0u);
+    }
+    auto dst = sv->tb_dest.find(r.req.id);
+    if (dst != sv->tb_dest.end()) {
+      if (st == LH_TB_CRC_OK && dst->second != nullptr) {
+        std::memcpy(dst->second, sl.h_tb + sl.tb_off[k], r.req.tbs_bits / 8);
+      }
+      sv->tb_dest.erase(dst);
+    }
+    auto h = sv->harq.find(r.req.harq_process);
+    if (h != sv->harq.end()) {
+      h->second.busy = false;
+      if (st == LH_TB_CRC_OK) {
+        sv->arena.release(h->second.off, h->second.bytes);
+        sv->harq.erase(h);
+      }
+    }
+    push_result(sv, r, st, ok, its);
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
+} // namespace
+
+extern "C" {
+
+int lh_server_create(lh_ctx* ctx, const lh_server_config* cfg, lh_server** out)
+{
+  if (ctx == nullptr || cfg == nullptr || out == nullptr || cfg->max_cbs_per_slot == 0 ||
+      cfg->max_cbs_per_slot > ctx->max_cb || cfg->max_slots_in_flight == 0 || cfg->max_slots_in_flight > 8 ||
+      cfg->harq_budget_bytes < 25344 || cfg->nof_ldpc_iterations == 0 || cfg->nof_ldpc_iterations > 255 ||
+      cfg->nof_fallback_iterations > 255) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_create");
+  }
+  (void)hipSetDevice(ctx->device);
+  lh_server* sv = new lh_server();
+  sv->ctx = ctx;
+  sv->cfg = *cfg;
+  hipError_t e = hipMalloc(&sv->d_arena, cfg->harq_budget_bytes);
+  sv->arena = arena_alloc(cfg->harq_budget_bytes);
+  const size_t ncb = cfg->max_cbs_per_slot;
+  sv->slots.resize(cfg->max_slots_in_flight);
+  for (auto& sl : sv->slots) {
+    if (e == hipSuccess) e = hipStreamCreate(&sl.stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_bits, ncb * kMaxMsgBytes);
+    if (e == hipSuccess) e = hipMemset(sl.d_bits, 0, ncb * kMaxMsgBytes);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_res, ncb * 4);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_tb, ncb * kMaxMsgBytes);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_st, ncb * 4);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_res, ncb * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_st, ncb * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_tb, ncb * kMaxMsgBytes, hipHostMallocDefault);
+  }
+  if (e != hipSuccess) {
+    lh_server_destroy(sv);
+    return hip_fail(e, "server allocation");
+  }
+  *out = sv;
+  return LH_OK;
+}
+
+int lh_server_destroy(lh_server* sv)
+{
+  if (sv == nullptr) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(sv->ctx->device);
+  for (auto& sl : sv->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.d_bits) (void)hipFree(sl.d_bits);
+    if (sl.d_res) (void)hipFree(sl.d_res);
+    if (sl.d_tb) (void)hipFree(sl.d_tb);
+    if (sl.d_st) (void)hipFree(sl.d_st);
+    if (sl.h_res) (void)hipHostFree(sl.h_res);
+    if (sl.h_st) (void)hipHostFree(sl.h_st);
+    if (sl.h_tb) (void)hipHostFree(sl.h_tb);
+  }
+  if (sv->d_arena) (void)hipFree(sv->d_arena);
+  delete sv;
+  return LH_OK;
+}
+
+int lh_server_submit(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out)
+{
+  if (sv == nullptr || req == nullptr || req->d_llrs == nullptr || (req->base_graph != 1 && req->base_graph != 2)) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_submit");
+  }
+  srv_request r;
+  r.req = *req;
+  r.rm.resize(512);
+  r.off.resize(512);
+  int rc = lh_segment_rx(req->base_graph, req->tbs_bits, req->modulation_order, req->nof_layers, req->nof_ch_symbols,
+                         &r.seg, r.rm.data(), r.off.data(), 512);
+  if (rc != LH_OK) {
+    return fail(rc, "request %llu: invalid transport block", (unsigned long long)req->id);
+  }
+  r.rm.resize(r.seg.nof_cbs);
+  r.off.resize(r.seg.nof_cbs);
+  const uint64_t bytes = (uint64_t)r.seg.nof_cbs * (req->base_graph == 1 ? 66u : 50u) * r.seg.lifting_size;
+  if (r.seg.nof_cbs > sv->cfg.max_cbs_per_slot || bytes > sv->cfg.harq_budget_bytes || req->rv > 3) {
+    return fail(LH_ERR_CAPACITY, "request %llu: %u codeblocks cannot be served", (unsigned long long)req->id,
+                r.seg.nof_cbs);
+  }
+  r.t_submit = now_us();
+  sv->tb_dest[req->id] = tb_out;
+  sv->pending.push_back(std::move(r));
+  admit(sv);
+  return LH_OK;
+}
+
+int lh_server_step(lh_server* sv)
+{
+  if (sv == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid server");
+  }
+  admit(sv);
+  if (sv->ready.empty()) {
+    return LH_OK;
+  }
+  // A free slot (launch resources); none free: the work waits for the next step.
+  srv_slot* sl = nullptr;
+  for (unsigned k = 0; k < sv->slots.size(); ++k) {
+    srv_slot& c = sv->slots[(sv->next_slot + k) % sv->slots.size()];
+    if (!c.busy) {
+      sl = &c;
+      sv->next_slot = (unsigned)((&c - sv->slots.data() + 1) % sv->slots.size());
+      break;
+    }
+  }
+  if (sl == nullptr) {
+    return LH_OK;
+  }
+  // Continuous batching: everything admitted since the last launch, up to the slot size.
+  unsigned ncb = 0;
+  while (!sv->ready.empty() && ncb + sv->ready.front().seg.nof_cbs <= sv->cfg.max_cbs_per_slot) {
+    ncb += sv->ready.front().seg.nof_cbs;
+    sl->reqs.push_back(std::move(sv->ready.front()));
+    sv->ready.pop_front();
+  }
+  const int8_t* cw_base = sl->reqs[0].req.d_llrs;
+  for (auto& r : sl->reqs) {
+    cw_base = std::min(cw_base, r.req.d_llrs);
+  }
+  std::vector<lh_pusch_cb_desc> cbs;
+  std::vector<uint8_t> nd;
+  std::vector<lh_tb_desc> tbs;
+  sl->first_cb.clear();
+  sl->tb_off.clear();
+  uint64_t bits = 0, tbo = 0;
+  for (auto& r : sl->reqs) {
+    const lh_tb_request& q = r.req;
+    const unsigned C = r.seg.nof_cbs, Z = r.seg.lifting_size, N = (q.base_graph == 1 ? 66u : 50u) * Z;
+    const unsigned mb = ((q.base_graph == 1 ? 22u : 10u) * Z + 7) / 8;
+    const uint32_t poly = C > 1 ? LH_CRC24B : (q.tbs_bits <= 3824 ? LH_CRC16 : LH_CRC24A);
+    sl->first_cb.push_back((uint32_t)cbs.size());
+    sl->tb_off.push_back((uint32_t)tbo);
+    lh_tb_desc t{};
+    t.tb_offset = tbo;
+    t.cb_bits_offset = bits;
+    t.cb_bits_stride = mb;
+    t.tbs_bits = q.tbs_bits;
+    t.first_cb = (uint32_t)cbs.size();
+    t.nof_cbs = C;
+    t.cb_data_bits = r.seg.cb_data_bits;
+    tbs.push_back(t);
+    for (unsigned c = 0; c < C; ++c) {
+      lh_pusch_cb_desc d{};
+      d.rm.cw_offset = (uint64_t)(q.d_llrs - cw_base) + r.off[c];
+      d.rm.rm_offset = r.harq_off + (uint64_t)c * N;
+      d.rm.rm_length = r.rm[c];
+      d.rm.base_graph = q.base_graph;
+      d.rm.lifting_size = Z;
+      d.rm.rv = q.rv;
+      d.rm.modulation_order = q.modulation_order;
+      d.rm.Nref = q.Nref;
+      d.rm.nof_filler_bits = r.seg.nof_filler_bits;
+      d.bits_offset = bits + (uint64_t)c * mb;
+      d.nof_crc_bits = r.seg.nof_cb_crc_bits;
+      d.crc_poly = poly;
+      cbs.push_back(d);
+      nd.push_back(q.new_data ? 1 : 0);
+    }
+    bits += (uint64_t)C * mb;
+    tbo += q.tbs_bits / 8;
+  }
+  launch_plan plan;
+  int rc = pusch_jobs(cbs.data(), (unsigned)cbs.size(), 1, 1, sv->cfg.nof_ldpc_iterations, plan, nd.data());
+  if (rc != LH_OK) {
+    return rc;
+  }
+  lh_ctx* ctx = sv->ctx;
+  (void)hipSetDevice(ctx->device);
+  slot* s = acquire_slot(ctx);
+  rc = run_plan(ctx, plan, sv->d_arena, cw_base, sl->d_bits, sl->d_res, sl->stream, s, 0,
+                sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0);
+  if (rc == LH_OK) {
+    rc = lh_tb_finalize(ctx, tbs.data(), (unsigned)tbs.size(), sl->d_bits, sl->d_res, sl->d_tb, sl->d_st, sl->stream);
+  }
+  hipError_t e = hipSuccess;
+  if (rc == LH_OK) {
+    e = hipMemcpyAsync(sl->h_res, sl->d_res, cbs.size() * 4, hipMemcpyDeviceToHost, sl->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sl->h_st, sl->d_st, tbs.size() * 4, hipMemcpyDeviceToHost, sl->stream);
+    if (e == hipSuccess && tbo > 0) e = hipMemcpyAsync(sl->h_tb, sl->d_tb, tbo, hipMemcpyDeviceToHost, sl->stream);
+    if (e == hipSuccess) e = hipEventRecord(sl->done, sl->stream);
+  }
+  if (rc != LH_OK || e != hipSuccess) {
+    // Nothing of this slot will complete: report its requests as rejected.
+    for (auto& r : sl->reqs) {
+      auto h = sv->harq.find(r.req.harq_process);
+      if (h != sv->harq.end()) {
+        h->second.busy = false;
+      }
+      sv->tb_dest.erase(r.req.id);
+      push_result(sv, r, LH_TB_REJECTED, 0, 0);
+    }
+    sl->reqs.clear();
+    return rc != LH_OK ? rc : hip_fail(e, "server slot launch");
+  }
+  sl->busy = true;
+  return LH_OK;
+}
+
+int lh_server_poll(lh_server* sv, lh_tb_result* out, unsigned max, unsigned* n_out, int wait)
+{
+  if (sv == nullptr || n_out == nullptr || (max > 0 && out == nullptr)) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_poll");
+  }
+  *n_out = 0;
+  bool any = false;
+  for (auto& sl : sv->slots) {
+    any = any || sl.busy;
+  }
+  if (wait && sv->results.empty() && any) {
+    // Block until the oldest in-flight slot completes.
+    for (unsigned k = 0; k < sv->slots.size(); ++k) {
+      srv_slot& sl = sv->slots[(sv->next_slot + k) % sv->slots.size()];
+      if (sl.busy) {
+        (void)hipEventSynchronize(sl.done);
+        break;
+      }
+    }
+  }
+  for (auto& sl : sv->slots) {
+    if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+      retire_slot(sv, sl);
+    }
+  }
+  admit(sv);
+  while (*n_out < max && !sv->results.empty()) {
+    out[(*n_out)++] = sv->results.front();
+    sv->results.pop_front();
+  }
+  return LH_OK;
+}
+
+int lh_server_harq_release(lh_server* sv, uint32_t harq_process)
+{
+  if (sv == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid server");
+  }
+  auto h = sv->harq.find(harq_process);
+  if (h == sv->harq.end()) {
+    return LH_OK;
+  }
+  if (h->second.busy) {
+    return fail(LH_ERR_INVALID, "HARQ process %u is in flight", harq_process);
+  }
+  sv->arena.release(h->second.off, h->second.bytes);
+  sv->harq.erase(h);
+  return LH_OK;
+}
+
+int lh_server_stats(lh_server* sv, lh_server_stats_t* st)
+{
+  if (sv == nullptr || st == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_stats");
+  }
+  st->harq_free_bytes = sv->arena.free_bytes();
+  st->harq_processes = (uint32_t)sv->harq.size();
+  st->pending = (uint32_t)sv->pending.size();
+  st->ready = (uint32_t)sv->ready.size();
+  uint32_t busy = 0;
+  for (auto& sl : sv->slots) {
+    busy += sl.busy ? 1u : 0u;
+  }
+  st->slots_in_flight = busy;
+  return LH_OK;
 }
 
 } // extern "C"

@@ -188,6 +188,32 @@ class GenTbDesc(C.Structure):
                 ("segment_length", C.c_uint32), ("nof_filler_bits", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("harq_budget_bytes", C.c_size_t), ("max_cbs_per_slot", C.c_uint), ("max_slots_in_flight", C.c_uint),
+                ("nof_ldpc_iterations", C.c_uint), ("nof_fallback_iterations", C.c_uint), ("learned", C.c_int)]
+
+
+class TbRequest(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("d_llrs", C.c_void_p), ("harq_process", C.c_uint32), ("new_data", C.c_int32),
+                ("base_graph", C.c_uint32), ("tbs_bits", C.c_uint32), ("modulation_order", C.c_uint32),
+                ("nof_layers", C.c_uint32), ("nof_ch_symbols", C.c_uint32), ("rv", C.c_uint32), ("Nref", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class TbResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("harq_process", C.c_uint32), ("tb_crc_ok", C.c_int32), ("nof_cbs", C.c_uint32),
+                ("nof_cbs_ok", C.c_uint32), ("max_iterations", C.c_uint32), ("pad", C.c_uint32),
+                ("latency_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("harq_free_bytes", C.c_uint64), ("harq_processes", C.c_uint32), ("pending", C.c_uint32),
+                ("ready", C.c_uint32), ("slots_in_flight", C.c_uint32)]
+
+
+LH_TB_CRC_OK, LH_TB_CB_FAILED, LH_TB_CRC_FAILED, LH_TB_REJECTED = 1, 0, -1, -2
+
+
 class Segmentation(C.Structure):
     _fields_ = [(f, C.c_uint32) for f in ("nof_cbs", "lifting_size", "segment_length", "nof_filler_bits",
                                            "nof_tb_crc_bits", "nof_cb_crc_bits", "cb_data_bits", "zero_pad")]
@@ -202,7 +228,8 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
-            "lh_gen_transport_blocks", "lh_segment_rx"]
+            "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
+            "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats"]
 
 
 def lib():
@@ -247,6 +274,13 @@ def lib():
                                               C.c_void_p]
         L.lh_segment_rx.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(Segmentation),
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint]
+        L.lh_server_create.argtypes = [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+        L.lh_server_destroy.argtypes = [C.c_void_p]
+        L.lh_server_submit.argtypes = [C.c_void_p, C.POINTER(TbRequest), C.c_void_p]
+        L.lh_server_step.argtypes = [C.c_void_p]
+        L.lh_server_poll.argtypes = [C.c_void_p, C.POINTER(TbResult), C.c_uint, C.POINTER(C.c_uint), C.c_int]
+        L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
+        L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -386,6 +420,49 @@ class SlotGraph:
     def close(self):
         if self.handle:
             lib().lh_slot_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The slot-deadline serving runtime (lh_server_*)."""
+
+    def __init__(self, ctx, harq_budget_bytes, max_cbs_per_slot=1024, max_slots_in_flight=2, nof_ldpc_iterations=8,
+                 learned=False, nof_fallback_iterations=8):
+        cfg = ServerConfig(harq_budget_bytes, max_cbs_per_slot, max_slots_in_flight, nof_ldpc_iterations,
+                           nof_fallback_iterations if learned else 0, int(learned))
+        h = C.c_void_p()
+        _lh(lib().lh_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "lh_server_create")
+        self.handle = h.value
+        self._buf = (TbResult * 4096)()
+
+    def submit(self, req, tb_out=None):
+        _lh(lib().lh_server_submit(self.handle, C.byref(req), tb_out), "lh_server_submit")
+
+    def step(self):
+        _lh(lib().lh_server_step(self.handle), "lh_server_step")
+
+    def poll(self, wait=False):
+        n = C.c_uint()
+        _lh(lib().lh_server_poll(self.handle, self._buf, len(self._buf), C.byref(n), int(wait)), "lh_server_poll")
+        return [TbResult.from_buffer_copy(self._buf[i]) for i in range(n.value)]
+
+    def harq_release(self, process):
+        _lh(lib().lh_server_harq_release(self.handle, process), "lh_server_harq_release")
+
+    def stats(self):
+        st = ServerStats()
+        _lh(lib().lh_server_stats(self.handle, C.byref(st)), "lh_server_stats")
+        return st
+
+    def close(self):
+        if self.handle:
+            lib().lh_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

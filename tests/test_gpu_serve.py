@@ -1,0 +1,85 @@
+"""Serving runtime (lh_server_*): every transport block's result -- TB CRC verdict,
+codeblocks passed, iterations, TB bytes -- is the same when it is served alone as when it
+is served in the mix of a continuous-batching run with overlapping slots, HARQ
+retransmissions (rv2 soft-combined into device buffers) included, for the plain and the
+learned decoder.  Bit-exact (tolerance zero)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ocudu_hip as H
+from gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+TABLES = Path(__file__).resolve().parent.parent / "ocudu-dapp-platform_amd" / "tables"
+
+
+@pytest.fixture(scope="module")
+def setup():
+    require_gpu()
+    from ocudu_hip import serve
+    ctx = H.Context(0, max_codeblocks=4096)
+    thr = serve.mcs_thresholds(ctx, n=64)
+    trace = serve.Trace(ctx, slots=12, mean_ues=8, thresholds=thr, seed=3, snr_lo=-0.6, snr_hi=0.4, prb_max=48)
+    yield ctx, trace
+    ctx.close()
+
+
+def _alone(ctx, trace, i, learned, mix):
+    from ocudu_hip import serve
+    srv = H.Server(ctx, 1 << 28, 1024, 1, 8, learned=learned, nof_fallback_iterations=8)
+    out = {}
+    for tx in (0, 1):
+        if 2 * i + tx not in mix:
+            break
+        buf = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
+        srv.submit(trace.request(i, tx), buf.ctypes.data)
+        srv.step()
+        got = []
+        while not got:
+            got = srv.poll(wait=True)
+        out[2 * i + tx] = (got[0], buf)
+    srv.close()
+    return out
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_serving_alone_equals_mix(setup, learned):
+    print(f"case serving alone vs mix learned={learned}", flush=True)
+    from ocudu_hip import nms, serve
+    ctx, trace = setup
+    if learned:
+        for bg in (1, 2):
+            _, aq, bq, _ = nms.load_table(TABLES / f"nms_bg{bg}.json")
+            ctx.set_scale_table(bg, aq, bq)
+    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, learned=learned, realtime=False, slots_in_flight=3)
+    assert len(mix) >= len(trace.tbs)
+    n_retx = sum(1 for k in mix if k % 2 == 1)
+    assert n_retx > 0, "the trace should exercise HARQ retransmissions"
+    for i in range(len(trace.tbs)):
+        alone = _alone(ctx, trace, i, learned, mix)
+        for key, (r, buf) in alone.items():
+            m = mix[key]
+            assert (r.tb_crc_ok, r.nof_cbs_ok, r.max_iterations) == (m.tb_crc_ok, m.nof_cbs_ok, m.max_iterations), \
+                f"TB {i} transmission {key % 2}: alone {r.tb_crc_ok, r.nof_cbs_ok, r.max_iterations} vs mix " \
+                f"{m.tb_crc_ok, m.nof_cbs_ok, m.max_iterations}"
+            if r.tb_crc_ok == H.LH_TB_CRC_OK:
+                assert np.array_equal(buf, bufs[key])
+                t = trace.tbs[i]
+                assert np.array_equal(buf, trace.tb_host[t["tb_off"]:t["tb_off"] + t["tbs"] // 8])
+    s = serve.summarize(trace, mix, slot_lat, wall, bufs)
+    print(s, flush=True)
+    assert s["wrong_tbs"] == 0
+
+
+def test_admission_waits_for_memory(setup):
+    """A HARQ budget that holds one TB at a time: every request is still served, in order,
+    as completions release memory."""
+    print("case admission by budget", flush=True)
+    ctx, trace = setup
+    from ocudu_hip import serve
+    need = max(t["seg"].nof_cbs * (66 if t["bg"] == 1 else 50) * t["seg"].lifting_size for t in trace.tbs)
+    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, realtime=False, harq_budget=need + 4096)
+    assert all(2 * i in mix for i in range(len(trace.tbs)))
+    assert all(r.tb_crc_ok != H.LH_TB_REJECTED for r in mix.values())
