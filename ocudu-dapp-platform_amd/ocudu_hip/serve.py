@@ -147,7 +147,7 @@ class Trace:
 
 
 def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs_per_slot=4096, slots_in_flight=3,
-          iters=8, realtime=True, log=None):
+          iters=8, realtime=True, log=None, timeout_s=600.0):
     """Runs the trace through a server: returns per-transmission results and per-slot
     latencies.  realtime: slots are released every slot_us; else as fast as possible."""
     srv = Server(ctx, harq_budget, max_cbs_per_slot, slots_in_flight, iters, learned=learned,
@@ -168,7 +168,18 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
     last_slot = nslots + HARQ_RTT_SLOTS + 2
     while k < last_slot or any(outstanding.values()):
         now = time.perf_counter()
-        if k < last_slot and (not realtime or now >= t0 + k * slot_us * 1e-6):
+        # Without the clock, slot k is released once the slots whose retransmissions are due
+        # in it have completed (the HARQ round trip), so the run is deterministic.
+        due = all(outstanding.get(s, 0) == 0 for s in range(max(0, k - HARQ_RTT_SLOTS + 1)))
+        if not realtime and not due:
+            # Requests waiting for HARQ memory with nothing in flight: only time frees it
+            # (a failed TB holds its buffers until its retransmission), so release a slot.
+            st = srv.stats()
+            due = st.slots_in_flight == 0 and st.ready == 0
+        if time.perf_counter() - t0 > timeout_s:
+            srv.close()
+            raise RuntimeError(f"serving loop exceeded {timeout_s} s at slot {k}")
+        if k < last_slot and ((realtime and now >= t0 + k * slot_us * 1e-6) or (not realtime and due)):
             slot_start[k] = now
             for i in by_slot.get(k, []):
                 buf = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
@@ -184,7 +195,7 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
                 outstanding[k] = outstanding.get(k, 0) + 1
             k += 1
         srv.step()
-        for r in srv.poll(wait=not realtime and k >= last_slot):
+        for r in srv.poll(wait=not realtime and (k >= last_slot or not due)):
             results[r.id] = r
             s = tx_slot[r.id]
             outstanding[s] -= 1
@@ -192,7 +203,10 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
                 slot_done[s] = time.perf_counter()
             i, tx = divmod(r.id, 2)
             if tx == 0 and r.tb_crc_ok != LH_TB_CRC_OK:
-                retx.setdefault(max(s + HARQ_RTT_SLOTS, k), []).append(i)
+                if max(s + HARQ_RTT_SLOTS, k) < last_slot:
+                    retx.setdefault(max(s + HARQ_RTT_SLOTS, k), []).append(i)
+                else:
+                    srv.harq_release(i)  # past the run: no retransmission
             elif tx == 1 and r.tb_crc_ok != LH_TB_CRC_OK:
                 srv.harq_release(i)
     wall = time.perf_counter() - t0

@@ -53,7 +53,8 @@ def test_serving_alone_equals_mix(setup, learned):
         for bg in (1, 2):
             _, aq, bq, _ = nms.load_table(TABLES / f"nms_bg{bg}.json")
             ctx.set_scale_table(bg, aq, bq)
-    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, learned=learned, realtime=False, slots_in_flight=3)
+    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, learned=learned, realtime=False, slots_in_flight=3,
+                                           timeout_s=120)
     assert len(mix) >= len(trace.tbs)
     n_retx = sum(1 for k in mix if k % 2 == 1)
     assert n_retx > 0, "the trace should exercise HARQ retransmissions"
@@ -80,6 +81,6 @@ def test_admission_waits_for_memory(setup):
     ctx, trace = setup
     from ocudu_hip import serve
     need = max(t["seg"].nof_cbs * (66 if t["bg"] == 1 else 50) * t["seg"].lifting_size for t in trace.tbs)
-    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, realtime=False, harq_budget=need + 4096)
+    mix, slot_lat, wall, bufs = serve.serve(ctx, trace, realtime=False, harq_budget=need + 4096, timeout_s=120)
     assert all(2 * i in mix for i in range(len(trace.tbs)))
     assert all(r.tb_crc_ok != H.LH_TB_REJECTED for r in mix.values())
