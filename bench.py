@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--snr", type=str, default="20,21,22,23,24,25,26", help="Es/N0 points (dB)")
     p.add_argument("--fine-step", type=float, default=0.05, help="grid of the operating-point search (dB)")
     p.add_argument("--table", default=str(ROOT / "ocudu-dapp-platform_amd" / "tables" / "nms_bg1.json"))
+    p.add_argument("--serve-slots", type=int, default=2000, help="slots of the serving-loop runs (0 = skip)")
+    p.add_argument("--uneven-ues", type=int, default=64, help="UEs of the uneven batch (0 = skip)")
     p.add_argument("--max-iter", type=int, default=8)
     p.add_argument("--latency-slots", type=int, default=2000)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -206,6 +208,9 @@ def main():
     print(f"# slot latency p50 {p50:.1f} us  p99 {p99:.1f} us over {a.latency_slots} slots", file=sys.stderr,
           flush=True)
 
+    uneven = uneven_batch(ctx, a) if a.uneven_ues else None
+    serving = serving_runs(ctx, a) if a.serve_slots else None
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(d_rm, n_cb, N, info_bits, a.max_iter, a.cpu_seconds, snr_h)
@@ -233,6 +238,8 @@ def main():
         "snr_sweep": sweep,
         "learned": learned,
         "operating_point": op,
+        "uneven_batch": uneven,
+        "serving": serving,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_ldpc_decode<1> (fused dematch + decode)", "kernel_ms": round(dec_ms, 4),
@@ -240,6 +247,63 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def uneven_batch(ctx, a):
+    """Uneven batch: a.uneven_ues UEs x 64 codeblocks of random BG, Z, rate and modulation at
+    each UE's 10% BLER Es/N0, in one persistent launch: default and autotuned launch shapes,
+    eager call and slot-graph replay."""
+    import ocudu_hip as H
+    from ocudu_hip import autotune, nms, workloads
+    t0 = time.time()
+    b, ues = workloads.uneven_batch(ctx, a.uneven_ues, 64, seed=a.seed)
+    gen_s = time.time() - t0
+    base = nms.measure(ctx, b, False, a.max_iter, reps=5)
+    n_tuned = autotune.apply(ctx, ROOT / "ocudu-dapp-platform_amd" / "tables" / "tune.json")
+    tuned = nms.measure(ctx, b, False, a.max_iter, reps=5)
+    g = ctx.slot_graph(b.pusch, b.d_cw, b.d_rm, b.d_bits, b.d_res, nof_ldpc_iterations=a.max_iter)
+    ev0, ev1 = H.Event(), H.Event()
+    best = float("inf")
+    for _ in range(5):
+        ev0.record()
+        g.launch()
+        ev1.record()
+        ev1.sync()
+        best = min(best, ev0.elapsed_ms(ev1))
+    err, _ = b.outcome()
+    g.close()
+    graph_gbps = b.good_info_bits(err) / (best * 1e-3) / 1e9
+    out = {"ues": a.uneven_ues, "codeblocks": b.n, "bler": round(base["bler"], 4), "mean_iters": round(base["iters"], 3),
+           "default": {"ms": round(base["ms"], 4), "info_gbps": round(base["gbps"], 3)},
+           "autotuned": {"ms": round(tuned["ms"], 4), "info_gbps": round(tuned["gbps"], 3), "entries": n_tuned},
+           "graph_replay": {"ms": round(best, 4), "info_gbps": round(graph_gbps, 3)},
+           "setup_s": round(gen_s, 1)}
+    print(f"# uneven batch: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
+def serving_runs(ctx, a):
+    """Slot-deadline serving loop (0.5 ms slots, Poisson(24) UEs per slot): at the specified
+    per-UE load (PRBs in [4, 273]) and at one carrier's load (PRBs in [4, 18]), plain and
+    learned decoders on the same traces."""
+    from ocudu_hip import nms, serve
+    tables = ROOT / "ocudu-dapp-platform_amd" / "tables"
+    for bg in (1, 2):
+        if (tables / f"nms_bg{bg}.json").exists():
+            _, aq, bq, _ = nms.load_table(tables / f"nms_bg{bg}.json")
+            ctx.set_scale_table(bg, aq, bq)
+    thr = serve.mcs_thresholds(ctx)
+    out = {"slots": a.serve_slots, "slot_us": 500, "mean_ues": 24}
+    for name, prb_max in (("spec_load", 273), ("carrier_load", 18)):
+        trace = serve.Trace(ctx, a.serve_slots, 24.0, thr, seed=a.seed, prb_max=prb_max)
+        entry = {"prb_max": prb_max, "tbs": len(trace.tbs),
+                 "codeblocks": int(sum(t["seg"].nof_cbs for t in trace.tbs))}
+        for dec in ("plain", "learned"):
+            entry[dec] = serve.summarize(trace, *serve.serve(ctx, trace, learned=dec == "learned"))
+            print(f"# serving {name} {dec}: {json.dumps(entry[dec])}", file=sys.stderr, flush=True)
+        out[name] = entry
+        del trace
+    return out
 
 
 def measured_traffic():

@@ -363,6 +363,8 @@ int lh_server_destroy(lh_server* server);
 /* Queues a request (its LLRs must stay valid until its result is polled); tb_out (host,
  * tbs_bits / 8 bytes, may be NULL) receives the TB when its CRC passes. */
 int lh_server_submit(lh_server* server, const lh_tb_request* req, uint8_t* tb_out);
+/* n requests at once (a slot's arrivals); tb_outs may be NULL. */
+int lh_server_submit_batch(lh_server* server, const lh_tb_request* reqs, unsigned n, uint8_t* const* tb_outs);
 /* Launches one slot of the admitted requests when a slot is free (non-blocking). */
 int lh_server_step(lh_server* server);
 /* Retires completed slots and returns up to max results; wait != 0 blocks until one

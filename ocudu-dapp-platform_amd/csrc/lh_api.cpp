@@ -1570,10 +1570,14 @@ int lh_server_destroy(lh_server* sv)
   return LH_OK;
 }
 
-int lh_server_submit(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out)
+} // extern "C"
+
+namespace {
+
+int submit_one(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out, double t_submit)
 {
-  if (sv == nullptr || req == nullptr || req->d_llrs == nullptr || (req->base_graph != 1 && req->base_graph != 2)) {
-    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_submit");
+  if (req == nullptr || req->d_llrs == nullptr || (req->base_graph != 1 && req->base_graph != 2)) {
+    return fail(LH_ERR_INVALID, "invalid transport-block request");
   }
   srv_request r;
   r.req = *req;
@@ -1591,11 +1595,38 @@ int lh_server_submit(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out)
     return fail(LH_ERR_CAPACITY, "request %llu: %u codeblocks cannot be served", (unsigned long long)req->id,
                 r.seg.nof_cbs);
   }
-  r.t_submit = now_us();
+  r.t_submit = t_submit;
   sv->tb_dest[req->id] = tb_out;
   sv->pending.push_back(std::move(r));
-  admit(sv);
   return LH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int lh_server_submit(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out)
+{
+  if (sv == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_submit");
+  }
+  int rc = submit_one(sv, req, tb_out, now_us());
+  admit(sv);
+  return rc;
+}
+
+int lh_server_submit_batch(lh_server* sv, const lh_tb_request* reqs, unsigned n, uint8_t* const* tb_outs)
+{
+  if (sv == nullptr || (n > 0 && reqs == nullptr)) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_submit_batch");
+  }
+  const double t = now_us();
+  int rc = LH_OK;
+  for (unsigned i = 0; i < n && rc == LH_OK; ++i) {
+    rc = submit_one(sv, &reqs[i], tb_outs ? tb_outs[i] : nullptr, t);
+  }
+  admit(sv);
+  return rc;
 }
 
 int lh_server_step(lh_server* sv)

@@ -429,7 +429,9 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   const int G = item.ngroups;
   const int lmax = item.lmax;
   const int t = (int)threadIdx.x;
-  const int Z = jobs[item.first_job].Z;
+  // Item-uniform values in scalar registers: every job of an item has the same (BG, Z)
+  // and graph, so the lifted edge table is read with scalar loads.
+  const int Z = __builtin_amdgcn_readfirstlane((int)jobs[item.first_job].Z);
   const int g = t / Z;
   const bool active = g < G;
   const int gg = active ? g : 0;
@@ -443,7 +445,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     jb.in_off = jb.rm.rm_off;
     jb.n_in = jb.rm.N;
   }
-  const uint32_t* __restrict__ gt = gtab + jb.gt_off;
+  const uint32_t* __restrict__ gt = gtab + __builtin_amdgcn_readfirstlane((int)jb.gt_off);
   // Fallback pass: codeblocks that an earlier pass decoded are left alone.
   const bool skip = active && (jb.flags & kFlagSkipPassed) && results[jb.out_idx] > 0;
   const bool work = active && !skip; // groups that load and decode
@@ -640,9 +642,26 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     __syncthreads();
     for (int l = 4; l < Litem; ++l) {
       if (run && l < L) {
-        const int e0 = T::rs(l);
-        layer_update<T::DEG_EXT, true, false, LEARNED>(soft0, goff, gt + e0, T::rs(l + 1) - e0, r, (unsigned)Z,
-                                                       &ext[(l - 4) * Z + (int)r], LEARNED ? abt[l] : 0u);
+        const int e0 = T::rs(l), dg = T::rs(l + 1) - e0;
+        const uint32_t ab = LEARNED ? abt[l] : 0u;
+        uint32_t* st = &ext[(l - 4) * Z + (int)r];
+        // Degree classes (uniform branch): the unrolled update does the work of its
+        // class's largest degree, so small rows take a small instance.
+        if (BG == 1) {
+          if (dg <= 5) {
+            layer_update<5, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+          } else if (dg <= 7) {
+            layer_update<7, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+          } else {
+            layer_update<10, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+          }
+        } else {
+          if (dg <= 4) {
+            layer_update<4, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+          } else {
+            layer_update<6, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+          }
+        }
       }
       __syncthreads();
     }

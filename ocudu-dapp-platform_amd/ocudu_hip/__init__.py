@@ -229,7 +229,7 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
-            "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats"]
+            "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch"]
 
 
 def lib():
@@ -278,6 +278,7 @@ def lib():
         L.lh_server_destroy.argtypes = [C.c_void_p]
         L.lh_server_submit.argtypes = [C.c_void_p, C.POINTER(TbRequest), C.c_void_p]
         L.lh_server_step.argtypes = [C.c_void_p]
+        L.lh_server_submit_batch.argtypes = [C.c_void_p, C.POINTER(TbRequest), C.c_uint, C.POINTER(C.c_void_p)]
         L.lh_server_poll.argtypes = [C.c_void_p, C.POINTER(TbResult), C.c_uint, C.POINTER(C.c_uint), C.c_int]
         L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
@@ -443,6 +444,10 @@ class Server:
 
     def submit(self, req, tb_out=None):
         _lh(lib().lh_server_submit(self.handle, C.byref(req), tb_out), "lh_server_submit")
+
+    def submit_batch(self, reqs, outs=None):
+        """reqs: a ctypes array of TbRequest; outs: a ctypes array of host pointers or None."""
+        _lh(lib().lh_server_submit_batch(self.handle, reqs, len(reqs), outs), "lh_server_submit_batch")
 
     def step(self):
         _lh(lib().lh_server_step(self.handle), "lh_server_step")

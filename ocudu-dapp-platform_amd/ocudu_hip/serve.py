@@ -12,6 +12,7 @@ generator), so the loop itself only serves.
     python -m ocudu_hip.serve --slots 2000 [--learned]
 """
 import argparse
+import ctypes as C
 import json
 import math
 import sys
@@ -163,6 +164,13 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
     outstanding = {}  # slot -> number of transmissions not completed
     tx_slot = {}
     out_bufs = {}
+    # New transmissions' requests and TB buffers prepared ahead (the loop only submits).
+    prepared = {}
+    for k, idx in by_slot.items():
+        ids = [(i, 0) for i in idx]
+        prepared[k] = (ids, (TbRequest * len(ids))(*[trace.request(i, 0) for i in idx]), (C.c_void_p * len(ids))())
+        for i in idx:
+            out_bufs[2 * i] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
     t0 = time.perf_counter()
     k = 0
     last_slot = nslots + HARQ_RTT_SLOTS + 2
@@ -180,19 +188,21 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
             srv.close()
             raise RuntimeError(f"serving loop exceeded {timeout_s} s at slot {k}")
         if k < last_slot and ((realtime and now >= t0 + k * slot_us * 1e-6) or (not realtime and due)):
-            slot_start[k] = now
-            for i in by_slot.get(k, []):
-                buf = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
-                out_bufs[2 * i] = buf
-                srv.submit(trace.request(i, 0), buf.ctypes.data)
-                tx_slot[2 * i] = k
-                outstanding[k] = outstanding.get(k, 0) + 1
-            for i in retx.pop(k, []):
-                buf = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
-                out_bufs[2 * i + 1] = buf
-                srv.submit(trace.request(i, 1), buf.ctypes.data)
-                tx_slot[2 * i + 1] = k
-                outstanding[k] = outstanding.get(k, 0) + 1
+            ids = prepared.get(k, ([], None, None))[0] + [(i, 1) for i in retx.pop(k, [])]
+            if ids:
+                if k in prepared and len(ids) == len(prepared[k][0]):
+                    _, reqs, outs = prepared.pop(k)
+                else:
+                    reqs = (TbRequest * len(ids))(*[trace.request(i, tx) for i, tx in ids])
+                    outs = (C.c_void_p * len(ids))()
+                for j, (i, tx) in enumerate(ids):
+                    if 2 * i + tx not in out_bufs:
+                        out_bufs[2 * i + tx] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
+                    outs[j] = out_bufs[2 * i + tx].ctypes.data
+                    tx_slot[2 * i + tx] = k
+                slot_start[k] = time.perf_counter()
+                srv.submit_batch(reqs, outs)
+                outstanding[k] = outstanding.get(k, 0) + len(ids)
             k += 1
         srv.step()
         for r in srv.poll(wait=not realtime and (k >= last_slot or not due)):
@@ -252,6 +262,8 @@ def _main(argv=None):
     ap.add_argument("--ues", type=float, default=24.0)
     ap.add_argument("--learned", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--prb-max", type=int, default=273, help="PRBs per UE uniform in [4, prb-max]")
+    ap.add_argument("--in-flight", type=int, default=3)
     args = ap.parse_args(argv)
     ctx = Context(0, max_codeblocks=8192)
 
@@ -259,7 +271,9 @@ def _main(argv=None):
         print(m, flush=True)
 
     thr = mcs_thresholds(ctx, log=log)
-    trace = Trace(ctx, args.slots, args.ues, thr, seed=args.seed)
+    t0 = time.time()
+    trace = Trace(ctx, args.slots, args.ues, thr, seed=args.seed, prb_max=args.prb_max)
+    log(f"trace generated in {time.time() - t0:.1f} s")
     log(f"{len(trace.tbs)} TBs, {sum(t['seg'].nof_cbs for t in trace.tbs)} codeblocks")
     if args.learned:
         from pathlib import Path
@@ -268,7 +282,7 @@ def _main(argv=None):
         for bg in (1, 2):
             _, aq, bq, _ = load_table(tables / f"nms_bg{bg}.json")
             ctx.set_scale_table(bg, aq, bq)
-    res = serve(ctx, trace, learned=args.learned, log=log)
+    res = serve(ctx, trace, learned=args.learned, slots_in_flight=args.in_flight, log=log)
     print(json.dumps(summarize(trace, *res)), flush=True)
     return 0
 
