@@ -27,8 +27,8 @@ size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage);
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         hipStream_t s);
-int decode_occupancy(int nt, size_t lds, bool learned);
+                         bool pairs, hipStream_t s);
+int decode_occupancy(int nt, size_t lds, bool learned, bool pairs);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
@@ -133,6 +133,7 @@ constexpr int kRing = 16;
 struct tune_entry {
   uint16_t groups = 0;
   uint8_t stage = 1;
+  int8_t pairs = -1; // row mode (lh_tune_set_rows)
 };
 
 struct slot {
@@ -308,41 +309,76 @@ struct launch_plan {
 // LDS budget for staging the codeword LLRs of one item.
 constexpr size_t kLdsBudget = 160 * 1024;
 
-// Codeblocks per work item: as many as fit the workgroup (G*Z <= threads).
-int groups_per_item(int Z, int nt)
+// Codeblocks per work item: as many as fit the workgroup (G * lanes <= threads).
+int groups_per_item(int lanes, int nt)
 {
-  return std::max(1, std::min(kMaxGroups, nt / Z));
+  return std::max(1, std::min(kMaxGroups, nt / lanes));
 }
 
-// A packed decoder launch: device block header (pass 1) | header (fallback pass) | items
-// | jobs, and the launch shape.
+// Row mode of a codeblock class in a launch of n codeblocks: two check rows per thread
+// (even Z; half the threads, each with 1.4x the instructions of one row) when a compute
+// unit's resident codeblocks are limited by its waves rather than its LDS and the launch
+// holds enough codeblocks to fill every compute unit at one row per thread; otherwise one
+// row per thread (lower latency per codeblock).
+// The tuning table can turn the mode off, ask for it under load, or force it.
+bool pair_rows(const lh_ctx* ctx, int bg, int Z, int lbound, size_t n)
+{
+  const tune_entry& te = ctx->tune[bg - 1][lifting_position((unsigned)Z)];
+  if ((Z & 1) || te.pairs == 0) {
+    return false;
+  }
+  if (te.pairs == 2) {
+    return true;
+  }
+  // 16 resident waves of 64 threads per compute unit at the decoder's 128 VGPRs.
+  const double by_waves = 1024.0 / Z;
+  const bool load = (double)n >= by_waves * ctx->nof_cus;
+  if (te.pairs == 1) {
+    return load;
+  }
+  const double by_lds = (double)kLdsBudget / (double)decode_item_lds_bytes(bg, Z, lbound, 1, 0);
+  return by_lds > 1.25 * by_waves && load;
+}
+
+// A packed decoder launch: four work-queue headers (pass 1 row-pair items, pass 1 one-row
+// items, then the same two for the fallback pass) | items (row-pair items first) | jobs,
+// and the launch shape of each row mode (index 1: row pairs).
 struct plan_block {
   std::vector<uint8_t> bytes;
   size_t o_items = 0, o_jobs = 0;
-  size_t n_items = 0;
-  int nt = 64;
-  size_t lds = 0;
+  size_t n_items[2] = {0, 0};
+  int nt[2] = {64, 64};
+  size_t lds[2] = {0, 0};
 };
 
 // Sorts the jobs largest first (longest-processing-time order for the dynamic queue; equal
-// (BG, Z) stay adjacent so they pack into shared items), packs items of up to G codeblocks
-// of one (BG, Z) -- G from the tuning table, else as many as fit the workgroup -- and
-// decides LDS staging of the codeword LLRs per item.
+// (BG, Z) stay adjacent so they pack into shared items), decides the row mode of every
+// (BG, Z, layer bound) class, packs items of up to G codeblocks of one class -- G from
+// the tuning table, else as many as fit the workgroup -- and decides LDS staging of the
+// codeword LLRs per item.
 int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
 {
   std::vector<dec_job>& jobs = plan.jobs;
-  // Workgroup size: the largest lifting size, or a tuned packing of G codeblocks of one
-  // (BG, Z), rounded up to whole waves.
-  int nt = 64;
+  const size_t n_cbs = jobs.size();
+  auto pairs_of_job = [&](const dec_job& j) {
+    const int bg = j.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
+    return pair_rows(ctx, bg, (int)j.Z, j.lbound, n_cbs);
+  };
+  // Workgroup size per row mode: the most threads per codeblock, or a tuned packing of G
+  // codeblocks of one (BG, Z), rounded up to whole waves.
+  int nt[2] = {64, 64};
   for (const dec_job& j : jobs) {
     const int bg = j.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
     const tune_entry& te = ctx->tune[bg - 1][lifting_position(j.Z)];
-    nt = std::max(nt, ((j.Z + 63) / 64) * 64);
+    const int m = pairs_of_job(j) ? 1 : 0;
+    const int lanes = m ? (int)j.Z / 2 : (int)j.Z;
+    nt[m] = std::max(nt[m], ((lanes + 63) / 64) * 64);
     if (te.groups > 0) {
-      nt = std::max(nt, ((te.groups * j.Z + 63) / 64) * 64);
+      nt[m] = std::max(nt[m], ((te.groups * lanes + 63) / 64) * 64);
     }
   }
-  nt = std::min(nt, kMaxThreads);
+  nt[0] = std::min(nt[0], kMaxThreads);
+  nt[1] = std::min(nt[1], kMaxThreads);
   std::stable_sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
     const int ka = (int)a.Z * (a.lbound + 22), kb = (int)b.Z * (b.lbound + 22);
     const int bga = a.gt_off < kNofLiftingSizes * 316 ? 1 : 2, bgb = b.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
@@ -355,12 +391,14 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     return a.Z > b.Z;
   });
   std::vector<dec_item> items;
-  size_t lds = 0;
+  size_t lds[2] = {0, 0};
   for (size_t i = 0; i < jobs.size();) {
     const int bg = jobs[i].gt_off < kNofLiftingSizes * 316 ? 1 : 2;
     const int Z = jobs[i].Z;
     const tune_entry& te = ctx->tune[bg - 1][lifting_position((unsigned)Z)];
-    int G = te.groups > 0 ? std::min<int>(te.groups, groups_per_item(Z, nt)) : groups_per_item(Z, nt);
+    const int m = pairs_of_job(jobs[i]) ? 1 : 0;
+    const int lanes = m ? Z / 2 : Z;
+    int G = te.groups > 0 ? std::min<int>(te.groups, groups_per_item(lanes, nt[m])) : groups_per_item(lanes, nt[m]);
     // Fewer codeblocks per item when their soft bits would not fit the LDS.
     while (G > 1 && decode_item_lds_bytes(bg, Z, jobs[i].lbound, G, 0) > kLdsBudget) {
       --G;
@@ -368,6 +406,7 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     dec_item it{};
     it.first_job = (uint32_t)i;
     it.bg = (uint8_t)bg;
+    it.rows2 = (uint8_t)m;
     int lmax = 4, maxE = 0;
     bool fused = false;
     size_t k = i;
@@ -391,26 +430,41 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
       it.stage = (uint32_t)stage;
       need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage);
     }
-    lds = std::max(lds, need);
+    lds[m] = std::max(lds[m], need);
     items.push_back(it);
     i = k;
   }
-  launch_header hdr[2]{};
-  hdr[0].n_items = hdr[1].n_items = (uint32_t)items.size();
+  // Row-pair items first (each mode keeps its largest-first order).
+  std::stable_partition(items.begin(), items.end(), [](const dec_item& it) { return it.rows2 != 0; });
+  size_t n_pair = 0;
+  while (n_pair < items.size() && items[n_pair].rows2) {
+    ++n_pair;
+  }
+  launch_header hdr[4]{};
+  for (int pass = 0; pass < 2; ++pass) {
+    hdr[2 * pass].n_items = (uint32_t)n_pair;
+    hdr[2 * pass].first_item = 0;
+    hdr[2 * pass + 1].n_items = (uint32_t)(items.size() - n_pair);
+    hdr[2 * pass + 1].first_item = (uint32_t)n_pair;
+  }
   pb.o_items = sizeof(hdr);
   pb.o_jobs = pb.o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
   pb.bytes.assign(pb.o_jobs + jobs.size() * sizeof(dec_job), 0);
   std::memcpy(pb.bytes.data(), hdr, sizeof(hdr));
   std::memcpy(pb.bytes.data() + pb.o_items, items.data(), items.size() * sizeof(dec_item));
   std::memcpy(pb.bytes.data() + pb.o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
-  pb.n_items = items.size();
-  pb.nt = nt;
-  pb.lds = lds;
+  pb.n_items[1] = n_pair;
+  pb.n_items[0] = items.size() - n_pair;
+  pb.nt[0] = nt[0];
+  pb.nt[1] = nt[1];
+  pb.lds[0] = lds[0];
+  pb.lds[1] = lds[1];
   return LH_OK;
 }
 
 // Launches a packed block resident at d_block: the decoder (learned with scale_tab), then
-// with fb_iters > 0 the plain decoder as its fallback over the same descriptors.
+// with fb_iters > 0 the plain decoder as its fallback over the same descriptors; each pass
+// runs the row-pair queue, then the one-row queue (each a persistent launch).
 hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int8_t* rm, const int8_t* cw,
                         uint8_t* bits, int32_t* results, hipStream_t stream, const uint32_t* scale_tab,
                         unsigned fb_iters)
@@ -418,15 +472,18 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
   launch_header* d_hdr = reinterpret_cast<launch_header*>(d_block);
   const dec_item* d_items = reinterpret_cast<const dec_item*>(d_block + pb.o_items);
   const dec_job* d_jobs = reinterpret_cast<const dec_job*>(d_block + pb.o_jobs);
-  const int per_cu = std::max(1, decode_occupancy(pb.nt, pb.lds, scale_tab != nullptr));
-  const unsigned grid = (unsigned)std::min<size_t>(pb.n_items, (size_t)ctx->nof_cus * per_cu);
-  hipError_t e = launch_decode(d_hdr, d_items, d_jobs, grid, pb.nt, pb.lds, rm, cw, bits, results, ctx->d_crc_x,
-                               ctx->d_gtab, scale_tab, 0, stream);
-  if (e == hipSuccess && fb_iters > 0) {
-    const int per_cu_fb = std::max(1, decode_occupancy(pb.nt, pb.lds, false));
-    const unsigned grid_fb = (unsigned)std::min<size_t>(pb.n_items, (size_t)ctx->nof_cus * per_cu_fb);
-    e = launch_decode(d_hdr + 1, d_items, d_jobs, grid_fb, pb.nt, pb.lds, rm, cw, bits, results, ctx->d_crc_x,
-                      ctx->d_gtab, nullptr, (int)fb_iters, stream);
+  hipError_t e = hipSuccess;
+  for (int pass = 0; pass < (fb_iters > 0 ? 2 : 1) && e == hipSuccess; ++pass) {
+    const uint32_t* tab = pass == 0 ? scale_tab : nullptr;
+    for (int m = 1; m >= 0 && e == hipSuccess; --m) {
+      if (pb.n_items[m] == 0) {
+        continue;
+      }
+      const int per_cu = std::max(1, decode_occupancy(pb.nt[m], pb.lds[m], tab != nullptr, m == 1));
+      const unsigned grid = (unsigned)std::min<size_t>(pb.n_items[m], (size_t)ctx->nof_cus * per_cu);
+      e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, grid, pb.nt[m], pb.lds[m], rm, cw, bits,
+                        results, ctx->d_crc_x, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1, stream);
+    }
   }
   return e;
 }
@@ -489,15 +546,29 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   }
   // Lifted edge tables: (col * Z) | ((V mod Z) << 16) per edge, rows in order
   // (ldpc_graph_impl.cpp:11-39, ldpc_luts_impl.cpp:4505-4523).
-  std::vector<uint32_t> gt(kGraphTableLen);
+  // Behind them the row-pair tables of the decoder's two-rows-per-thread mode (even Z):
+  // two words per edge, col * Z | S << 16 and S - T, with h = Z / 2, f = (shift >= h),
+  // S = 2 (shift mod h) + f and T = Z - 1 + 2f.
+  std::vector<uint32_t> gt(3 * (size_t)kGraphTableLen);
   for (int pos = 0; pos < kNofLiftingSizes; ++pos) {
     const unsigned Z = kLiftingSizes[pos];
     const int ils = lifting_index(Z);
+    auto put = [&](int bg, int k, unsigned col, unsigned v) {
+      const unsigned sh = v % Z;
+      const size_t o = graph_offset(bg, Z) + (size_t)k;
+      gt[o] = (uint32_t)(col * Z) | ((uint32_t)sh << 16);
+      if (Z % 2 == 0) {
+        const unsigned h = Z / 2, f = sh >= h ? 1u : 0u;
+        const unsigned S = 2 * (sh - f * h) + f, T = Z - 1 + 2 * f;
+        gt[kGraphTableLen + 2 * o] = (uint32_t)(col * Z) | (S << 16);
+        gt[kGraphTableLen + 2 * o + 1] = S - T;
+      }
+    };
     for (int k = 0; k < 316; ++k) {
-      gt[graph_offset(1, Z) + k] = (uint32_t)(lh_bg1_col[k] * Z) | ((uint32_t)(lh_bg1_v[ils][k] % Z) << 16);
+      put(1, k, lh_bg1_col[k], lh_bg1_v[ils][k]);
     }
     for (int k = 0; k < 197; ++k) {
-      gt[graph_offset(2, Z) + k] = (uint32_t)(lh_bg2_col[k] * Z) | ((uint32_t)(lh_bg2_v[ils][k] % Z) << 16);
+      put(2, k, lh_bg2_col[k], lh_bg2_v[ils][k]);
     }
   }
   // CRC24A byte table and x^(2^k) mod G for the transport-block CRC.
@@ -990,6 +1061,16 @@ int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigne
   tune_entry& t = ctx->tune[base_graph - 1][lifting_position(lifting_size)];
   t.groups = (uint16_t)groups;
   t.stage = stage_llrs ? 1 : 0;
+  return LH_OK;
+}
+
+int lh_tune_set_rows(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, int row_pairs)
+{
+  if (ctx == nullptr || (base_graph != 1 && base_graph != 2) || lifting_position(lifting_size) < 0 ||
+      row_pairs < -1 || row_pairs > 2 || (row_pairs > 0 && (lifting_size & 1))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_tune_set_rows");
+  }
+  ctx->tune[base_graph - 1][lifting_position(lifting_size)].pairs = (int8_t)row_pairs;
   return LH_OK;
 }
 

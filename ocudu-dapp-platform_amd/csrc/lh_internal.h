@@ -67,13 +67,14 @@ struct alignas(16) dec_job {
 static_assert(sizeof(dec_job) == 112, "dec_job layout");
 
 // Work item of the persistent decoder: `ngroups` consecutive jobs with the same (BG, Z);
-// group g runs on threads [g*Z, (g+1)*Z) of the workgroup (16 bytes).
+// group g runs on threads [g*n, (g+1)*n) of the workgroup, n = Z (one check row per
+// thread) or Z/2 (rows2: rows p and p + Z/2 in the two 16-bit lanes of a register).
 constexpr int kMaxGroups = 64;
 struct alignas(16) dec_item {
   uint32_t first_job;
   uint16_t ngroups;
   uint8_t bg;
-  uint8_t pad0;
+  uint8_t rows2; // 1: two check rows per thread (even Z)
   uint16_t lmax;  // layer bound of the item (max over its jobs)
   uint16_t pad1;
   uint32_t stage; // LDS bytes per group for the codeword LLRs (0: read them from HBM)
@@ -82,9 +83,10 @@ static_assert(sizeof(dec_item) == 16, "dec_item layout");
 
 // Header of the device descriptor block of one decoder launch.
 struct alignas(16) launch_header {
-  uint32_t next_item; // work-queue counter, zero at upload
-  uint32_t n_items;
-  uint32_t pad[2];
+  uint32_t next_item;  // work-queue counter, zero at upload
+  uint32_t n_items;    // items of this queue: [first_item, first_item + n_items)
+  uint32_t first_item;
+  uint32_t pad;
 };
 
 // One synthetic codeblock of the generator (128 bytes): message bits from seed_msg,

@@ -1,14 +1,14 @@
 // HIP kernels of the PUSCH LDPC receive chain for gfx950 (MI355X).
 //
-// Decoder: one workgroup per codeblock, one thread per lifted check row r (r < Z).
-// Thread r owns, for every layer, the check-to-variable messages of check row r, so
-// the layered min-sum update needs no cross-lane exchange: the cyclic shift of each
-// edge becomes the LDS address (col*Z + (r + shift) mod Z) of the variable-node soft
-// bit that row r reads and writes.  Soft bits live in LDS as int8; the four core
-// layers keep their compressed check-to-variable state in VGPRs, the extension layers
-// in thread-private LDS words.  The lifted edge table (col*Z, shift) of each graph is
-// read with scalar loads (uniform per workgroup).  Barriers separate layers (strict
-// layer order of the reference).
+// Decoder: a persistent workgroup pulls work items of G codeblocks; one thread per lifted
+// check row r (r < Z), or per row pair (r, r + Z/2) in the two 16-bit lanes of a
+// register.  A thread owns, for every layer, the check-to-variable messages of its rows,
+// so the layered min-sum update needs no cross-lane exchange: the cyclic shift of each
+// edge becomes the LDS address of the variable-node soft bit that the row reads and
+// writes.  Soft bits live in LDS as int8; the four core layers keep their packed
+// check-to-variable messages in VGPRs, the extension layers a compressed word per row in
+// LDS.  The lifted edge table of each graph is read with scalar loads (uniform per
+// workgroup).  Barriers separate layers (strict layer order of the reference).
 #include <hip/hip_runtime.h>
 
 #include "bg_tables.inc"
@@ -36,7 +36,6 @@ struct bgt<2> {
 // (lib/phy/upper/log_likelihood_ratio.cpp:22-70).  Infinite LLR: |x| > 120.
 __device__ __forceinline__ int inf_mask(int x) { return -(int)((unsigned)(x + 120) > 240u); }
 __device__ __forceinline__ int llr_sat(int x) { return min(max(x, -120), 120); }
-__device__ __forceinline__ int llr_promote(int x) { return x > 120 ? 127 : (x < -120 ? -127 : x); }
 __device__ __forceinline__ int sel(int m, int a, int b) { return (a & m) | (b & ~m); }
 // Saturated sum with the special cases in the reference's order.
 __device__ __forceinline__ int llr_add(int a, int b)
@@ -46,10 +45,7 @@ __device__ __forceinline__ int llr_add(int a, int b)
   s = sel(inf_mask(a), a, s);
   return a == -b ? 0 : s;
 }
-__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c)
-{
-  return max(min(a, b), min(max(a, b), c));
-}
+__device__ __forceinline__ int med3i(int a, int lo, int hi) { return min(max(a, lo), hi); }
 // x * x for |x| < 2^11 with the full-rate 24-bit multiplier.
 __device__ __forceinline__ uint32_t sq24(int x)
 {
@@ -63,7 +59,6 @@ __device__ __forceinline__ uint32_t med3u_asm(uint32_t a, uint32_t b, uint32_t c
   asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
   return d;
 }
-__device__ __forceinline__ int med3i(int a, int lo, int hi) { return min(max(a, lo), hi); }
 // sqrt of a perfect square n^2 <= 14400 (rounded, exact).
 __device__ __forceinline__ int isqrt_sq(uint32_t q) { return (int)(__fsqrt_rn((float)q) + 0.5f); }
 // round(0.8 * m) for 0 <= m <= 127 (0.8m never ends in .5).
@@ -77,13 +72,17 @@ __device__ __forceinline__ int learned_mag(int m, uint32_t ab)
   return med3i(((int)(ab & 0xffffu) * m - ((int)ab >> 16) + 128) >> 8, 0, 120);
 }
 
+// Soft bits in LDS hold a finite LLR in [-120, 120] or the infinity +-121 (any input
+// beyond +-120, i.e. the reference's +-127, becomes +-121 on load): the promotion sum of
+// the layer update is then a single clamp to [-121, 121].
+//
 // v2c from a soft bit and the previous check-to-variable message: saturated difference
-// for a finite soft bit; an infinite soft bit (|sv| = 127) passes through as +-(269..358),
-// which never becomes a minimum and promotes back to +-127 whatever the new message.
+// for a finite soft bit; an infinite soft bit passes through as +-(242..361), which never
+// becomes a minimum and promotes back to +-121 whatever the new message.
 __device__ __forceinline__ int v2c_of(int sv, int c)
 {
-  const int ex = sv - med3i(sv, -120, 120); // 0, or +-7 for an infinite soft bit
-  return med3i(sv - c, -120, 120) + 34 * ex;
+  const int ex = sv - med3i(sv, -120, 120); // 0, or +-1 for an infinite soft bit
+  return med3i(sv - c, -120, 120) + 241 * ex;
 }
 
 // One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
@@ -173,7 +172,7 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
       } else {
         ns |= (uint32_t)(neg & 1) << e;
       }
-      soft[addr[e]] = (int8_t)llr_promote(c + v[e]);
+      soft[addr[e]] = (int8_t)med3i(c + v[e], -121, 121); // promotion sum
     }
   }
   if (PACKED) {
@@ -186,13 +185,186 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
   }
 }
 
-template <int BG, int ROW, bool LEARNED>
+// --- Two check rows per thread (even Z) ---------------------------------------------
+// Thread p < h = Z/2 owns check rows p and p + h.  Row p + h reads column
+// (p + h + shift) mod Z, which is row p's column +-h.  In this row-pair layout the rows
+// j and j + h of a column are adjacent bytes (row j at byte 2 (j mod h) + (j >= h) of
+// the column), so the two soft bits of an edge sit at a and a ^ 1 and one address
+// serves both rows; their values travel in the two 16-bit lanes of a register and every
+// packed 16-bit instruction (v_pk_*) updates both rows.  Half the threads per codeblock
+// do 1.4x the instructions of one row each: the mode pays where a compute unit's
+// resident codeblocks are limited by waves rather than by LDS.
+
+// LDS byte of row (r + shift) mod Z of a column for thread r (and a ^ 1 for row r + h).
+// With u = r + (shift mod h), f = (shift >= h), S = 2 (shift mod h) + f and
+// T = Z - 1 + 2f, the byte is min(2r + S, 2r + S - T) in unsigned arithmetic (the second
+// wraps exactly when the row stays in its half).  The row-pair edge table holds two
+// words per edge: col * Z | S << 16, and S - T.
+__device__ __forceinline__ int pair_addr(uint32_t g0, uint32_t g1, unsigned r2)
+{
+  return (int)((g0 & 0xffffu) + min(r2 + (g0 >> 16), r2 + g1));
+}
+// LDS byte of soft-bit index q = column * Z + j (j = q mod Z): q itself in the one-row
+// layout, the row-pair byte with PAIR.
+template <bool PAIR>
+__device__ __forceinline__ int soft_pos(int q, int j, int Z)
+{
+  return !PAIR ? q : (j < Z / 2 ? q + j : q + j - Z + 1);
+}
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ i16x2 as_i16x2(uint32_t w) { return __builtin_bit_cast(i16x2, w); }
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
+template <typename V>
+__device__ __forceinline__ uint32_t as_u32(V v)
+{
+  return __builtin_bit_cast(uint32_t, v);
+}
+__device__ __forceinline__ i16x2 clamp2(i16x2 x, short lo, short hi)
+{
+  return __builtin_elementwise_min(__builtin_elementwise_max(x, (i16x2)lo), (i16x2)hi);
+}
+// Scaled magnitudes of both rows (lanes of `m`, each <= 120).
+template <bool LEARNED>
+__device__ __forceinline__ uint32_t scale_pair(uint32_t m, uint32_t ab)
+{
+  const int lo = (int)(m & 0xffffu), hi = (int)(m >> 16);
+  if constexpr (LEARNED) {
+    return (uint32_t)learned_mag(lo, ab) | ((uint32_t)learned_mag(hi, ab) << 16);
+  } else {
+    return (uint32_t)scale08(lo) | ((uint32_t)scale08(hi) << 16);
+  }
+}
+
+// layer_update for the row pair (r, r + h): the same arithmetic per row, with keys
+// |v2c| << 5 | edge (ordered like v2c^2 << 5 | edge).  State forms:
+//  * PACKED: word k holds the int8 messages of edges 2k and 2k+1 as bytes
+//    [2k row r, 2k+1 row r, 2k row r+h, 2k+1 row r+h];
+//  * otherwise two words, row r then row r+h, each in layer_update's one-word form.
+template <int DEG, bool RT, bool PACKED, bool LEARNED>
+__device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int goff,
+                                                  const uint32_t* __restrict__ gt, int deg, unsigned r,
+                                                  uint32_t* __restrict__ state, uint32_t ab)
+{
+  constexpr int NW = (DEG + 1) / 2;
+  // Opaque (1, 1): min(x, 1) then stays one v_pk_min_u16 (with a literal 1 the compiler
+  // turns it into per-lane compares and selects whose lane masks crowd the SGPRs).
+  uint32_t one_w = 0x00010001u;
+  asm volatile("" : "+v"(one_w));
+  const u16x2 one2 = as_u16x2(one_w);
+  int a1[DEG];
+  i16x2 sv[DEG];
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (!RT || e < deg) {
+      a1[e] = pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + goff;
+      i16x2 x;
+      x.x = soft[a1[e]];
+      x.y = soft[a1[e] ^ 1];
+      sv[e] = x;
+    }
+  }
+  // Previous messages of the compressed form: magnitudes, argmin edges, sign bits.
+  u16x2 S2o = 0, Do = 0, AMo = 0;
+  uint32_t SGo = 0;
+  if (!PACKED) {
+    const uint32_t lo = state[0], hi = state[1];
+    const u16x2 s1 = as_u16x2((lo & 127u) | ((hi & 127u) << 16));
+    S2o = as_u16x2(((lo >> 7) & 127u) | (((hi >> 7) & 127u) << 16));
+    Do = s1 - S2o;
+    AMo = as_u16x2(((lo >> 14) & 31u) | (((hi >> 14) & 31u) << 16));
+    SGo = (lo >> 19) | ((hi >> 19) << 16);
+  }
+  u16x2 m1 = (u16x2)(unsigned short)(120u << 5), m2 = (u16x2)(unsigned short)((120u << 5) | 31u);
+  uint32_t sp = 0;
+  i16x2 v[DEG];
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (!RT || e < deg) {
+      i16x2 c;
+      if (PACKED) {
+        const i16x2 w = as_i16x2(state[e >> 1]);
+        c = (e & 1) ? (w >> (short)8) : ((i16x2)(w << (short)8) >> (short)8);
+      } else {
+        const u16x2 sel = __builtin_elementwise_min(AMo ^ (u16x2)(unsigned short)e, one2);
+        const i16x2 mag = __builtin_bit_cast(i16x2, (u16x2)(S2o + sel * Do));
+        const i16x2 neg = as_i16x2(SGo << (15 - e)) >> (short)15;
+        c = (mag ^ neg) - neg;
+      }
+      const i16x2 s = sv[e];
+      const i16x2 x = clamp2(s - c, -120, 120) + (s - clamp2(s, -120, 120)) * (short)241;
+      v[e] = x;
+      const u16x2 ax = __builtin_bit_cast(u16x2, __builtin_elementwise_max(x, -x));
+      const u16x2 key = ax * (unsigned short)32 + (unsigned short)e;
+      m2 = __builtin_elementwise_min(m2, __builtin_elementwise_max(m1, key));
+      m1 = __builtin_elementwise_min(m1, key);
+      sp ^= as_u32(x);
+    }
+  }
+  const uint32_t m1w = as_u32(m1), m2w = as_u32(m2);
+  const u16x2 amin = as_u16x2(m1w & 0x001f001fu);
+  const uint32_t s1 = scale_pair<LEARNED>((m1w >> 5) & 0x07ff07ffu, ab);
+  const uint32_t s2 = scale_pair<LEARNED>((m2w >> 5) & 0x07ff07ffu, ab);
+  // Row signs folded into the magnitudes: message e = sign(v2c_e) * (+-s1 or +-s2).
+  const i16x2 rs = as_i16x2(sp) >> (short)15;
+  const i16x2 S1 = (as_i16x2(s1) ^ rs) - rs;
+  const i16x2 S2 = (as_i16x2(s2) ^ rs) - rs;
+  const i16x2 D = S1 - S2;
+  uint32_t w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    w[k] = 0;
+  }
+  uint32_t ns = 0;
+  i16x2 ce = 0;
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (!RT || e < deg) {
+      const u16x2 sel = __builtin_elementwise_min(amin ^ (u16x2)(unsigned short)e, one2);
+      const i16x2 mag = S2 + __builtin_bit_cast(i16x2, sel) * D;
+      const i16x2 neg = v[e] >> (short)15;
+      const i16x2 c = (mag ^ neg) - neg;
+      if (PACKED) {
+        if (e & 1) {
+          w[e >> 1] = __builtin_amdgcn_perm(as_u32(c), as_u32(ce), 0x06020400u);
+        } else if (e + 1 == DEG) {
+          w[e >> 1] = __builtin_amdgcn_perm(0u, as_u32(c), 0x06020400u);
+        } else {
+          ce = c;
+        }
+      } else {
+        ns |= ((as_u32(c) >> 15) & 0x00010001u) << e;
+      }
+      const i16x2 y = clamp2(c + v[e], -121, 121); // promotion sum
+      soft[a1[e]] = (int8_t)y.x;
+      soft[a1[e] ^ 1] = (int8_t)y.y;
+    }
+  }
+  if (PACKED) {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      state[k] = w[k];
+    }
+  } else {
+    const uint32_t mg = (s1 & 0x007f007fu) | ((s2 & 0x007f007fu) << 7);
+    const uint32_t am = as_u32(amin);
+    state[0] = (mg & 0x3fffu) | ((am & 31u) << 14) | ((ns & 0x1fffu) << 19);
+    state[1] = ((mg >> 16) & 0x3fffu) | (((am >> 16) & 31u) << 14) | (((ns >> 16) & 0x1fffu) << 19);
+  }
+}
+
+template <int BG, int ROW, bool LEARNED, bool PAIR>
 __device__ __forceinline__ void core_layer(int8_t* soft, int goff, const uint32_t* gt, unsigned r, unsigned Z,
                                            uint32_t* cpk, const uint32_t* abt)
 {
   constexpr int E0 = bgt<BG>::rs(ROW);
   constexpr int DEG = bgt<BG>::rs(ROW + 1) - E0;
-  layer_update<DEG, false, true, LEARNED>(soft, goff, gt + E0, DEG, r, Z, cpk, LEARNED ? abt[ROW] : 0u);
+  if constexpr (PAIR) {
+    layer_update_pair<DEG, false, true, LEARNED>(soft, goff, gt + 2 * E0, DEG, r, cpk, LEARNED ? abt[ROW] : 0u);
+  } else {
+    layer_update<DEG, false, true, LEARNED>(soft, goff, gt + E0, DEG, r, Z, cpk, LEARNED ? abt[ROW] : 0u);
+  }
 }
 
 // Wave64 + workgroup reductions.  `red` holds 16 words.
@@ -241,37 +413,61 @@ template <int BG>
 __device__ uint32_t syndrome_fail(const int8_t* soft, int goff, const uint32_t* gt, unsigned r, unsigned Z, int L)
 {
   uint32_t bad = 0;
-  if (r < Z) {
-    for (int l = 0; l < L; ++l) {
-      const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
-      uint32_t p = 0;
-      for (int e = e0; e < e1; ++e) {
-        const uint32_t g = gt[e];
-        unsigned j = r + (g >> 16);
-        j = min(j, j - Z);
-        const int s = soft[(int)((g & 0xffffu) + j) + goff];
-        bad |= (s == 0);
-        p ^= (s <= 0);
-      }
-      bad |= p;
+  for (int l = 0; l < L; ++l) {
+    const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
+    uint32_t p = 0;
+    for (int e = e0; e < e1; ++e) {
+      const uint32_t g = gt[e];
+      unsigned j = r + (g >> 16);
+      j = min(j, j - Z);
+      const int s = soft[(int)((g & 0xffffu) + j) + goff];
+      bad |= (s == 0);
+      p ^= (s <= 0);
     }
+    bad |= p;
+  }
+  return bad;
+}
+
+// The same for the row pair (r, r + Z/2) of thread r in the row-pair layout.
+template <int BG>
+__device__ uint32_t syndrome_fail_pair(const int8_t* soft, int goff, const uint32_t* gt, unsigned r, int L)
+{
+  uint32_t bad = 0;
+  for (int l = 0; l < L; ++l) {
+    const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
+    uint32_t p = 0;
+    for (int e = e0; e < e1; ++e) {
+      const int a = pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + goff;
+      const int s0 = soft[a], s1 = soft[a ^ 1];
+      bad |= (uint32_t)(s0 == 0) | (uint32_t)(s1 == 0);
+      p ^= (uint32_t)(s0 <= 0) | ((uint32_t)(s1 <= 0) << 1);
+    }
+    bad |= p;
   }
   return bad;
 }
 
 // Hard decisions of soft[0, nbits), MSB-first; bits of the last byte beyond nbits keep
 // their previous value.  all_ones: ones instead (no-CRC failure path).
-__device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones, int first, int stride)
+// PAIR: soft bits in the row-pair layout of lifting size Z; stride * 8 is a multiple of
+// Z (the row of a thread's first bit is the same in every pass).
+template <bool PAIR>
+__device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones, int first, int stride,
+                                int Z)
 {
   const int nbytes = (nbits + 7) >> 3;
+  const int j0 = (8 * first) % Z;
   for (int b = first; b < nbytes; b += stride) {
     uint32_t byte = 0;
+    int j = j0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int i = 8 * b + k;
       if (i < nbits) {
-        byte |= (uint32_t)(all_ones || soft[i] <= 0) << (7 - k);
+        byte |= (uint32_t)(all_ones || soft[soft_pos<PAIR>(i, j, Z)] <= 0) << (7 - k);
       }
+      j = j + 1 == Z ? 0 : j + 1;
     }
     const int valid = nbits - 8 * b;
     if (valid < 8) {
@@ -315,11 +511,16 @@ __device__ __forceinline__ int dematch_at(const rm_params& rp, const int8_t* __r
 
 // Fused dematching of positions [0, n_in): four consecutive positions per thread, one
 // dword HARQ access when aligned; writes the clamped soft bits of positions < cap.
+// Soft bits of lifting size Z, NT threads per codeblock; PAIR: row-pair layout (4 NT is
+// then a multiple of Z: the four positions of a thread start at the same row in every
+// pass).
+template <bool PAIR>
 __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_t* __restrict__ src,
                                                 int8_t* __restrict__ harq, bool new_data, bool wr, int n_in, int cap,
-                                                int8_t* __restrict__ soft2, unsigned r, unsigned NT, int& last)
+                                                int8_t* __restrict__ soft2, unsigned r, unsigned NT, int Z, int& last)
 {
   const bool al4 = ((reinterpret_cast<uintptr_t>(harq)) & 3) == 0;
+  const int j0 = PAIR ? (4 * (int)r) % Z : 0;
   for (int p0 = 4 * (int)r; p0 < n_in; p0 += 4 * (int)NT) {
     uint32_t oldw = 0;
     if (!new_data) {
@@ -340,7 +541,9 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
         outw |= (uint32_t)(uint8_t)x << (8 * k);
         last = x != 0 ? p : last;
         if (p < cap) {
-          soft2[p] = (int8_t)min(max(x, -64), 64);
+          int j = j0 + k;
+          j = j >= Z ? j - Z : j;
+          soft2[soft_pos<PAIR>(p, j, Z)] = (int8_t)min(max(x, -64), 64);
         }
       }
     }
@@ -410,14 +613,20 @@ struct grp_ctl {
 };
 
 constexpr int kCtlBytes = (int)sizeof(grp_ctl);
+// The last 16 bytes of the dynamic LDS hold the work-queue slot of the workgroup: with
+// no static allocation the soft bits start at LDS address 0, so a soft-bit index is the
+// LDS byte address itself (and a ^ 1 its row-pair partner, with no offset to fold).
+constexpr int kQueueBytes = 16;
 
 // Decodes one work item (G codeblocks of one (BG, Z)), behaviour of
 // ldpc_decoder_impl::decode (lib/phy/upper/channel_coding/ldpc/ldpc_decoder_impl.cpp:41-145)
 // with the arithmetic of ldpc_decoder_generic.cpp:12-104 for every codeblock.  With
 // kFlagFused the group first rate-dematches its codeblock (writing the HARQ buffer when
-// asked) instead of reading a dematched buffer.  Thread t works on row t mod Z of group
-// t / Z; the threads beyond G*Z only take part in barriers.
-template <int BG, bool LEARNED>
+// asked) instead of reading a dematched buffer.  With n threads per codeblock (Z, or Z/2
+// with PAIR), thread t works on row t mod n (and t mod n + Z/2 with PAIR) of group t / n;
+// the threads beyond G*n only take part in barriers.  Both modes use the same LDS
+// amounts (only the order of the soft bits in a column differs).
+template <int BG, bool LEARNED, bool PAIR>
 __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs, int8_t* __restrict__ rm_base,
                             const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
                             int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x,
@@ -432,10 +641,12 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   // Item-uniform values in scalar registers: every job of an item has the same (BG, Z)
   // and graph, so the lifted edge table is read with scalar loads.
   const int Z = __builtin_amdgcn_readfirstlane((int)jobs[item.first_job].Z);
-  const int g = t / Z;
+  // Threads per codeblock.
+  const int NT = PAIR ? Z / 2 : Z;
+  const int g = t / NT;
   const bool active = g < G;
   const int gg = active ? g : 0;
-  const unsigned r = (unsigned)(t - g * Z);
+  const unsigned r = (unsigned)(t - g * NT);
   dec_job jb = jobs[item.first_job + gg];
   if (fb_iters > 0) {
     // Fallback pass over the jobs of a fused PUSCH pass: the plain decoder on the HARQ
@@ -445,21 +656,21 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     jb.in_off = jb.rm.rm_off;
     jb.n_in = jb.rm.N;
   }
-  const uint32_t* __restrict__ gt = gtab + __builtin_amdgcn_readfirstlane((int)jb.gt_off);
   // Fallback pass: codeblocks that an earlier pass decoded are left alone.
   const bool skip = active && (jb.flags & kFlagSkipPassed) && results[jb.out_idx] > 0;
   const bool work = active && !skip; // groups that load and decode
 
   // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
   const int soft_stride = ((K + lmax) * Z + 15) & ~15;
-  const int ext_words = (lmax - 4) * Z;
+  const int ext_words = (lmax - 4) * Z; // one word per check row and layer
   const int stage_stride = (int)item.stage;
   int8_t* const soft0 = reinterpret_cast<int8_t*>(lds); // uniform base of the soft bits
   const int goff = gg * soft_stride;                      // this group's soft bits
   int8_t* soft = soft0 + goff;
-  uint32_t* ext = reinterpret_cast<uint32_t*>(lds + G * soft_stride) + gg * ext_words;
-  int8_t* stage = reinterpret_cast<int8_t*>(lds + G * soft_stride + 4 * G * ext_words) + gg * stage_stride;
-  grp_ctl* ctl = reinterpret_cast<grp_ctl*>(lds + G * soft_stride + 4 * G * ext_words + G * stage_stride);
+  uint8_t* const ldsd = lds;
+  uint32_t* ext = reinterpret_cast<uint32_t*>(ldsd + G * soft_stride) + gg * ext_words;
+  int8_t* stage = reinterpret_cast<int8_t*>(ldsd + G * soft_stride + 4 * G * ext_words) + gg * stage_stride;
+  grp_ctl* ctl = reinterpret_cast<grp_ctl*>(ldsd + G * soft_stride + 4 * G * ext_words + G * stage_stride);
 
   if (t < G) {
     ctl->last[t] = -1;
@@ -473,8 +684,8 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   __syncthreads();
 
   // Soft-bit load (load_soft_bits, ldpc_decoder_impl.cpp:147-188): two punctured nodes at
-  // zero, full nodes clamped to [-64, 64], a partial last node unclamped; track the last
-  // nonzero input (:67-71).
+  // zero, full nodes clamped to [-64, 64], a partial last node unclamped (infinity as
+  // +-121); track the last nonzero input (:67-71).
   const int n_in = (int)jb.n_in;
   const int full = (n_in / Z) * Z;
   const int cap = (K + lmax - 2) * Z;
@@ -493,7 +704,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
       const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
       const int lead = (int)(a & 15);
       const int nblk = (lead + (int)rp.E + 15) >> 4;
-      for (int b = (int)r; b < nblk; b += Z) {
+      for (int b = (int)r; b < nblk; b += NT) {
         const uint4 w = blk[b];
         const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -510,40 +721,50 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     }
     if (work) {
       if (staged) {
-        fused_positions(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
+        fused_positions<PAIR>(rp, stage, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)NT, Z, last);
       } else {
-        fused_positions(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)Z, last);
+        fused_positions<PAIR>(rp, in, harq, new_data, wr, n_in, cap, soft + 2 * Z, r, (unsigned)NT, Z, last);
       }
     }
   } else if (work) {
-    // Aligned 16-byte loads of the dematched soft bits.
+    // Aligned 4-byte loads of the dematched soft bits.
     const int8_t* in = rm_base + jb.in_off;
     const uintptr_t a = reinterpret_cast<uintptr_t>(in);
-    const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
-    const int lead = (int)(a & 15);
-    const int nblk = (lead + n_in + 15) >> 4;
-    for (int b = (int)r; b < nblk; b += Z) {
-      const uint4 w = blk[b];
-      const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+    const uint32_t* blk = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    const int lead = (int)(a & 3);
+    const int nblk = (lead + n_in + 3) >> 2;
+    // Row of the first position of a word (the same in every pass: the stride 4 NT is a
+    // multiple of Z).
+    const int jb = PAIR ? ((4 * (int)r - lead) % Z + Z) % Z : 0;
+    for (int b = (int)r; b < nblk; b += NT) {
+      const uint32_t w = blk[b];
+      int j = jb;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int p = 16 * b + k - lead;
-        const int x = (int)(int8_t)(ww[k >> 2] >> (8 * (k & 3)));
+      for (int k = 0; k < 4; ++k) {
+        const int p = 4 * b + k - lead;
+        const int x = (int)(int8_t)(w >> (8 * k));
         if (p >= 0 && p < n_in) {
           last = x != 0 ? p : last;
           if (p < cap) {
-            soft[2 * Z + p] = (int8_t)(p < full ? min(max(x, -64), 64) : x);
+            soft[soft_pos<PAIR>(2 * Z + p, j, Z)] = (int8_t)(p < full ? min(max(x, -64), 64) : med3i(x, -121, 121));
           }
+        }
+        if (PAIR) {
+          j = j + 1 == Z ? 0 : j + 1;
         }
       }
     }
   }
   if (work) {
-    for (int i = (int)r; i < 2 * Z; i += Z) {
+    for (int i = (int)r; i < 2 * Z; i += NT) {
       soft[i] = 0;
     }
-    for (int i = n_in + (int)r; i < cap; i += Z) {
-      soft[2 * Z + i] = 0;
+    int j = PAIR ? (n_in + (int)r) % Z : 0;
+    for (int i = n_in + (int)r; i < cap; i += NT) {
+      soft[soft_pos<PAIR>(2 * Z + i, j, Z)] = 0;
+      if (PAIR) {
+        j = j + NT >= Z ? j + NT - Z : j + NT;
+      }
     }
   }
   group_max(last, g, work, ctl->last);
@@ -566,7 +787,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     } else if (input_size < msg && !(jb.flags & kFlagForce)) {
       // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
       if (jb.crc_idx == 0 || (jb.flags & kFlagCrcAfter)) {
-        write_hard_bits(soft, out, msg, true, (int)r, Z);
+        write_hard_bits<PAIR>(soft, out, msg, true, (int)r, NT, Z);
       }
       done = true;
       result = 0;
@@ -582,7 +803,13 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   }
   if (active && !done) {
     for (int l = 4; l < L; ++l) {
-      ext[(l - 4) * Z + (int)r] = 0;
+      // One state word per check row: thread r owns word r (words 2r, 2r + 1 with PAIR).
+      if (PAIR) {
+        ext[(l - 4) * Z + 2 * (int)r] = 0;
+        ext[(l - 4) * Z + 2 * (int)r + 1] = 0;
+      } else {
+        ext[(l - 4) * Z + (int)r] = 0;
+      }
     }
   }
   if (active && r == 0) {
@@ -594,8 +821,12 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   }
 
   // Core-layer check-to-variable messages, int8 packed four per word (VGPRs).
-  constexpr int CW0 = (bgt<BG>::rs(1) - bgt<BG>::rs(0) + 3) / 4, CW1 = (bgt<BG>::rs(2) - bgt<BG>::rs(1) + 3) / 4;
-  constexpr int CW2 = (bgt<BG>::rs(3) - bgt<BG>::rs(2) + 3) / 4, CW3 = (bgt<BG>::rs(4) - bgt<BG>::rs(3) + 3) / 4;
+  // Words per core row: four int8 messages per word, or two edges of the row pair.
+  constexpr int EPW = PAIR ? 2 : 4;
+  constexpr int CW0 = (bgt<BG>::rs(1) - bgt<BG>::rs(0) + EPW - 1) / EPW;
+  constexpr int CW1 = (bgt<BG>::rs(2) - bgt<BG>::rs(1) + EPW - 1) / EPW;
+  constexpr int CW2 = (bgt<BG>::rs(3) - bgt<BG>::rs(2) + EPW - 1) / EPW;
+  constexpr int CW3 = (bgt<BG>::rs(4) - bgt<BG>::rs(3) + EPW - 1) / EPW;
   uint32_t c0[CW0], c1[CW1], c2[CW2], c3[CW3];
 #pragma unroll
   for (int k = 0; k < CW0; ++k) c0[k] = 0;
@@ -607,11 +838,12 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
   for (int k = 0; k < CW3; ++k) c3[k] = 0;
 
   // CRC weights: hard bit i of the message contributes x^(len-1-i+order) mod G; thread r
-  // reads the weights of bits n*Z + r (n < K) from the table when it checks the CRC.
+  // reads the weights of its bits n*Z + rr (n < K) from the table when it checks the CRC
+  // (index clamped at 0 for bits beyond len, whose weights are masked off).
   const int len = msg - (int)jb.F;
   const int order = jb.crc_idx == 3 ? 16 : 24;
-  const uint32_t* __restrict__ xtab =
-      crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0) + (len - 1 + order) - (int)r;
+  const uint32_t* __restrict__ xbase = crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0);
+  const int xi0 = (len - 1 + order) - (int)r; // weight index of bit r
   __syncthreads();
   const int Litem = ctl->lmax_item;
 
@@ -622,44 +854,66 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     }
     const bool run = !done;
     const int par = it & 1;
+    // The edge table is re-read every iteration: values derived from it stay in the
+    // layer that uses them instead of being hoisted out of the loop into registers.
+    // Row pairs read the two-word row-pair edge table behind the plain one.
+    int gto = PAIR ? kGraphTableLen + 2 * (int)jb.gt_off : (int)jb.gt_off;
+    asm volatile("" : "+v"(gto));
+    const uint32_t* __restrict__ gti = gtab + __builtin_amdgcn_readfirstlane(gto);
     // Learned tables: one row of M (A, B) words per iteration, the last row repeated.
     const uint32_t* abt = LEARNED ? scale_tab + min(it, kLearnedIters - 1) * T::M : nullptr;
     if (run) {
-      core_layer<BG, 0, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c0, abt);
+      core_layer<BG, 0, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c0, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 1, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c1, abt);
+      core_layer<BG, 1, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c1, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 2, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c2, abt);
+      core_layer<BG, 2, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c2, abt);
     }
     __syncthreads();
     if (run) {
-      core_layer<BG, 3, LEARNED>(soft0, goff, gt, r, (unsigned)Z, c3, abt);
+      core_layer<BG, 3, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c3, abt);
     }
     __syncthreads();
     for (int l = 4; l < Litem; ++l) {
       if (run && l < L) {
         const int e0 = T::rs(l), dg = T::rs(l + 1) - e0;
         const uint32_t ab = LEARNED ? abt[l] : 0u;
-        uint32_t* st = &ext[(l - 4) * Z + (int)r];
+        uint32_t* st = &ext[(l - 4) * Z + (PAIR ? 2 * (int)r : (int)r)];
         // Degree classes (uniform branch): the unrolled update does the work of its
         // class's largest degree, so small rows take a small instance.
-        if (BG == 1) {
-          if (dg <= 5) {
-            layer_update<5, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
-          } else if (dg <= 7) {
-            layer_update<7, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+        if constexpr (PAIR) {
+          if (BG == 1) {
+            if (dg <= 5) {
+              layer_update_pair<5, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            } else if (dg <= 7) {
+              layer_update_pair<7, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            } else {
+              layer_update_pair<10, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            }
           } else {
-            layer_update<10, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+            if (dg <= 4) {
+              layer_update_pair<4, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            } else {
+              layer_update_pair<6, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            }
+          }
+        } else if (BG == 1) {
+          if (dg <= 5) {
+            layer_update<5, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+          } else if (dg <= 7) {
+            layer_update<7, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+          } else {
+            layer_update<10, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
           }
         } else {
           if (dg <= 4) {
-            layer_update<4, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<4, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
           } else {
-            layer_update<6, true, false, LEARNED>(soft0, goff, gt + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<6, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
           }
         }
       }
@@ -673,22 +927,41 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
     uint32_t x = 0, z = 0;
     if (check) {
       const bool want_crc = use_crc && (es_crc || last_it);
-      const uint32_t* xt = xtab;
-      asm volatile("" : "+v"(xt)); // re-read the weights each check (keeps them out of VGPRs)
-      int s[K];
-      uint32_t xw[K];
+      // Check rows of this thread: r, and r + Z/2 with row pairs, eight message columns at
+      // a time (loads in flight without holding all K in registers).  Row rr is byte rr of
+      // a column, or byte 2r + half with row pairs.
+#pragma unroll 1
+      for (int half = 0; half < (PAIR ? 2 : 1); ++half) {
+        const int rr = (int)r + half * NT;
+        // Opaque base and index: the weight addresses are formed at each check instead of
+        // being hoisted out of the iteration loop into registers.
+        const uint32_t* xt = xbase;
+        int xi = xi0 - half * NT;
+        asm volatile("" : "+v"(xt), "+v"(xi));
+        const int8_t* sc = soft0 + goff + (PAIR ? 2 * (int)r + half : rr);
+#pragma unroll 1
+        for (int n0 = 0; n0 < K; n0 += 8) {
+          int sv[8];
+          uint32_t xw[8];
 #pragma unroll
-      for (int n = 0; n < K; ++n) {
-        s[n] = soft0[goff + (int)r + n * Z];
-        xw[n] = (want_crc && n * Z + (int)r < len) ? xt[-n * Z] : 0u;
-      }
+          for (int k = 0; k < 8; ++k) {
+            const int n = n0 + k;
+            sv[k] = n < K ? sc[n * Z] : 1;
+            xw[k] = (n < K && want_crc && n * Z + rr < len) ? xt[max(xi - n * Z, 0)] : 0u;
+          }
 #pragma unroll
-      for (int n = 0; n < K; ++n) {
-        z |= (s[n] == 0);
-        x ^= (s[n] <= 0) ? xw[n] : 0u;
+          for (int k = 0; k < 8; ++k) {
+            z |= (sv[k] == 0);
+            x ^= xw[k] & (uint32_t)((sv[k] - 1) >> 31); // weight of a hard one (soft <= 0)
+          }
+        }
       }
       if (!es_crc) {
-        z |= syndrome_fail<BG>(soft0, goff, gt, r, (unsigned)Z, L);
+        if (PAIR) {
+          z |= syndrome_fail_pair<BG>(soft0, goff, gti, r, L);
+        } else {
+          z |= syndrome_fail<BG>(soft0, goff, gti, r, (unsigned)Z, L);
+        }
       }
     }
     group_xor_or(x, z, g, check, ctl->crc[par], ctl->zf[par]);
@@ -731,7 +1004,7 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
 
   if (active && !skip) {
     if (result >= 0 && !(input_size < msg && !(jb.flags & kFlagForce))) {
-      write_hard_bits(soft, out, msg, false, (int)r, Z);
+      write_hard_bits<PAIR>(soft, out, msg, false, (int)r, NT, Z);
     }
     if (r == 0) {
       results[jb.out_idx] = result;
@@ -743,21 +1016,17 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
 // Persistent decoder: workgroups pull work items from the (BG, Z)-sorted queue until it
 // is empty.  Items were sorted largest first by the host, so the dynamic pull balances
 // uneven codeblocks.
-template <bool LEARNED>
-__global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(4)))
-k_ldpc_decode(launch_header* __restrict__ hdr,
-                                                             const dec_item* __restrict__ items,
-                                                             const dec_job* __restrict__ jobs,
-                                                             int8_t* __restrict__ rm_base,
-                                                             const int8_t* __restrict__ cw_base,
-                                                             uint8_t* __restrict__ bits_base,
-                                                             int32_t* __restrict__ results,
-                                                             const uint32_t* __restrict__ crc_x,
-                                                             const uint32_t* __restrict__ gtab,
-                                                             const uint32_t* __restrict__ scale_tab, int fb_iters)
+// One persistent decoder per row mode: the row-pair instance holds more state per thread
+// and runs at three waves per SIMD (168 VGPRs, no spills), the one-row instance at four.
+template <bool LEARNED, bool PAIR>
+__global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(PAIR ? 3 : 4)))
+k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ items, const dec_job* __restrict__ jobs,
+              int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
+              int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab,
+              const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot)
 {
   extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ uint32_t item_id;
+  uint32_t& item_id = *reinterpret_cast<uint32_t*>(lds + qslot);
   const uint32_t n_items = hdr->n_items;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -769,13 +1038,13 @@ k_ldpc_decode(launch_header* __restrict__ hdr,
     if (id >= n_items) {
       break;
     }
-    const dec_item item = items[id];
+    const dec_item item = items[hdr->first_item + id];
     if (item.bg == 1) {
-      decode_item<1, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab, fb_iters,
-                              lds);
+      decode_item<1, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab,
+                                    fb_iters, lds);
     } else {
-      decode_item<2, LEARNED>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
-                              LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
+      decode_item<2, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
+                                    LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
     }
   }
 }
@@ -907,33 +1176,48 @@ __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __res
 size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
 {
   const int K = bg == 1 ? 22 : 10;
-  return (size_t)G * ((((K + lmax) * Z + 15) & ~15) + 4 * (lmax - 4) * Z + stage) + kCtlBytes;
+  return kQueueBytes + (size_t)G * ((((K + lmax) * Z + 15) & ~15) + 4 * (lmax - 4) * Z + stage) + kCtlBytes;
 }
 
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         hipStream_t s)
+                         bool pairs, hipStream_t s)
 {
-  const void* fn = scale_tab ? (const void*)k_ldpc_decode<true> : (const void*)k_ldpc_decode<false>;
+  const bool learned = scale_tab != nullptr;
+  const void* fn = pairs ? (learned ? (const void*)k_ldpc_decode<true, true> : (const void*)k_ldpc_decode<false, true>)
+                         : (learned ? (const void*)k_ldpc_decode<true, false> : (const void*)k_ldpc_decode<false, false>);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
     return e;
   }
-  if (scale_tab) {
-    hipLaunchKernelGGL(k_ldpc_decode<true>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
-                       results, crc_x, gtab, scale_tab, fb_iters);
+  // lds (from decode_item_lds_bytes, a multiple of 4) ends with the work-queue slot.
+  const unsigned qslot = (unsigned)lds - kQueueBytes;
+#define LH_LAUNCH(L, P)                                                                                           \
+  hipLaunchKernelGGL((k_ldpc_decode<L, P>), dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, \
+                     results, crc_x, gtab, scale_tab, fb_iters, qslot)
+  if (pairs) {
+    if (learned) {
+      LH_LAUNCH(true, true);
+    } else {
+      LH_LAUNCH(false, true);
+    }
   } else {
-    hipLaunchKernelGGL(k_ldpc_decode<false>, dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits,
-                       results, crc_x, gtab, scale_tab, fb_iters);
+    if (learned) {
+      LH_LAUNCH(true, false);
+    } else {
+      LH_LAUNCH(false, false);
+    }
   }
+#undef LH_LAUNCH
   return hipGetLastError();
 }
 
-int decode_occupancy(int nt, size_t lds, bool learned)
+int decode_occupancy(int nt, size_t lds, bool learned, bool pairs)
 {
   int n = 0;
-  const void* fn = learned ? (const void*)k_ldpc_decode<true> : (const void*)k_ldpc_decode<false>;
+  const void* fn = pairs ? (learned ? (const void*)k_ldpc_decode<true, true> : (const void*)k_ldpc_decode<false, true>)
+                         : (learned ? (const void*)k_ldpc_decode<true, false> : (const void*)k_ldpc_decode<false, false>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, lds) != hipSuccess) {
     return 1;
   }
@@ -943,7 +1227,7 @@ int decode_occupancy(int nt, size_t lds, bool learned)
 // Rewinds the work-queue counters of a resident descriptor block (graph replays).
 __global__ void k_reset_headers(launch_header* hdr)
 {
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 4) {
     hdr[threadIdx.x].next_item = 0;
   }
 }

@@ -228,7 +228,7 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
-            "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
+            "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
             "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch"]
 
 
@@ -270,6 +270,7 @@ def lib():
         L.lh_slot_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
         L.lh_slot_graph_destroy.argtypes = [C.c_void_p]
         L.lh_tune_set.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_int]
+        L.lh_tune_set_rows.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_int]
         L.lh_gen_transport_blocks.argtypes = [C.c_void_p, C.POINTER(GenTbDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
         L.lh_segment_rx.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(Segmentation),
@@ -387,6 +388,11 @@ class Context:
 
     def tune_set(self, bg, Z, groups, stage_llrs=True):
         _lh(lib().lh_tune_set(self.handle, bg, Z, groups, int(stage_llrs)), "lh_tune_set")
+
+    def tune_rows(self, bg, Z, row_pairs):
+        """Row mode of (bg, Z): -1 from occupancy, 0 one row per thread, 1 row pairs under
+        load, 2 row pairs always (even Z)."""
+        _lh(lib().lh_tune_set_rows(self.handle, bg, Z, int(row_pairs)), "lh_tune_set_rows")
 
     def gen_transport_blocks(self, descs, d_tb, d_msgs, stream=None):
         arr = (GenTbDesc * len(descs))(*descs)

@@ -11,6 +11,23 @@ def require_gpu():
     H.lib()  # raises when the library is not built
 
 
+# Row modes of the decoder under test: "auto" (chosen from occupancy and load; the small
+# test batches run one check row per thread) and "pairs" (two rows per thread forced for
+# every even lifting size).
+ROW_MODES = ["auto", "pairs"]
+
+
+def make_ctx(max_codeblocks, rows="auto"):
+    require_gpu()
+    c = H.Context(0, max_codeblocks=max_codeblocks)
+    if rows == "pairs":
+        for bg in (1, 2):
+            for Z in O.LIFTING_SIZES:
+                if Z % 2 == 0:
+                    c.tune_rows(bg, Z, 2)
+    return c
+
+
 def decode_batch_gpu(ctx, llr_list, cfgs, crc_polys, flags=0):
     """llr_list: list of int8 arrays (decoder input); cfgs: (bg, Z, F, crc_bits, max_iter)."""
     n = len(llr_list)

@@ -5,15 +5,14 @@ import pytest
 
 import ocudu_hip as H
 import oracle as O
-from gpu_util import decode_batch_gpu, require_gpu
+from gpu_util import ROW_MODES, decode_batch_gpu, make_ctx, require_gpu
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    require_gpu()
-    c = H.Context(0, 4096)
+@pytest.fixture(scope="module", params=ROW_MODES)
+def ctx(request):
+    c = make_ctx(4096, request.param)
     yield c
     c.close()
 

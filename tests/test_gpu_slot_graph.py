@@ -10,17 +10,16 @@ import pytest
 
 import ocudu_hip as H
 import oracle as O
-from gpu_util import require_gpu
+from gpu_util import ROW_MODES, make_ctx, require_gpu
 from tb_util import cb_data_bits, tb_codeblocks, tb_crc_poly
 
 pytestmark = pytest.mark.gpu
 TABLES = Path(__file__).resolve().parent.parent / "ocudu-dapp-platform_amd" / "tables"
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    require_gpu()
-    c = H.Context(0, max_codeblocks=1024)
+@pytest.fixture(scope="module", params=ROW_MODES)
+def ctx(request):
+    c = make_ctx(1024, request.param)
     yield c
     c.close()
 
