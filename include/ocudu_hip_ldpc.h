@@ -228,11 +228,11 @@ int lh_slot_graph_destroy(lh_slot_graph* graph);
  * LLRs (0 = read them from HBM).  Applies to later decode calls and graph builds. */
 int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigned groups, int stage_llrs);
 
-/* Row mode of the decoder for one (base graph, lifting size): -1 decides from occupancy
- * (default: two check rows per thread when a compute unit's resident codeblocks are
- * limited by waves rather than LDS and the launch fills every compute unit), 0 one row
- * per thread, 1 two rows per thread when the launch fills every compute unit, 2 two rows
- * per thread always.  1 and 2 need an even lifting size.  Results do not depend on it. */
+/* Row mode of the decoder for one (base graph, lifting size): 0 one check row per thread,
+ * 1 two rows per thread (even lifting size), -1 decided per launch (default: two rows per
+ * thread when every class of the launch has a compute unit's resident codeblocks limited
+ * by waves rather than LDS and the launch fills every compute unit at one row per thread).
+ * Applies to later decode calls and graph builds; results do not depend on it. */
 int lh_tune_set_rows(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, int row_pairs);
 
 /* One synthetic codeblock of lh_gen_codeblocks. */

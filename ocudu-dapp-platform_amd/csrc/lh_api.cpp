@@ -315,29 +315,30 @@ int groups_per_item(int lanes, int nt)
   return std::max(1, std::min(kMaxGroups, nt / lanes));
 }
 
-// Row mode of a codeblock class in a launch of n codeblocks: two check rows per thread
-// (even Z; half the threads, each with 1.4x the instructions of one row) when a compute
+// Whether a codeblock class in a launch of n codeblocks favours two check rows per thread
+// (even Z; half the threads, each with 1.4x the instructions of one row): when a compute
 // unit's resident codeblocks are limited by its waves rather than its LDS and the launch
 // holds enough codeblocks to fill every compute unit at one row per thread; otherwise one
-// row per thread (lower latency per codeblock).
-// The tuning table can turn the mode off, ask for it under load, or force it.
-bool pair_rows(const lh_ctx* ctx, int bg, int Z, int lbound, size_t n)
+// row per thread has the lower latency per codeblock.
+// A tuned row mode (lh_tune_set_rows) overrides the choice.  Resident codeblocks per
+// compute unit: 16 waves of 64 threads at one row per thread (128 VGPRs), 12 waves at two
+// (168 VGPRs, Z/2 threads per codeblock), i.e. at most 1.5x as many; with 1.4x the
+// instructions per codeblock the mode needs the LDS to hold nearly all of them.
+bool pair_rows(const lh_ctx* ctx, const dec_job& j, size_t n)
 {
+  const int bg = j.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
+  const int Z = (int)j.Z;
   const tune_entry& te = ctx->tune[bg - 1][lifting_position((unsigned)Z)];
   if ((Z & 1) || te.pairs == 0) {
     return false;
   }
-  if (te.pairs == 2) {
+  if (te.pairs == 1) {
     return true;
   }
-  // 16 resident waves of 64 threads per compute unit at the decoder's 128 VGPRs.
+  const int stage = ((j.flags & kFlagFused) && te.stage != 0) ? (((int)j.rm.E + 15) & ~15) : 0;
   const double by_waves = 1024.0 / Z;
-  const bool load = (double)n >= by_waves * ctx->nof_cus;
-  if (te.pairs == 1) {
-    return load;
-  }
-  const double by_lds = (double)kLdsBudget / (double)decode_item_lds_bytes(bg, Z, lbound, 1, 0);
-  return by_lds > 1.25 * by_waves && load;
+  const double by_lds = (double)kLdsBudget / (double)decode_item_lds_bytes(bg, Z, j.lbound, 1, stage);
+  return by_lds >= 1.45 * by_waves && (double)n >= by_waves * ctx->nof_cus;
 }
 
 // A packed decoder launch: four work-queue headers (pass 1 row-pair items, pass 1 one-row
@@ -360,9 +361,20 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
 {
   std::vector<dec_job>& jobs = plan.jobs;
   const size_t n_cbs = jobs.size();
+  // Row mode: a class with a tuned mode keeps it; the others run row pairs when every
+  // class of the launch favours them (a mixed launch runs its two queues one after the
+  // other), else one row per thread.
+  bool pairs_launch = !jobs.empty();
+  for (const dec_job& j : jobs) {
+    if (!pair_rows(ctx, j, n_cbs)) {
+      pairs_launch = false;
+      break;
+    }
+  }
   auto pairs_of_job = [&](const dec_job& j) {
     const int bg = j.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
-    return pair_rows(ctx, bg, (int)j.Z, j.lbound, n_cbs);
+    const int tuned = ctx->tune[bg - 1][lifting_position(j.Z)].pairs;
+    return tuned >= 0 ? tuned == 1 : pairs_launch;
   };
   // Workgroup size per row mode: the most threads per codeblock, or a tuned packing of G
   // codeblocks of one (BG, Z), rounded up to whole waves.
@@ -1067,7 +1079,7 @@ int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigne
 int lh_tune_set_rows(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, int row_pairs)
 {
   if (ctx == nullptr || (base_graph != 1 && base_graph != 2) || lifting_position(lifting_size) < 0 ||
-      row_pairs < -1 || row_pairs > 2 || (row_pairs > 0 && (lifting_size & 1))) {
+      row_pairs < -1 || row_pairs > 1 || (row_pairs == 1 && (lifting_size & 1))) {
     return fail(LH_ERR_INVALID, "invalid arguments to lh_tune_set_rows");
   }
   ctx->tune[base_graph - 1][lifting_position(lifting_size)].pairs = (int8_t)row_pairs;

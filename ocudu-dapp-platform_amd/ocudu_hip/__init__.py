@@ -390,8 +390,8 @@ class Context:
         _lh(lib().lh_tune_set(self.handle, bg, Z, groups, int(stage_llrs)), "lh_tune_set")
 
     def tune_rows(self, bg, Z, row_pairs):
-        """Row mode of (bg, Z): -1 from occupancy, 0 one row per thread, 1 row pairs under
-        load, 2 row pairs always (even Z)."""
+        """Row mode of (bg, Z): -1 chosen per launch from occupancy and load, 0 one check row
+        per thread, 1 two rows per thread (even Z)."""
         _lh(lib().lh_tune_set_rows(self.handle, bg, Z, int(row_pairs)), "lh_tune_set_rows")
 
     def gen_transport_blocks(self, descs, d_tb, d_msgs, stream=None):

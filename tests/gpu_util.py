@@ -24,7 +24,7 @@ def make_ctx(max_codeblocks, rows="auto"):
         for bg in (1, 2):
             for Z in O.LIFTING_SIZES:
                 if Z % 2 == 0:
-                    c.tune_rows(bg, Z, 2)
+                    c.tune_rows(bg, Z, 1)
     return c
 
 
