@@ -283,7 +283,9 @@ int make_rm_params(const lh_rm_desc& d, bool new_data, rm_params& j, uint64_t& m
   j.P = Ncb - F;
   j.start = k0 < j.info ? k0 : (k0 < nsys ? j.info : k0 - F);
   j.Kq = d.rm_length / qm;
-  j.rcp_kq = 1.0f / (float)j.Kq;
+  // Reciprocal biased up by 2^-21: (float)i * rcp_kq then truncates to i / Kq exactly for
+  // i, Kq < 2^19 (the product's rounding errors stay below the bias).
+  j.rcp_kq = (float)((1.0 + 1.0 / 2097152.0) / (double)j.Kq);
   j.Qm = (uint16_t)qm;
   j.flags = new_data ? kFlagNewData : 0;
   if (!new_data) {

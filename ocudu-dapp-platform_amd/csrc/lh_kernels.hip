@@ -521,7 +521,83 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
 {
   const bool al4 = ((reinterpret_cast<uintptr_t>(harq)) & 3) == 0;
   const int j0 = PAIR ? (4 * (int)r) % Z : 0;
-  for (int p0 = 4 * (int)r; p0 < n_in; p0 += 4 * (int)NT) {
+  int p_from = 4 * (int)r; // first position of the general loop below
+  if (new_data && rp.E <= rp.P) {
+    // New data in one pass over the circular buffer (E <= P): position p receives at
+    // most the LLR e[i] = src[(i mod Kq) Qm + i div Kq], i its rank from k0 among the
+    // non-filler positions (the saturated sum onto 0 is the LLR itself); fillers are
+    // +127 and every other position 0.  rcp_kq is biased so that the float quotient
+    // truncates to i div Kq exactly (i, Kq < 2^19).
+    const uint32_t F = rp.nsys - rp.info;
+    // Positions from `tail` on hold 0: past the last LLR and the fillers, or past Ncb.
+    uint32_t tail = rp.Ncb;
+    if (rp.start + rp.E <= rp.P) {
+      const uint32_t ql = rp.start + rp.E - 1;
+      tail = max(ql < rp.info ? ql + 1 : ql + F + 1, rp.nsys);
+    }
+    const int n_main = min(n_in, (int)((tail + 3) & ~3u));
+    for (int p0 = 4 * (int)r; p0 < n_main; p0 += 4 * (int)NT) {
+      uint32_t outw = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t p = (uint32_t)(p0 + k);
+        if ((int)p < n_in) {
+          int x = 0;
+          if (p - rp.info < F) {
+            x = 127;
+          } else if (p < rp.Ncb) {
+            const uint32_t q = p < rp.info ? p : p - F;
+            const uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start;
+            if (i < rp.E) {
+              const uint32_t qt = (uint32_t)((float)i * rp.rcp_kq);
+              x = src[(i - qt * rp.Kq) * rp.Qm + qt];
+            }
+          }
+          outw |= (uint32_t)(uint8_t)x << (8 * k);
+          last = x != 0 ? (int)p : last;
+          if ((int)p < cap) {
+            int j = j0 + k;
+            j = j >= Z ? j - Z : j;
+            soft2[soft_pos<PAIR>((int)p, j, Z)] = (int8_t)min(max(x, -64), 64);
+          }
+        }
+      }
+      if (wr) {
+        if (al4 && p0 + 4 <= n_in) {
+          *reinterpret_cast<uint32_t*>(harq + p0) = outw;
+        } else {
+          for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
+            harq[p0 + k] = (int8_t)(outw >> (8 * k));
+          }
+        }
+      }
+    }
+    // The zero tail: soft bits below cap and the HARQ buffer.
+    const int jt0 = PAIR ? (n_main + 4 * (int)r) % Z : 0;
+    for (int p0 = n_main + 4 * (int)r; p0 < n_in; p0 += 4 * (int)NT) {
+      if (p0 < cap) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (p0 + k < min(n_in, cap)) {
+            int j = jt0 + k;
+            j = j >= Z ? j - Z : j;
+            soft2[soft_pos<PAIR>(p0 + k, j, Z)] = 0;
+          }
+        }
+      }
+      if (wr) {
+        if (al4 && p0 + 4 <= n_in) {
+          *reinterpret_cast<uint32_t*>(harq + p0) = 0u;
+        } else {
+          for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
+            harq[p0 + k] = 0;
+          }
+        }
+      }
+    }
+    p_from = n_in; // nothing left for the general loop
+  }
+  for (int p0 = p_from; p0 < n_in; p0 += 4 * (int)NT) {
     uint32_t oldw = 0;
     if (!new_data) {
       if (al4 && p0 + 4 <= n_in) {
@@ -704,8 +780,19 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
       const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
       const int lead = (int)(a & 15);
       const int nblk = (lead + (int)rp.E + 15) >> 4;
+      // The stage of a group starts at a multiple of 4 bytes: with a 16-byte aligned
+      // codeblock, whole blocks go in as four dwords.
+      const bool whole = lead == 0;
       for (int b = (int)r; b < nblk; b += NT) {
         const uint4 w = blk[b];
+        if (whole && 16 * b + 16 <= (int)rp.E) {
+          uint32_t* st4 = reinterpret_cast<uint32_t*>(stage + 16 * b);
+          st4[0] = w.x;
+          st4[1] = w.y;
+          st4[2] = w.z;
+          st4[3] = w.w;
+          continue;
+        }
         const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
