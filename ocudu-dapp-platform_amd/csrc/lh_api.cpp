@@ -139,7 +139,8 @@ struct tune_entry {
 struct slot {
   uint8_t* host = nullptr;
   uint8_t* dev = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t done = nullptr;   // the kernels that read the slot have completed
+  hipEvent_t copied = nullptr; // the upload of the slot has completed
   bool used = false;
 };
 
@@ -158,6 +159,9 @@ struct lh_ctx {
   bool scale_set[2] = {false, false};
   slot ring[kRing];
   int next = 0;
+  // Descriptor uploads run here, so that a call's upload overlaps the kernels of the
+  // previous call on the caller's stream (default priority, non-blocking).
+  hipStream_t copy = nullptr;
 };
 
 namespace {
@@ -180,9 +184,17 @@ void retire(slot* s, hipStream_t stream)
   (void)hipEventRecord(s->done, stream);
 }
 
+// Uploads a slot on the copy stream; `stream` waits for the copy (not for earlier work
+// of the copy stream's other users beyond what the copy itself needs).
 int upload(lh_ctx* ctx, slot* s, size_t bytes, hipStream_t stream)
 {
-  hipError_t e = hipMemcpyAsync(s->dev, s->host, bytes, hipMemcpyHostToDevice, stream);
+  hipError_t e = hipMemcpyAsync(s->dev, s->host, bytes, hipMemcpyHostToDevice, ctx->copy);
+  if (e == hipSuccess) {
+    e = hipEventRecord(s->copied, ctx->copy);
+  }
+  if (e == hipSuccess) {
+    e = hipStreamWaitEvent(stream, s->copied, 0);
+  }
   if (e != hipSuccess) {
     return hip_fail(e, "descriptor upload");
   }
@@ -191,7 +203,6 @@ int upload(lh_ctx* ctx, slot* s, size_t bytes, hipStream_t stream)
     return hip_fail(e, "event record");
   }
   s->used = true;
-  (void)ctx;
   return LH_OK;
 }
 
@@ -638,6 +649,12 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
     if (e == hipSuccess) {
       e = hipEventCreateWithFlags(&c->ring[i].done, hipEventDisableTiming);
     }
+    if (e == hipSuccess) {
+      e = hipEventCreateWithFlags(&c->ring[i].copied, hipEventDisableTiming);
+    }
+  }
+  if (e == hipSuccess) {
+    e = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking);
   }
   if (e != hipSuccess) {
     lh_ctx_destroy(c);
@@ -658,12 +675,20 @@ int lh_ctx_destroy(lh_ctx* c)
       (void)hipEventSynchronize(s.done);
       (void)hipEventDestroy(s.done);
     }
+    if (s.copied) {
+      (void)hipEventSynchronize(s.copied);
+      (void)hipEventDestroy(s.copied);
+    }
     if (s.host) {
       (void)hipHostFree(s.host);
     }
     if (s.dev) {
       (void)hipFree(s.dev);
     }
+  }
+  if (c->copy) {
+    (void)hipStreamSynchronize(c->copy);
+    (void)hipStreamDestroy(c->copy);
   }
   if (c->d_crc_x) {
     (void)hipFree(c->d_crc_x);

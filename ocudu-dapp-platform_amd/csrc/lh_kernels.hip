@@ -537,28 +537,34 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
     }
     const int n_main = min(n_in, (int)((tail + 3) & ~3u));
     for (int p0 = 4 * (int)r; p0 < n_main; p0 += 4 * (int)NT) {
-      uint32_t outw = 0;
+      // Branch-free: the four gathers go out together (clamped to a valid byte when the
+      // position takes no LLR) and one wait covers them.
+      int x[4];
+      uint32_t kind = 0; // bit k: filler, bit 4+k: LLR
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t p = (uint32_t)(p0 + k);
-        if ((int)p < n_in) {
-          int x = 0;
-          if (p - rp.info < F) {
-            x = 127;
-          } else if (p < rp.Ncb) {
-            const uint32_t q = p < rp.info ? p : p - F;
-            const uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start;
-            if (i < rp.E) {
-              const uint32_t qt = (uint32_t)((float)i * rp.rcp_kq);
-              x = src[(i - qt * rp.Kq) * rp.Qm + qt];
-            }
-          }
-          outw |= (uint32_t)(uint8_t)x << (8 * k);
-          last = x != 0 ? (int)p : last;
-          if ((int)p < cap) {
+        const uint32_t q = p < rp.info ? p : p - F;
+        const uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start;
+        const bool fill = p - rp.info < F;
+        const bool llr = !fill && p < rp.Ncb && i < rp.E;
+        const uint32_t qt = (uint32_t)((float)i * rp.rcp_kq);
+        const uint32_t at = llr ? (i - qt * rp.Kq) * rp.Qm + qt : 0u;
+        x[k] = src[at];
+        kind |= ((uint32_t)fill << k) | ((uint32_t)llr << (4 + k));
+      }
+      uint32_t outw = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + k;
+        const int v = ((kind >> k) & 1u) ? 127 : (((kind >> (4 + k)) & 1u) ? x[k] : 0);
+        if (p < n_in) {
+          outw |= (uint32_t)(uint8_t)v << (8 * k);
+          last = v != 0 ? p : last;
+          if (p < cap) {
             int j = j0 + k;
             j = j >= Z ? j - Z : j;
-            soft2[soft_pos<PAIR>((int)p, j, Z)] = (int8_t)min(max(x, -64), 64);
+            soft2[soft_pos<PAIR>(p, j, Z)] = (int8_t)min(max(v, -64), 64);
           }
         }
       }
