@@ -242,7 +242,8 @@ def main():
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_ldpc_decode (fused dematch + layered min-sum, row-pair mode at this load)", "kernel_ms": round(dec_ms, 4),
+                     "kernel": "k_ldpc_decode (fused dematch + layered min-sum, row-pair mode at this load)",
+                     "kernel_ms": round(dec_ms, 4),
                      "bytes_per_launch": dec_bytes},
         "cpu_baseline": cpu,
     }
