@@ -212,6 +212,14 @@ __device__ __forceinline__ int soft_pos(int q, int j, int Z)
   return !PAIR ? q : (j < Z / 2 ? q + j : q + j - Z + 1);
 }
 typedef short i16x2 __attribute__((ext_vector_type(2)));
+// LDS bytes addressed by their 32-bit LDS address: the row-pair partner is then address
+// ^ 1 with no base to add again.
+typedef __attribute__((address_space(3))) int8_t lds_i8;
+__device__ __forceinline__ lds_i8* lds_at(uint32_t a) { return reinterpret_cast<lds_i8*>((uintptr_t)a); }
+__device__ __forceinline__ uint32_t lds_addr(const int8_t* p)
+{
+  return (uint32_t)(uintptr_t)(const lds_i8*)p;
+}
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ i16x2 as_i16x2(uint32_t w) { return __builtin_bit_cast(i16x2, w); }
@@ -253,15 +261,17 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
   uint32_t one_w = 0x00010001u;
   asm volatile("" : "+v"(one_w));
   const u16x2 one2 = as_u16x2(one_w);
-  int a1[DEG];
+  // LDS address of the group's soft bits (even: the partner byte is address ^ 1).
+  const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
+  uint32_t a1[DEG];
   i16x2 sv[DEG];
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (!RT || e < deg) {
-      a1[e] = pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + goff;
+      a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + gbase;
       i16x2 x;
-      x.x = soft[a1[e]];
-      x.y = soft[a1[e] ^ 1];
+      x.x = *lds_at(a1[e]);
+      x.y = *lds_at(a1[e] ^ 1u);
       sv[e] = x;
     }
   }
@@ -337,8 +347,8 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
         ns |= ((as_u32(c) >> 15) & 0x00010001u) << e;
       }
       const i16x2 y = clamp2(c + v[e], -121, 121); // promotion sum
-      soft[a1[e]] = (int8_t)y.x;
-      soft[a1[e] ^ 1] = (int8_t)y.y;
+      *lds_at(a1[e]) = (int8_t)y.x;
+      *lds_at(a1[e] ^ 1u) = (int8_t)y.y;
     }
   }
   if (PACKED) {
