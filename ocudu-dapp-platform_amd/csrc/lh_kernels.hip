@@ -233,6 +233,19 @@ __device__ __forceinline__ i16x2 clamp2(i16x2 x, short lo, short hi)
 {
   return __builtin_elementwise_min(__builtin_elementwise_max(x, (i16x2)lo), (i16x2)hi);
 }
+// Infinity offsets of two soft bits s in [-121, 121]: sat16(273 s) - 273 s, i.e. -266 for
+// +121, 265 for -121 and 0 for a finite soft bit (273 * 120 still fits 16 bits): the
+// clamp form of v_pk_mad_i16, then a wrapping v_pk_mad_i16 by -273, with wait states
+// around them (the compiler's hazard tracking does not look inside the asm).
+__device__ __forceinline__ i16x2 inf_offset2(i16x2 s, uint32_t k273, uint32_t km273)
+{
+  uint32_t d;
+  asm("s_nop 1\n\tv_pk_mad_i16 %0, %1, %2, 0 clamp\n\ts_nop 1\n\tv_pk_mad_i16 %0, %1, %3, %0\n\ts_nop 1"
+      : "=&v"(d)
+      : "v"(as_u32(s)), "v"(k273), "v"(km273));
+  return as_i16x2(d);
+}
+
 // Scaled magnitudes of both rows (lanes of `m`, each <= 120).
 template <bool LEARNED>
 __device__ __forceinline__ uint32_t scale_pair(uint32_t m, uint32_t ab)
@@ -261,6 +274,11 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
   uint32_t one_w = 0x00010001u;
   asm volatile("" : "+v"(one_w));
   const u16x2 one2 = as_u16x2(one_w);
+  const uint32_t k273 = 0x01110111u, km273 = 0xfeeffeefu; // (273, 273), (-273, -273)
+  // Opaque (32, 32): the key |v2c| * 32 + edge stays one v_pk_mad_u16.
+  uint32_t k32_w = 0x00200020u;
+  asm volatile("" : "+v"(k32_w));
+  const u16x2 k32 = as_u16x2(k32_w);
   // LDS address of the group's soft bits (even: the partner byte is address ^ 1).
   const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
   uint32_t a1[DEG];
@@ -303,10 +321,11 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
         c = (mag ^ neg) - neg;
       }
       const i16x2 s = sv[e];
-      const i16x2 x = clamp2(s - c, -120, 120) + (s - clamp2(s, -120, 120)) * (short)241;
+      // v2c: the saturated difference, an infinite soft bit pushed to +-(267..386).
+      const i16x2 x = clamp2(s - c, -120, 120) - inf_offset2(s, k273, km273);
       v[e] = x;
       const u16x2 ax = __builtin_bit_cast(u16x2, __builtin_elementwise_max(x, -x));
-      const u16x2 key = ax * (unsigned short)32 + (unsigned short)e;
+      const u16x2 key = ax * k32 + (u16x2)(unsigned short)e;
       m2 = __builtin_elementwise_min(m2, __builtin_elementwise_max(m1, key));
       m1 = __builtin_elementwise_min(m1, key);
       sp ^= as_u32(x);
