@@ -184,16 +184,24 @@ void retire(slot* s, hipStream_t stream)
   (void)hipEventRecord(s->done, stream);
 }
 
-// Uploads a slot on the copy stream; `stream` waits for the copy (not for earlier work
-// of the copy stream's other users beyond what the copy itself needs).
+// Uploads a slot.  A large block goes on the copy stream, which `stream` then waits for,
+// so that the copy overlaps the kernels already queued there; a small one (a single
+// slot's launch, latency-bound) goes directly on `stream`, without the event hop.
+constexpr size_t kCopyStreamBytes = 64 * 1024;
+
 int upload(lh_ctx* ctx, slot* s, size_t bytes, hipStream_t stream)
 {
-  hipError_t e = hipMemcpyAsync(s->dev, s->host, bytes, hipMemcpyHostToDevice, ctx->copy);
-  if (e == hipSuccess) {
-    e = hipEventRecord(s->copied, ctx->copy);
-  }
-  if (e == hipSuccess) {
-    e = hipStreamWaitEvent(stream, s->copied, 0);
+  hipError_t e;
+  if (bytes < kCopyStreamBytes) {
+    e = hipMemcpyAsync(s->dev, s->host, bytes, hipMemcpyHostToDevice, stream);
+  } else {
+    e = hipMemcpyAsync(s->dev, s->host, bytes, hipMemcpyHostToDevice, ctx->copy);
+    if (e == hipSuccess) {
+      e = hipEventRecord(s->copied, ctx->copy);
+    }
+    if (e == hipSuccess) {
+      e = hipStreamWaitEvent(stream, s->copied, 0);
+    }
   }
   if (e != hipSuccess) {
     return hip_fail(e, "descriptor upload");

@@ -266,10 +266,12 @@ def calibrate(ctx, bg, Z, E, qm, n=512, seed=1, iters=ITERS, lo=-4.0, hi=30.0, t
     return out
 
 
-def train(ctx, bg, Z, configs, total_cbs, batch, seed=0, iters=ITERS, lr=0.01, log=print):
+def train(ctx, bg, Z, configs, total_cbs, batch, seed=0, iters=ITERS, lr=0.01, log=print, weights=None):
     """configs: list of (rate, Qm, snr_lo, snr_hi); each codeblock draws a configuration
-    and an SNR uniformly.  Returns the trainer."""
-    tr = Trainer(ctx, bg, iters=iters, lr=lr)
+    and an SNR uniformly.  weights: per-iteration loss weights (uniform by default; larger
+    early weights reward early convergence, i.e. fewer iterations under CRC early stop).
+    Returns the trainer."""
+    tr = Trainer(ctx, bg, iters=iters, lr=lr, weights=weights)
     rng = np.random.default_rng(seed)
     steps = max(1, total_cbs // batch)
     t0 = time.time()
@@ -336,6 +338,9 @@ def _main(argv=None):
     t.add_argument("--lr", type=float, default=0.01)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", required=True)
+    t.add_argument("--weights", help="per-iteration loss weights, comma-separated (default uniform)")
+    t.add_argument("--snr-hi-extra", type=float, default=0.0,
+                   help="extend each configuration's training SNR range upward by this many dB")
     t.add_argument("--curves", help="also write BLER curves (markdown) of the trained table here")
     args = ap.parse_args(argv)
     ctx = Context(0, max_codeblocks=max(8192, args.batch))
@@ -352,10 +357,14 @@ def _main(argv=None):
             s30, s1 = calibrate(ctx, args.bg, args.Z, E, q)
             log(f"rate {r} Qm {q} E {E}: plain BLER 30% at {s30:.2f} dB, 1% at {s1:.2f} dB")
             configs.append((r, q, s30, s1))
-    tr = train(ctx, args.bg, args.Z, configs, args.cbs, args.batch, seed=args.seed, lr=args.lr, log=log)
+    weights = [float(x) for x in args.weights.split(",")] if args.weights else None
+    train_cfgs = [(r, q, lo, hi + args.snr_hi_extra) for r, q, lo, hi in configs]
+    tr = train(ctx, args.bg, args.Z, train_cfgs, args.cbs, args.batch, seed=args.seed, lr=args.lr, log=log,
+               weights=weights)
     aq, bq = tr.quantized()
     save_table(args.out, args.bg, aq, bq, {"Z": args.Z, "configs": configs, "codeblocks": args.cbs,
-                                          "batch": args.batch, "lr": args.lr})
+                                          "batch": args.batch, "lr": args.lr, "weights": weights,
+                                          "snr_hi_extra": args.snr_hi_extra})
     log(f"table written to {args.out}")
     if args.curves:
         n = 8192
