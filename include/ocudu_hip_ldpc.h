@@ -225,7 +225,8 @@ int lh_slot_graph_destroy(lh_slot_graph* graph);
 
 /* Launch tuning of one (base graph, lifting size), as an autotuner found it: codeblocks
  * packed per workgroup (0 = as many as fit the workgroup) and LDS staging of the codeword
- * LLRs (0 = read them from HBM).  Applies to later decode calls and graph builds. */
+ * LLRs (0 = read them from HBM, the default).  Applies to later decode calls and graph
+ * builds. */
 int lh_tune_set(lh_ctx* ctx, unsigned base_graph, unsigned lifting_size, unsigned groups, int stage_llrs);
 
 /* Row mode of the decoder for one (base graph, lifting size): 0 one check row per thread,

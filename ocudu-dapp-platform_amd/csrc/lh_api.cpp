@@ -132,7 +132,7 @@ constexpr int kRing = 16;
 // and LDS staging of the codeword LLRs (1 = when it fits, 0 = never).
 struct tune_entry {
   uint16_t groups = 0;
-  uint8_t stage = 1;
+  uint8_t stage = 0; // LDS staging of the codeword LLRs (off: the one-pass gathers read HBM)
   int8_t pairs = -1; // row mode (lh_tune_set_rows)
 };
 
@@ -405,7 +405,10 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     const tune_entry& te = ctx->tune[bg - 1][lifting_position(j.Z)];
     const int m = pairs_of_job(j) ? 1 : 0;
     const int lanes = m ? (int)j.Z / 2 : (int)j.Z;
-    nt[m] = std::max(nt[m], ((lanes + 63) / 64) * 64);
+    // Untuned: codeblocks of more than half a wave are packed up to 256 threads (at most
+    // 4 per workgroup); smaller ones fill one wave.
+    const int g_default = lanes > 32 ? std::min(4, std::max(1, 256 / lanes)) : 1;
+    nt[m] = std::max(nt[m], ((g_default * lanes + 63) / 64) * 64);
     if (te.groups > 0) {
       nt[m] = std::max(nt[m], ((te.groups * lanes + 63) / 64) * 64);
     }

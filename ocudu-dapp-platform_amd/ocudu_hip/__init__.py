@@ -386,7 +386,7 @@ class Context:
         return SlotGraph(self, cbs, d_cw, d_rm, d_bits, d_results, new_data, use_early_stop, nof_ldpc_iterations,
                          learned, nof_fallback_iterations, tbs, d_tb, d_tb_status)
 
-    def tune_set(self, bg, Z, groups, stage_llrs=True):
+    def tune_set(self, bg, Z, groups, stage_llrs=False):
         _lh(lib().lh_tune_set(self.handle, bg, Z, groups, int(stage_llrs)), "lh_tune_set")
 
     def tune_rows(self, bg, Z, row_pairs):

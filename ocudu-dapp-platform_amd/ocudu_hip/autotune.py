@@ -48,10 +48,10 @@ def candidates(Z, pairs=False):
 
 def tune_one(ctx, bg, Z, n_cb, seed=1, reps=3):
     """Times every (row mode, G, staging) candidate; returns the best and the default-shape
-    time (mode from occupancy, G as many as fit, staging on)."""
+    time (mode from occupancy, G as many as fit, staging off)."""
     cbs = [(rm_length(bg, Z, r, q), q, snr) for r, q, snr in MIX for _ in range(n_cb // len(MIX))]
     b = DeviceBatch(ctx, bg, Z, cbs, seed)
-    ctx.tune_set(bg, Z, 0, True)
+    ctx.tune_set(bg, Z, 0, False)
     ctx.tune_rows(bg, Z, -1)
     b.timed_decode(ctx, reps=1)  # warm-up
     default = b.timed_decode(ctx, reps=reps)

@@ -52,7 +52,7 @@ def test_uneven_mix_equals_alone(ctx):
     res_t, bits_t = _outputs(mix)
     assert np.array_equal(res_t, res_mix) and np.array_equal(bits_t, bits_mix)
     for bg, Z, *_ in ues:
-        ctx.tune_set(bg, Z, 0, 1)
+        ctx.tune_set(bg, Z, 0, 0)
     # Oracle sample.
     cw = mix.d_cw.download(np.int8, mix.d_cw.nbytes)
     for i in range(0, len(cbs), 37):
