@@ -287,7 +287,8 @@ def uneven_batch(ctx, a):
 def serving_runs(ctx, a):
     """Slot-deadline serving loop (0.5 ms slots, Poisson(24) UEs per slot): at the specified
     per-UE load (PRBs in [4, 273]) and at one carrier's load (PRBs in [4, 18]), plain and
-    learned decoders on the same traces."""
+    learned decoders on the same traces, at most 6 LDPC iterations (the reference gNB's
+    PUSCH default)."""
     from ocudu_hip import nms, serve
     tables = ROOT / "ocudu-dapp-platform_amd" / "tables"
     for bg in (1, 2):
@@ -295,7 +296,7 @@ def serving_runs(ctx, a):
             _, aq, bq, _ = nms.load_table(tables / f"nms_bg{bg}.json")
             ctx.set_scale_table(bg, aq, bq)
     thr = serve.mcs_thresholds(ctx)
-    out = {"slots": a.serve_slots, "slot_us": 500, "mean_ues": 24}
+    out = {"slots": a.serve_slots, "slot_us": 500, "mean_ues": 24, "ldpc_iterations": serve.PUSCH_LDPC_ITERATIONS}
     for name, prb_max in (("spec_load", 273), ("carrier_load", 18)):
         trace = serve.Trace(ctx, a.serve_slots, 24.0, thr, seed=a.seed, prb_max=prb_max)
         entry = {"prb_max": prb_max, "tbs": len(trace.tbs),

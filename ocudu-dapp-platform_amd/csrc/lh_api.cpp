@@ -30,7 +30,8 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
                          bool pairs, hipStream_t s);
 int decode_occupancy(int nt, size_t lds, bool learned, bool pairs);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
-                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s);
+                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s,
+                              const tb_signal* sig = nullptr);
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s);
 hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
@@ -570,7 +571,8 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   lh_ctx* c = new lh_ctx();
   c->device = device;
   c->max_cb = max_codeblocks;
-  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params) + sizeof(dec_item)) + 4096;
+  c->slot_bytes =
+      (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params) + sizeof(dec_item) + sizeof(lh_tb_desc)) + 4096;
   (void)hipDeviceGetAttribute(&c->nof_cus, hipDeviceAttributeMultiprocessorCount, device);
   std::vector<uint32_t> x(3 * (size_t)kCrcXLen);
   crc_x_table(LH_CRC24A, x.data());
@@ -1459,13 +1461,21 @@ struct srv_request {
   double t_submit = 0;
 };
 
+// A launch's results (TB verdicts | codeblock results | TB bytes) are written by the TB
+// finalize kernel straight into one pinned, device-mapped host block (offsets per launch);
+// the kernel's last workgroup then stores the launch's sequence number into `h_flag`, which
+// the host polls.  The decoder's codeblock results stay in device memory (d_res).
 struct srv_slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   uint8_t* d_bits = nullptr;
   int32_t* d_res = nullptr;
-  uint8_t* d_tb = nullptr;
-  int32_t* d_st = nullptr;
+  uint32_t* d_counter = nullptr;
+  uint8_t* h_out = nullptr;
+  uint8_t* h_out_dev = nullptr; // device address of h_out
+  uint32_t* h_flag = nullptr;
+  uint32_t* h_flag_dev = nullptr;
+  uint32_t seq = 0;
   int32_t* h_res = nullptr;
   int32_t* h_st = nullptr;
   uint8_t* h_tb = nullptr;
@@ -1522,6 +1532,14 @@ struct arena_alloc {
 };
 
 constexpr unsigned kMaxMsgBytes = (22 * 384) / 8;
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Result block of a launch of at most ncb codeblocks (and as many TBs).
+size_t srv_out_bytes(size_t ncb)
+{
+  return 2 * align256(ncb * 4) + ncb * kMaxMsgBytes;
+}
 
 } // namespace
 
@@ -1664,11 +1682,19 @@ int lh_server_create(lh_ctx* ctx, const lh_server_config* cfg, lh_server** out)
     if (e == hipSuccess) e = hipMalloc(&sl.d_bits, ncb * kMaxMsgBytes);
     if (e == hipSuccess) e = hipMemset(sl.d_bits, 0, ncb * kMaxMsgBytes);
     if (e == hipSuccess) e = hipMalloc(&sl.d_res, ncb * 4);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_tb, ncb * kMaxMsgBytes);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_st, ncb * 4);
-    if (e == hipSuccess) e = hipHostMalloc(&sl.h_res, ncb * 4, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(&sl.h_st, ncb * 4, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(&sl.h_tb, ncb * kMaxMsgBytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 4);
+    if (e == hipSuccess) e = hipMemset(sl.d_counter, 0, 4);
+    if (e == hipSuccess) {
+      e = hipHostMalloc(&sl.h_out, srv_out_bytes(ncb), hipHostMallocMapped | hipHostMallocCoherent);
+    }
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.h_out_dev), sl.h_out, 0);
+    if (e == hipSuccess) {
+      e = hipHostMalloc(&sl.h_flag, 64, hipHostMallocMapped | hipHostMallocCoherent);
+    }
+    if (e == hipSuccess) {
+      *sl.h_flag = 0;
+      e = hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.h_flag_dev), sl.h_flag, 0);
+    }
   }
   if (e != hipSuccess) {
     lh_server_destroy(sv);
@@ -1690,11 +1716,9 @@ int lh_server_destroy(lh_server* sv)
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.d_bits) (void)hipFree(sl.d_bits);
     if (sl.d_res) (void)hipFree(sl.d_res);
-    if (sl.d_tb) (void)hipFree(sl.d_tb);
-    if (sl.d_st) (void)hipFree(sl.d_st);
-    if (sl.h_res) (void)hipHostFree(sl.h_res);
-    if (sl.h_st) (void)hipHostFree(sl.h_st);
-    if (sl.h_tb) (void)hipHostFree(sl.h_tb);
+    if (sl.d_counter) (void)hipFree(sl.d_counter);
+    if (sl.h_out) (void)hipHostFree(sl.h_out);
+    if (sl.h_flag) (void)hipHostFree(sl.h_flag);
   }
   if (sv->d_arena) (void)hipFree(sv->d_arena);
   delete sv;
@@ -1837,22 +1861,37 @@ int lh_server_step(lh_server* sv)
   }
   launch_plan plan;
   int rc = pusch_jobs(cbs.data(), (unsigned)cbs.size(), 1, 1, sv->cfg.nof_ldpc_iterations, plan, nd.data());
+  if (rc == LH_OK) {
+    rc = validate_tbs(tbs.data(), (unsigned)tbs.size());
+  }
   if (rc != LH_OK) {
     return rc;
   }
   lh_ctx* ctx = sv->ctx;
   (void)hipSetDevice(ctx->device);
+  // Results block of this launch in host memory: TB verdicts, codeblock results, TB bytes.
+  const size_t res_at = align256(tbs.size() * 4), tb_at = res_at + align256(cbs.size() * 4);
+  sl->h_st = reinterpret_cast<int32_t*>(sl->h_out);
+  sl->h_res = reinterpret_cast<int32_t*>(sl->h_out + res_at);
+  sl->h_tb = sl->h_out + tb_at;
+  tb_signal sig{};
+  sig.res_out = reinterpret_cast<int32_t*>(sl->h_out_dev + res_at);
+  sig.counter = sl->d_counter;
+  sig.flag = sl->h_flag_dev;
+  sig.seq = ++sl->seq;
+  // One upload carries the TB descriptors (at the start of the staging slot) and the
+  // decoder's descriptor block behind them, ahead of the decode launch.
   slot* s = acquire_slot(ctx);
-  rc = run_plan(ctx, plan, sv->d_arena, cw_base, sl->d_bits, sl->d_res, sl->stream, s, 0,
+  const size_t tb_desc_bytes = align256(tbs.size() * sizeof(lh_tb_desc));
+  std::memcpy(s->host, tbs.data(), tbs.size() * sizeof(lh_tb_desc));
+  rc = run_plan(ctx, plan, sv->d_arena, cw_base, sl->d_bits, sl->d_res, sl->stream, s, tb_desc_bytes,
                 sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0);
-  if (rc == LH_OK) {
-    rc = lh_tb_finalize(ctx, tbs.data(), (unsigned)tbs.size(), sl->d_bits, sl->d_res, sl->d_tb, sl->d_st, sl->stream);
-  }
   hipError_t e = hipSuccess;
   if (rc == LH_OK) {
-    e = hipMemcpyAsync(sl->h_res, sl->d_res, cbs.size() * 4, hipMemcpyDeviceToHost, sl->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(sl->h_st, sl->d_st, tbs.size() * 4, hipMemcpyDeviceToHost, sl->stream);
-    if (e == hipSuccess && tbo > 0) e = hipMemcpyAsync(sl->h_tb, sl->d_tb, tbo, hipMemcpyDeviceToHost, sl->stream);
+    e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), (unsigned)tbs.size(), sl->d_bits,
+                           sl->d_res, sl->h_out_dev + tb_at, reinterpret_cast<int32_t*>(sl->h_out_dev), ctx->d_tbcrc,
+                           ctx->d_tbcrc + 256, sl->stream, &sig);
+    retire(s, sl->stream);
     if (e == hipSuccess) e = hipEventRecord(sl->done, sl->stream);
   }
   if (rc != LH_OK || e != hipSuccess) {
@@ -1892,8 +1931,10 @@ int lh_server_poll(lh_server* sv, lh_tb_result* out, unsigned max, unsigned* n_o
       }
     }
   }
+  // A launch is complete once its finalize kernel has stored the launch's sequence number
+  // (the kernel's host writes precede it).
   for (auto& sl : sv->slots) {
-    if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+    if (sl.busy && __atomic_load_n(sl.h_flag, __ATOMIC_ACQUIRE) == sl.seq) {
       retire_slot(sv, sl);
     }
   }

@@ -150,6 +150,18 @@ struct alignas(8) lh_tb_desc_dev {
 };
 static_assert(sizeof(lh_tb_desc_dev) == 40, "lh_tb_desc_dev layout");
 
+// Completion signal of a TB finalize launch whose outputs live in host memory: every
+// workgroup also copies its transport block's codeblock results to res_out, and the last
+// workgroup to finish stores seq into *flag (system scope, after all the launch's
+// writes), so the host sees the whole launch complete by reading one word.
+struct tb_signal {
+  int32_t* res_out;  // host-visible codeblock results, indexed like the decoder's
+  uint32_t* counter; // device word, zero between launches (the last workgroup rewinds it)
+  uint32_t* flag;    // host-visible completion word
+  uint32_t seq;
+  uint32_t pad;
+};
+
 inline int crc_index(uint32_t poly)
 {
   return poly == 0x1864cfbu ? 1 : poly == 0x1800063u ? 2 : poly == 0x11021u ? 3 : (poly == 0 ? 0 : -1);

@@ -17,6 +17,25 @@
 
 namespace lh {
 
+// Row starts widened to 32-bit words for runtime indices in device code: the scalar unit
+// loads whole dwords only, so the 16-bit table would be read through the vector memory
+// path (an L1 round trip on every extension layer's critical path).
+template <int N>
+struct row_start_words {
+  uint32_t v[N];
+};
+template <int N>
+constexpr row_start_words<N> widen_row_starts(const uint16_t (&a)[N])
+{
+  row_start_words<N> w{};
+  for (int i = 0; i < N; ++i) {
+    w.v[i] = a[i];
+  }
+  return w;
+}
+__constant__ const row_start_words<47> kRowStart1 = widen_row_starts(lh_bg1_row_start);
+__constant__ const row_start_words<43> kRowStart2 = widen_row_starts(lh_bg2_row_start);
+
 template <int BG>
 struct bgt;
 
@@ -24,12 +43,14 @@ template <>
 struct bgt<1> {
   static constexpr int K = 22, M = 46, NNZ = 316, DEG_EXT = 10, NSHORT = 66;
   __host__ __device__ static constexpr int rs(int i) { return lh_bg1_row_start[i]; }
+  __device__ static int rs_word(int i) { return (int)kRowStart1.v[i]; }
 };
 
 template <>
 struct bgt<2> {
   static constexpr int K = 10, M = 42, NNZ = 197, DEG_EXT = 6, NSHORT = 50;
   __host__ __device__ static constexpr int rs(int i) { return lh_bg2_row_start[i]; }
+  __device__ static int rs_word(int i) { return (int)kRowStart2.v[i]; }
 };
 
 // --- int8 log_likelihood_ratio arithmetic, values held in int -----------------------
@@ -443,7 +464,7 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, int goff, const uint32_t* 
 {
   uint32_t bad = 0;
   for (int l = 0; l < L; ++l) {
-    const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
+    const int e0 = bgt<BG>::rs_word(l), e1 = bgt<BG>::rs_word(l + 1);
     uint32_t p = 0;
     for (int e = e0; e < e1; ++e) {
       const uint32_t g = gt[e];
@@ -464,7 +485,7 @@ __device__ uint32_t syndrome_fail_pair(const int8_t* soft, int goff, const uint3
 {
   uint32_t bad = 0;
   for (int l = 0; l < L; ++l) {
-    const int e0 = bgt<BG>::rs(l), e1 = bgt<BG>::rs(l + 1);
+    const int e0 = bgt<BG>::rs_word(l), e1 = bgt<BG>::rs_word(l + 1);
     uint32_t p = 0;
     for (int e = e0; e < e1; ++e) {
       const int a = pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + goff;
@@ -737,12 +758,13 @@ constexpr int kQueueBytes = 16;
 // with PAIR), thread t works on row t mod n (and t mod n + Z/2 with PAIR) of group t / n;
 // the threads beyond G*n only take part in barriers.  Both modes use the same LDS
 // amounts (only the order of the soft bits in a column differs).
+// Inlined into each kernel instance: as a call its register state would spill.
 template <int BG, bool LEARNED, bool PAIR>
-__device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs, int8_t* __restrict__ rm_base,
-                            const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
-                            int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x,
-                            const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ scale_tab, int fb_iters,
-                            uint8_t* lds)
+__device__ __forceinline__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs,
+                                            int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base,
+                                            uint8_t* __restrict__ bits_base, int32_t* __restrict__ results,
+                                            const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab,
+                                            const uint32_t* __restrict__ scale_tab, int fb_iters, uint8_t* lds)
 {
   using T = bgt<BG>;
   constexpr int K = T::K;
@@ -1000,9 +1022,13 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
       core_layer<BG, 3, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c3, abt);
     }
     __syncthreads();
+    // Row starts of the extension layers: the next layer's end is read one layer ahead,
+    // off the critical path of the layer chain.
+    int e_cur = T::rs(4), e_nxt = T::rs(5);
     for (int l = 4; l < Litem; ++l) {
+      const int e_after = T::rs_word(min(l + 2, T::M));
       if (run && l < L) {
-        const int e0 = T::rs(l), dg = T::rs(l + 1) - e0;
+        const int e0 = e_cur, dg = e_nxt - e_cur;
         const uint32_t ab = LEARNED ? abt[l] : 0u;
         uint32_t* st = &ext[(l - 4) * Z + (PAIR ? 2 * (int)r : (int)r)];
         // Degree classes (uniform branch): the unrolled update does the work of its
@@ -1040,6 +1066,8 @@ __device__ void decode_item(const dec_item& item, const dec_job* __restrict__ jo
         }
       }
       __syncthreads();
+      e_cur = e_nxt;
+      e_nxt = e_after;
     }
 
     // Per-iteration check: CRC (ldpc_decoder_impl.cpp:112-119), syndrome (:120-127), or
@@ -1207,17 +1235,13 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 }
 
 // GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
-// TB reassembly + TB CRC24A, one workgroup per transport block (pusch_decoder_impl.cpp:612-781).
-__global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
-                                                     const uint8_t* __restrict__ cb_bits,
-                                                     const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
-                                                     int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
-                                                     const uint32_t* __restrict__ pow2_24a)
+// TB reassembly + TB CRC24A of transport block d, one workgroup (pusch_decoder_impl.cpp:612-781).
+__device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const uint8_t* __restrict__ cb_bits,
+                                                const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
+                                                int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
+                                                const uint32_t* __restrict__ pow2_24a, uint32_t* red, uint32_t* lut,
+                                                int& all_ok)
 {
-  __shared__ uint32_t red[16];
-  __shared__ uint32_t lut[256];
-  __shared__ int all_ok;
-  const lh_tb_desc_dev d = tbs[blockIdx.x];
   const int t = (int)threadIdx.x, nt = (int)blockDim.x;
   lut[t] = lut24a[t];
   if (t == 0) {
@@ -1251,29 +1275,44 @@ __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __res
     }
     return;
   }
-  // Concatenation: TB bit i comes from codeblock i / D, bit i mod D.
+  // Concatenation: TB bit i comes from codeblock i / D, bit i mod D.  Each thread builds a
+  // contiguous chunk of TB bytes and folds them into its CRC24A remainder in the same pass
+  // (table driven); the remainders are moved to their place by x^(8 * bytes after the
+  // chunk) and XOR-reduced.
   const uint32_t D = d.cb_data_bits;
-  for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint32_t i = 8 * b + (uint32_t)k;
-      const uint32_t c = i / D, j = i - c * D;
-      const uint8_t* src = first + (size_t)c * d.cb_bits_stride;
-      v |= (uint32_t)((src[j >> 3] >> (7 - (j & 7))) & 1u) << (7 - k);
-    }
-    out[b] = (uint8_t)v;
-  }
-  __threadfence_block();
-  __syncthreads();
-  // CRC24A: each thread the remainder of a chunk of bytes (table driven), moved to its
-  // place by x^(8 * bytes after the chunk), XOR-reduced.
   const uint32_t poly = 0x1864cfbu;
   const uint32_t chunk = (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
-  const uint32_t a0 = (uint32_t)t * chunk, a1 = min(nbytes, a0 + chunk);
+  const uint32_t a0 = min(nbytes, (uint32_t)t * chunk), a1 = min(nbytes, a0 + chunk);
   uint32_t rem = 0;
+  // Codeblock c and bit j of TB bit 8 * b (one division per thread, then stepped).
+  uint32_t c = (8u * a0) / D, j = 8u * a0 - c * D;
   for (uint32_t b = a0; b < a1; ++b) {
-    rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ out[b]) & 0xffu];
+    const uint8_t* src = first + (size_t)c * d.cb_bits_stride;
+    uint32_t v;
+    if (j + 8u <= D) {
+      // The eight bits lie in one codeblock: bits j..j+7 of its MSB-first bytes (the
+      // byte after is inside the codeblock's message, whose CRC follows the data).
+      const uint32_t q = j >> 3, sh = j & 7u;
+      v = ((((uint32_t)src[q] << 8) | src[q + 1]) >> (8u - sh)) & 0xffu;
+    } else {
+      v = 0;
+      uint32_t cc = c, jj = j;
+      for (int k = 0; k < 8; ++k) {
+        const uint8_t* sk = first + (size_t)cc * d.cb_bits_stride;
+        v |= (uint32_t)((sk[jj >> 3] >> (7 - (jj & 7))) & 1u) << (7 - k);
+        if (++jj == D) {
+          jj = 0;
+          ++cc;
+        }
+      }
+    }
+    out[b] = (uint8_t)v;
+    rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ v) & 0xffu];
+    j += 8u;
+    if (j >= D) {
+      j -= D;
+      ++c;
+    }
   }
   uint32_t x = 0, z = 0;
   if (a0 < a1) {
@@ -1291,6 +1330,38 @@ __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __res
       ck = (ck << 1) | ((src[j >> 3] >> (7 - (j & 7))) & 1u);
     }
     status[blockIdx.x] = (x == ck) ? 1 : -1;
+  }
+}
+
+// One workgroup per transport block.  SIGNAL: the outputs are host-visible and the launch
+// reports its completion through `sig` (no copy back, no event on the latency path).
+template <bool SIGNAL>
+__global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
+                                                     const uint8_t* __restrict__ cb_bits,
+                                                     const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
+                                                     int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
+                                                     const uint32_t* __restrict__ pow2_24a, tb_signal sig)
+{
+  __shared__ uint32_t red[16];
+  __shared__ uint32_t lut[256];
+  __shared__ int all_ok;
+  const lh_tb_desc_dev d = tbs[blockIdx.x];
+  tb_finalize_one(d, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
+  if constexpr (SIGNAL) {
+    for (uint32_t c = threadIdx.x; c < d.nof_cbs; c += blockDim.x) {
+      sig.res_out[d.first_cb + c] = cb_res[d.first_cb + c];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t before = atomicAdd(sig.counter, 1u);
+      if (before == gridDim.x - 1) {
+        // Every workgroup's writes are fenced before its increment: the launch is whole.
+        *sig.counter = 0;
+        __threadfence_system();
+        __hip_atomic_store(sig.flag, sig.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
   }
 }
 
@@ -1369,9 +1440,16 @@ hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned max
 }
 
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
-                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s)
+                              uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s,
+                              const tb_signal* sig)
 {
-  hipLaunchKernelGGL(k_tb_finalize, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2);
+  if (sig != nullptr) {
+    hipLaunchKernelGGL(k_tb_finalize<true>, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
+                       *sig);
+  } else {
+    hipLaunchKernelGGL(k_tb_finalize<false>, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
+                       tb_signal{});
+  }
   return hipGetLastError();
 }
 

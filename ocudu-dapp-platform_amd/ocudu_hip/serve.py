@@ -7,12 +7,15 @@ graph per TS38.212 7.2.2; Es/N0 near the plain decoder's 10% BLER point of its M
 that fails is retransmitted with rv2 four slots later and soft-combined in its HARQ
 buffers on the device; the HARQ process is released after the retransmission.  All
 transmissions' LLRs are generated on the device before the loop (TB generator + codeblock
-generator), so the loop itself only serves.
+generator), so the loop itself only serves.  The decoders run at most 6 LDPC iterations,
+the reference gNB's PUSCH default (apps/units/flexible_o_du/o_du_low/du_low_config.h:30,
+pusch_decoder.h:55), and the per-MCS operating points are found at that count.
 
     python -m ocudu_hip.serve --slots 2000 [--learned]
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import math
 import sys
@@ -35,6 +38,8 @@ TBS_TABLE = [24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 104, 112, 120, 128, 136, 14
              2664, 2728, 2792, 2856, 2976, 3104, 3240, 3368, 3496, 3624, 3752, 3824]
 RE_PER_PRB = 144
 HARQ_RTT_SLOTS = 4
+# Maximum LDPC iterations of the PUSCH decoder (the reference's default).
+PUSCH_LDPC_ITERATIONS = 6
 
 
 def tbs_of(n_prb, qm, r1024, layers):
@@ -62,9 +67,9 @@ def base_graph_of(tbs, r):
     return 2 if (tbs <= 292 or (tbs <= 3824 and r <= 0.67) or r <= 0.25) else 1
 
 
-def mcs_thresholds(ctx, n=128, log=None):
+def mcs_thresholds(ctx, n=128, log=None, iters=PUSCH_LDPC_ITERATIONS):
     """Es/N0 where the plain decoder's BLER is 10% for one codeblock of each MCS (Z = 384,
-    or the BG2 equivalent at low rate)."""
+    or the BG2 equivalent at low rate), at `iters` LDPC iterations."""
     from .nms import rm_length
     from .workloads import snr_for_bler
     out = []
@@ -73,7 +78,7 @@ def mcs_thresholds(ctx, n=128, log=None):
         bg = 2 if r <= 0.25 else 1
         Z = 384
         E = rm_length(bg, Z, r, qm)
-        s = snr_for_bler(ctx, bg, Z, E, qm, 0.10, n=n)
+        s = snr_for_bler(ctx, bg, Z, E, qm, 0.10, n=n, iters=iters)
         out.append(s)
         if log:
             log(f"MCS {i}: Qm {qm} R {r:.3f} BG{bg}: 10% BLER at {s:.2f} dB")
@@ -148,7 +153,7 @@ class Trace:
 
 
 def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs_per_slot=4096, slots_in_flight=3,
-          iters=8, realtime=True, log=None, timeout_s=600.0):
+          iters=PUSCH_LDPC_ITERATIONS, realtime=True, log=None, timeout_s=600.0):
     """Runs the trace through a server: returns per-transmission results and per-slot
     latencies.  realtime: slots are released every slot_us; else as fast as possible."""
     srv = Server(ctx, harq_budget, max_cbs_per_slot, slots_in_flight, iters, learned=learned,
@@ -157,72 +162,106 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
     for i, t in enumerate(trace.tbs):
         by_slot.setdefault(t["slot"], []).append(i)
     nslots = max(by_slot) + 1 if by_slot else 0
-    retx = {}  # slot -> TB indices to retransmit
-    results = {}
-    slot_done = {}
-    slot_start = {}
-    outstanding = {}  # slot -> number of transmissions not completed
     tx_slot = {}
     out_bufs = {}
     # New transmissions' requests and TB buffers prepared ahead (the loop only submits).
     prepared = {}
     for k, idx in by_slot.items():
-        ids = [(i, 0) for i in idx]
-        prepared[k] = (ids, (TbRequest * len(ids))(*[trace.request(i, 0) for i in idx]), (C.c_void_p * len(ids))())
         for i in idx:
             out_bufs[2 * i] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
+            tx_slot[2 * i] = k
+        prepared[k] = ((TbRequest * len(idx))(*[trace.request(i, 0) for i in idx]),
+                       (C.c_void_p * len(idx))(*[out_bufs[2 * i].ctypes.data for i in idx]))
+    # No garbage-collector pauses inside the slot loop: the objects made so far (the trace)
+    # are frozen out of the collector's scans and collection resumes after the loop.
+    gc.collect()
+    gc.freeze()
+    gc_was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        results, slot_done, slot_start, wall = _serve_loop(srv, trace, prepared, by_slot, out_bufs, tx_slot, slot_us,
+                                                           realtime, timeout_s, nslots)
+    finally:
+        if gc_was_enabled:
+            gc.enable()
+        gc.unfreeze()
+    srv.close()
+    slot_lat = [(slot_done[s] - slot_start[s]) * 1e6 for s in slot_done if s in slot_start]
+    return results, slot_lat, wall, out_bufs
+
+
+def _serve_loop(srv, trace, prepared, by_slot, out_bufs, tx_slot, slot_us, realtime, timeout_s, nslots):
+    retx = {}  # slot -> [(TB index, request)] of the retransmissions due in it (built ahead)
+    results = {}
+    slot_done = {}
+    slot_start = {}
+    outstanding = {}  # slot -> number of transmissions not completed (open slots only)
     t0 = time.perf_counter()
     k = 0
     last_slot = nslots + HARQ_RTT_SLOTS + 2
-    while k < last_slot or any(outstanding.values()):
+
+    def complete(r):
+        results[r.id] = r
+        s = tx_slot[r.id]
+        outstanding[s] -= 1
+        if outstanding[s] == 0:
+            del outstanding[s]
+            slot_done[s] = time.perf_counter()
+        i, tx = divmod(r.id, 2)
+        if tx == 0 and r.tb_crc_ok != LH_TB_CRC_OK:
+            due_slot = max(s + HARQ_RTT_SLOTS, k)
+            if due_slot < last_slot:
+                # The retransmission's request and TB buffer are made now, not at its slot.
+                out_bufs[2 * i + 1] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
+                retx.setdefault(due_slot, []).append((i, trace.request(i, 1)))
+            else:
+                srv.harq_release(i)  # past the run: no retransmission
+        elif tx == 1 and r.tb_crc_ok != LH_TB_CRC_OK:
+            srv.harq_release(i)
+
+    while k < last_slot or outstanding:
+        if realtime:
+            # Completions first: a slot boundary's submission does not delay them.
+            for r in srv.poll():
+                complete(r)
         now = time.perf_counter()
         # Without the clock, slot k is released once the slots whose retransmissions are due
         # in it have completed (the HARQ round trip), so the run is deterministic.
-        due = all(outstanding.get(s, 0) == 0 for s in range(max(0, k - HARQ_RTT_SLOTS + 1)))
+        due = realtime or not outstanding or min(outstanding) >= k - HARQ_RTT_SLOTS + 1
         if not realtime and not due:
             # Requests waiting for HARQ memory with nothing in flight: only time frees it
             # (a failed TB holds its buffers until its retransmission), so release a slot.
             st = srv.stats()
             due = st.slots_in_flight == 0 and st.ready == 0
-        if time.perf_counter() - t0 > timeout_s:
+        if now - t0 > timeout_s:
             srv.close()
             raise RuntimeError(f"serving loop exceeded {timeout_s} s at slot {k}")
         if k < last_slot and ((realtime and now >= t0 + k * slot_us * 1e-6) or (not realtime and due)):
-            ids = prepared.get(k, ([], None, None))[0] + [(i, 1) for i in retx.pop(k, [])]
-            if ids:
-                if k in prepared and len(ids) == len(prepared[k][0]):
-                    _, reqs, outs = prepared.pop(k)
-                else:
-                    reqs = (TbRequest * len(ids))(*[trace.request(i, tx) for i, tx in ids])
-                    outs = (C.c_void_p * len(ids))()
-                for j, (i, tx) in enumerate(ids):
-                    if 2 * i + tx not in out_bufs:
-                        out_bufs[2 * i + tx] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
-                    outs[j] = out_bufs[2 * i + tx].ctypes.data
-                    tx_slot[2 * i + tx] = k
+            new = prepared.pop(k, None)
+            again = retx.pop(k, [])
+            if new is not None or again:
+                n = 0
+                if new is not None:
+                    reqs, outs = new
+                    n += len(reqs)
+                if again:
+                    rreqs = (TbRequest * len(again))(*[q for _, q in again])
+                    routs = (C.c_void_p * len(again))(*[out_bufs[2 * i + 1].ctypes.data for i, _ in again])
+                    for i, _ in again:
+                        tx_slot[2 * i + 1] = k
+                    n += len(again)
                 slot_start[k] = time.perf_counter()
-                srv.submit_batch(reqs, outs)
-                outstanding[k] = outstanding.get(k, 0) + len(ids)
+                if new is not None:
+                    srv.submit_batch(reqs, outs)
+                if again:
+                    srv.submit_batch(rreqs, routs)
+                outstanding[k] = outstanding.get(k, 0) + n
             k += 1
         srv.step()
         for r in srv.poll(wait=not realtime and (k >= last_slot or not due)):
-            results[r.id] = r
-            s = tx_slot[r.id]
-            outstanding[s] -= 1
-            if outstanding[s] == 0:
-                slot_done[s] = time.perf_counter()
-            i, tx = divmod(r.id, 2)
-            if tx == 0 and r.tb_crc_ok != LH_TB_CRC_OK:
-                if max(s + HARQ_RTT_SLOTS, k) < last_slot:
-                    retx.setdefault(max(s + HARQ_RTT_SLOTS, k), []).append(i)
-                else:
-                    srv.harq_release(i)  # past the run: no retransmission
-            elif tx == 1 and r.tb_crc_ok != LH_TB_CRC_OK:
-                srv.harq_release(i)
+            complete(r)
     wall = time.perf_counter() - t0
-    srv.close()
-    slot_lat = [(slot_done[s] - slot_start[s]) * 1e6 for s in slot_done if s in slot_start]
-    return results, slot_lat, wall, out_bufs
+    return results, slot_done, slot_start, wall
 
 
 def summarize(trace, results, slot_lat, wall, out_bufs):

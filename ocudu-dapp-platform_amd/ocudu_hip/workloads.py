@@ -25,11 +25,12 @@ def ue_configs(n_ue=64, seed=0, rate_lo=0.2, rate_hi=0.93):
     return out
 
 
-def snr_for_bler(ctx, bg, Z, E, qm, target=0.10, n=128, seed=3, lo=-6.0, hi=36.0, steps=10):
+def snr_for_bler(ctx, bg, Z, E, qm, target=0.10, n=128, seed=3, lo=-6.0, hi=36.0, steps=10, iters=None):
     """Es/N0 (dB) where the plain decoder's BLER crosses `target` (bisection)."""
+    kw = {} if iters is None else {"iters": iters}
     for _ in range(steps):
         m = 0.5 * (lo + hi)
-        e, _ = bler(ctx, bg, Z, E, qm, m, n, seed)
+        e, _ = bler(ctx, bg, Z, E, qm, m, n, seed, **kw)
         if e > target:
             lo = m
         else:

@@ -28,7 +28,8 @@ def setup():
 
 def _alone(ctx, trace, i, learned, mix):
     from ocudu_hip import serve
-    srv = H.Server(ctx, 1 << 28, 1024, 1, 8, learned=learned, nof_fallback_iterations=8)
+    it = serve.PUSCH_LDPC_ITERATIONS  # as in the mix
+    srv = H.Server(ctx, 1 << 28, 1024, 1, it, learned=learned, nof_fallback_iterations=it)
     out = {}
     for tx in (0, 1):
         if 2 * i + tx not in mix:
