@@ -1857,7 +1857,7 @@ int lh_server_step(lh_server* sv)
       nd.push_back(q.new_data ? 1 : 0);
     }
     bits += (uint64_t)C * mb;
-    tbo += q.tbs_bits / 8;
+    tbo += ((q.tbs_bits / 8) + 3) & ~3u; // word-aligned TB offsets (the finalize writes words)
   }
   launch_plan plan;
   int rc = pusch_jobs(cbs.data(), (unsigned)cbs.size(), 1, 1, sv->cfg.nof_ldpc_iterations, plan, nd.data());

@@ -153,7 +153,8 @@ static_assert(sizeof(lh_tb_desc_dev) == 40, "lh_tb_desc_dev layout");
 // Completion signal of a TB finalize launch whose outputs live in host memory: every
 // workgroup also copies its transport block's codeblock results to res_out, and the last
 // workgroup to finish stores seq into *flag (system scope, after all the launch's
-// writes), so the host sees the whole launch complete by reading one word.
+// writes), so the host sees the whole launch complete by reading one word.  The TB
+// offsets of such a launch are multiples of 4 (the bytes go out as words).
 struct tb_signal {
   int32_t* res_out;  // host-visible codeblock results, indexed like the decoder's
   uint32_t* counter; // device word, zero between launches (the last workgroup rewinds it)

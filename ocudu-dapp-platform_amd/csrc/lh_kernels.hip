@@ -1236,6 +1236,9 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 
 // GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
 // TB reassembly + TB CRC24A of transport block d, one workgroup (pusch_decoder_impl.cpp:612-781).
+// WIDE: the TB bytes go out as aligned 4-byte words (the TB offset is a multiple of 4),
+// which matters when they are written across the bus into host memory.
+template <bool WIDE>
 __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const uint8_t* __restrict__ cb_bits,
                                                 const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
                                                 int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
@@ -1260,8 +1263,22 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
   if (d.nof_cbs == 1) {
     // One codeblock: its CRC is the TB CRC; copy the data on success.
     if (all_ok) {
-      for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
-        out[b] = first[b];
+      if (WIDE) {
+        uint32_t* out4 = reinterpret_cast<uint32_t*>(out);
+        for (uint32_t w = (uint32_t)t; 4 * w < nbytes; w += (uint32_t)nt) {
+          if (4 * w + 4 <= nbytes) {
+            out4[w] = (uint32_t)first[4 * w] | ((uint32_t)first[4 * w + 1] << 8) | ((uint32_t)first[4 * w + 2] << 16) |
+                      ((uint32_t)first[4 * w + 3] << 24);
+          } else {
+            for (uint32_t b = 4 * w; b < nbytes; ++b) {
+              out[b] = first[b];
+            }
+          }
+        }
+      } else {
+        for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
+          out[b] = first[b];
+        }
       }
     }
     if (t == 0) {
@@ -1281,9 +1298,12 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
   // chunk) and XOR-reduced.
   const uint32_t D = d.cb_data_bits;
   const uint32_t poly = 0x1864cfbu;
-  const uint32_t chunk = (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
+  // Chunks of whole words when WIDE: every chunk but the last ends on a word boundary.
+  const uint32_t chunk = WIDE ? ((nbytes + 4u * (uint32_t)nt - 1) / (4u * (uint32_t)nt)) * 4u
+                              : (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
   const uint32_t a0 = min(nbytes, (uint32_t)t * chunk), a1 = min(nbytes, a0 + chunk);
   uint32_t rem = 0;
+  uint32_t word = 0;
   // Codeblock c and bit j of TB bit 8 * b (one division per thread, then stepped).
   uint32_t c = (8u * a0) / D, j = 8u * a0 - c * D;
   for (uint32_t b = a0; b < a1; ++b) {
@@ -1306,12 +1326,26 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
         }
       }
     }
-    out[b] = (uint8_t)v;
+    if (WIDE) {
+      word |= v << (8 * (b & 3u));
+      if ((b & 3u) == 3u) {
+        *reinterpret_cast<uint32_t*>(out + b - 3) = word;
+        word = 0;
+      }
+    } else {
+      out[b] = (uint8_t)v;
+    }
     rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ v) & 0xffu];
     j += 8u;
     if (j >= D) {
       j -= D;
       ++c;
+    }
+  }
+  if (WIDE && (a1 & 3u) != 0 && a0 < a1) {
+    // The TB's last, partial word.
+    for (uint32_t b = a1 & ~3u; b < a1; ++b) {
+      out[b] = (uint8_t)(word >> (8 * (b & 3u)));
     }
   }
   uint32_t x = 0, z = 0;
@@ -1346,7 +1380,7 @@ __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __res
   __shared__ uint32_t lut[256];
   __shared__ int all_ok;
   const lh_tb_desc_dev d = tbs[blockIdx.x];
-  tb_finalize_one(d, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
+  tb_finalize_one<SIGNAL>(d, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
   if constexpr (SIGNAL) {
     for (uint32_t c = threadIdx.x; c < d.nof_cbs; c += blockDim.x) {
       sig.res_out[d.first_cb + c] = cb_res[d.first_cb + c];
