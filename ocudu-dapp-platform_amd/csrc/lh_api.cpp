@@ -1926,7 +1926,14 @@ int lh_server_poll(lh_server* sv, lh_tb_result* out, unsigned max, unsigned* n_o
     for (unsigned k = 0; k < sv->slots.size(); ++k) {
       srv_slot& sl = sv->slots[(sv->next_slot + k) % sv->slots.size()];
       if (sl.busy) {
-        (void)hipEventSynchronize(sl.done);
+        const hipError_t e = hipEventSynchronize(sl.done);
+        if (e != hipSuccess) {
+          return hip_fail(e, "server slot wait");
+        }
+        // The slot's work has completed, so its completion word must be there.
+        if (__atomic_load_n(sl.h_flag, __ATOMIC_ACQUIRE) != sl.seq) {
+          return fail(LH_ERR_HIP, "server slot completed without its completion word");
+        }
         break;
       }
     }
