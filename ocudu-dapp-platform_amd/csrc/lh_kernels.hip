@@ -106,6 +106,15 @@ __device__ __forceinline__ int v2c_of(int sv, int c)
   return med3i(sv - c, -120, 120) + 241 * ex;
 }
 
+// Soft bits are addressed by their LDS byte address (one add per edge, no generic-pointer
+// arithmetic).
+typedef __attribute__((address_space(3))) int8_t lds_i8;
+__device__ __forceinline__ lds_i8* lds_at(uint32_t a) { return reinterpret_cast<lds_i8*>((uintptr_t)a); }
+__device__ __forceinline__ uint32_t lds_addr(const int8_t* p)
+{
+  return (uint32_t)(uintptr_t)(const lds_i8*)p;
+}
+
 // One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
 // two-min over composite keys; pass 2: c2v, promoted soft-bit update).  DEG is the
 // compile-time degree (core rows) or the maximum degree with a runtime `deg`.
@@ -120,7 +129,8 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
                                              uint32_t ab)
 {
   constexpr int NW = (DEG + 3) / 4;
-  int addr[DEG];
+  const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
+  uint32_t addr[DEG];
   int sv[DEG];
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
@@ -128,8 +138,8 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
       const uint32_t g = gt[e];
       unsigned j = r + (g >> 16);
       j = min(j, j - Z);
-      addr[e] = (int)((g & 0xffffu) + j) + goff; // one three-input add
-      sv[e] = soft[addr[e]];
+      addr[e] = (g & 0xffffu) + j + gbase; // one three-input add
+      sv[e] = *lds_at(addr[e]);
     }
   }
   int s1o = 0, s2o = 0, amo = 0;
@@ -193,7 +203,7 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
       } else {
         ns |= (uint32_t)(neg & 1) << e;
       }
-      soft[addr[e]] = (int8_t)med3i(c + v[e], -121, 121); // promotion sum
+      *lds_at(addr[e]) = (int8_t)med3i(c + v[e], -121, 121); // promotion sum
     }
   }
   if (PACKED) {
@@ -235,12 +245,6 @@ __device__ __forceinline__ int soft_pos(int q, int j, int Z)
 typedef short i16x2 __attribute__((ext_vector_type(2)));
 // LDS bytes addressed by their 32-bit LDS address: the row-pair partner is then address
 // ^ 1 with no base to add again.
-typedef __attribute__((address_space(3))) int8_t lds_i8;
-__device__ __forceinline__ lds_i8* lds_at(uint32_t a) { return reinterpret_cast<lds_i8*>((uintptr_t)a); }
-__device__ __forceinline__ uint32_t lds_addr(const int8_t* p)
-{
-  return (uint32_t)(uintptr_t)(const lds_i8*)p;
-}
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ i16x2 as_i16x2(uint32_t w) { return __builtin_bit_cast(i16x2, w); }
