@@ -167,15 +167,35 @@ struct lh_ctx {
 
 namespace {
 
-// Returns a staging slot whose previous upload has completed.
+// Returns a staging slot whose previous upload has completed.  If the slot's event can no
+// longer be waited on (its stream was destroyed since), the device is drained instead and
+// the runtime's error state cleared, so the failure does not surface in a later launch.
 slot* acquire_slot(lh_ctx* ctx)
 {
   slot* s = &ctx->ring[ctx->next];
   ctx->next = (ctx->next + 1) % kRing;
-  if (s->used) {
-    (void)hipEventSynchronize(s->done);
+  if (s->used && hipEventSynchronize(s->done) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipDeviceSynchronize();
+    (void)hipGetLastError();
   }
   return s;
+}
+
+// Before streams that staging-slot events may have been recorded on are destroyed: wait
+// for every slot and re-record its events on the context's own stream, so that no event
+// refers to a destroyed stream.
+void rebind_ring(lh_ctx* ctx)
+{
+  for (auto& s : ctx->ring) {
+    if (s.used) {
+      (void)hipEventSynchronize(s.done);
+      (void)hipEventSynchronize(s.copied);
+      (void)hipEventRecord(s.done, ctx->copy);
+      (void)hipEventRecord(s.copied, ctx->copy);
+    }
+  }
+  (void)hipGetLastError();
 }
 
 // Marks the slot busy until the work enqueued on `stream` so far (the kernels that read
@@ -1712,6 +1732,9 @@ int lh_server_destroy(lh_server* sv)
   (void)hipSetDevice(sv->ctx->device);
   for (auto& sl : sv->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  }
+  rebind_ring(sv->ctx); // the staging slots' events may name the server's streams
+  for (auto& sl : sv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.d_bits) (void)hipFree(sl.d_bits);

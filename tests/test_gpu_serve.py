@@ -85,3 +85,22 @@ def test_admission_waits_for_memory(setup):
     mix, slot_lat, wall, bufs = serve.serve(ctx, trace, realtime=False, harq_budget=need + 4096, timeout_s=120)
     assert all(2 * i in mix for i in range(len(trace.tbs)))
     assert all(r.tb_crc_ok != H.LH_TB_REJECTED for r in mix.values())
+
+
+def test_servers_recreated(setup):
+    """Servers created and destroyed one after another on one context (plain, learned,
+    plain, the bench's order): every run completes without a runtime error, and the two
+    plain runs give the same result for every transmission."""
+    print("case servers recreated", flush=True)
+    from ocudu_hip import nms, serve
+    ctx, trace = setup
+    for bg in (1, 2):
+        _, aq, bq, _ = nms.load_table(TABLES / f"nms_bg{bg}.json")
+        ctx.set_scale_table(bg, aq, bq)
+    runs = [serve.serve(ctx, trace, learned=learned, realtime=False, timeout_s=120)[0]
+            for learned in (False, True, False)]
+    a, b = runs[0], runs[2]
+    assert set(a) == set(b)
+    for k in a:
+        assert (a[k].tb_crc_ok, a[k].nof_cbs_ok, a[k].max_iterations) == \
+            (b[k].tb_crc_ok, b[k].nof_cbs_ok, b[k].max_iterations), k
