@@ -268,6 +268,30 @@ typedef struct lh_gen_desc {
 int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t* d_llrs, uint8_t* d_msgs,
                       void* stream);
 
+/* One codeblock of the LDPC encoder + rate matcher (transmit side). */
+typedef struct lh_encode_desc {
+  uint64_t msg_offset;  /* its message in d_msgs: K_b*Z bits packed MSB first (data, CRC, zero fillers) */
+  uint64_t out_offset;  /* its rate-matched bits in d_out: rm_length bits packed MSB first (the last byte's
+                           bits past rm_length are 0) */
+  uint64_t cw_offset;   /* its codeword in d_codewords: N_short*Z bits, one per byte */
+  uint32_t rm_length;   /* E, multiple of modulation_order */
+  uint32_t base_graph;
+  uint32_t lifting_size;
+  uint32_t rv;
+  uint32_t modulation_order; /* 1, 2, 4, 6, 8 */
+  uint32_t Nref;        /* 0: unlimited buffer */
+  uint32_t nof_filler_bits;
+  uint32_t pad;
+} lh_encode_desc;
+
+/* LDPC encoding and rate matching of a batch of codeblocks, one workgroup each.  Replaces
+ * ldpc_encoder::encode (include/ocudu/phy/upper/channel_coding/ldpc/ldpc_encoder.h:41) with
+ * ldpc_encoder_buffer::write_codeblock (ldpc_encoder_buffer.h:31) for the codeword (when
+ * d_codewords is not NULL), followed by ldpc_rate_matcher::rate_match (ldpc_rate_matcher.h:29)
+ * for the rate-matched, bit-interleaved output (TS38.212 5.3.2, 5.4.2). */
+int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned n, const uint8_t* d_msgs,
+                              uint8_t* d_out, uint8_t* d_codewords, void* stream);
+
 /* One transport block of lh_gen_transport_blocks. */
 typedef struct lh_gen_tb_desc {
   uint64_t tb_offset;      /* TB bytes (tbs_bits / 8) written to d_tb */

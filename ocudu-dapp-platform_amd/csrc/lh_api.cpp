@@ -36,6 +36,8 @@ hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned max
                                hipStream_t s);
 hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
                                  const uint32_t* gtab, hipStream_t s);
+hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
+                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
 size_t nms_train_lds_bytes(int bg, int Z, int L);
 hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt, size_t lds, int T,
                             const float* alpha, const float* beta, const float* wt, float tau, const int8_t* in,
@@ -1250,6 +1252,59 @@ int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t*
                                        ctx->d_gtab, st);
   retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "generator launch");
+}
+
+int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned n, const uint8_t* d_msgs,
+                              uint8_t* d_out, uint8_t* d_codewords, void* stream)
+{
+  if (ctx == nullptr || (n > 0 && (descs == nullptr || d_msgs == nullptr || d_out == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_ldpc_encode_rate_match");
+  }
+  if (n > ctx->max_cb) {
+    return fail(LH_ERR_CAPACITY, "batch of %u codeblocks exceeds the context capacity %u", n, ctx->max_cb);
+  }
+  if (n == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  gen_job* jobs = reinterpret_cast<gen_job*>(s->host);
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_encode_desc& d = descs[i];
+    lh_rm_desc rd{};
+    rd.cw_offset = d.out_offset;
+    rd.rm_length = d.rm_length;
+    rd.base_graph = d.base_graph;
+    rd.lifting_size = d.lifting_size;
+    rd.rv = d.rv;
+    rd.modulation_order = d.modulation_order;
+    rd.Nref = d.Nref;
+    rd.nof_filler_bits = d.nof_filler_bits;
+    gen_job& j = jobs[i];
+    j = gen_job{};
+    uint64_t mp = 0;
+    int rc = make_rm_params(rd, true, j.rm, mp, i);
+    if (rc != LH_OK) {
+      return rc;
+    }
+    j.msg_off = d.msg_offset;
+    j.gt_off = graph_offset(d.base_graph, d.lifting_size);
+    j.Z = (uint16_t)d.lifting_size;
+    j.bg = (uint8_t)d.base_graph;
+    j.F = (uint16_t)d.nof_filler_bits;
+    j.msg_input = 1;
+    j.want_cw = d_codewords != nullptr ? 1 : 0;
+    j.cwb_off = d.cw_offset;
+  }
+  int rc = upload(ctx, s, n * sizeof(gen_job), st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = launch_encode_rate_match(reinterpret_cast<const gen_job*>(s->dev), n, const_cast<uint8_t*>(d_msgs),
+                                          d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, st);
+  retire(s, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "encoder launch");
 }
 
 } // extern "C"

@@ -100,10 +100,15 @@ constexpr int kGenMaxBytes = 68 * kMaxZ; // (K_b + M) * Z of BG1 at Z = 384
 
 } // namespace
 
-template <int BG>
+// ENCODE: the transmit side only (ldpc_encoder + ldpc_rate_matcher): the message comes
+// from msgs, and the rate-matched bits are written packed (MSB first) to out_bits +
+// rm.cw_off, with the codeword (N_short * Z bits, one per byte, as
+// ldpc_encoder_buffer::write_codeblock) at cw_bits + cwb_off when asked.
+template <int BG, bool ENCODE>
 __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint8_t* __restrict__ msgs,
                               const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab, uint8_t* cw,
-                              uint8_t* lam, uint32_t* red)
+                              uint8_t* lam, uint32_t* red, uint8_t* __restrict__ out_bits = nullptr,
+                              uint8_t* __restrict__ cw_bits = nullptr)
 {
   constexpr int K = BG == 1 ? 22 : 10, M = BG == 1 ? 46 : 42;
   const uint16_t* rs = BG == 1 ? lh_bg1_row_start : lh_bg2_row_start;
@@ -244,6 +249,37 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   }
   __syncthreads();
 
+  if constexpr (ENCODE) {
+    // Codeword and rate matching (TS38.212 5.4.2, as the generator's step 5 below): output
+    // bit j = s Qm + q is rate-matched bit e_i, i = q Kq + s, read from the circular
+    // buffer at rank start + i (mod P), fillers skipped.
+    const rm_params& rp = jb.rm;
+    const uint8_t* d = cw + 2 * Z;
+    if (jb.want_cw) {
+      uint8_t* co = cw_bits + jb.cwb_off;
+      for (int p = t; p < (K + M - 2) * Z; p += NT) {
+        co[p] = d[p];
+      }
+    }
+    uint8_t* ro = out_bits + rp.cw_off;
+    const uint32_t qm = rp.Qm;
+    for (uint32_t b = (uint32_t)t; 8 * b < rp.E; b += (uint32_t)NT) {
+      uint32_t v = 0;
+      for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t j = 8 * b + k;
+        if (j < rp.E) {
+          const uint32_t s = j / qm, q = j - s * qm;
+          const uint32_t i = q * rp.Kq + s;
+          const uint32_t rank = (uint32_t)(((uint64_t)rp.start + i) % rp.P);
+          const uint32_t p = rank < rp.info ? rank : rank + (rp.nsys - rp.info);
+          v |= (uint32_t)d[p] << (7 - k);
+        }
+      }
+      ro[b] = (uint8_t)v;
+    }
+    return;
+  }
+
   // 4. Packed message bits (MSB first).
   if (jb.write_msg && !jb.msg_input) {
     uint8_t* out = msgs + jb.msg_off;
@@ -311,9 +347,26 @@ __global__ void __launch_bounds__(384) k_gen_codeblocks(const gen_job* __restric
   __shared__ uint32_t red[16];
   const gen_job jb = jobs[blockIdx.x];
   if (jb.bg == 1) {
-    gen_codeblock<1>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
+    gen_codeblock<1, false>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
   } else {
-    gen_codeblock<2>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
+    gen_codeblock<2, false>(jb, llrs, msgs, crc_x, gtab, cw, lam, red);
+  }
+}
+
+// LDPC encoder + rate matcher, one workgroup per codeblock (message from msgs).
+__global__ void __launch_bounds__(384) k_encode_rate_match(const gen_job* __restrict__ jobs, uint8_t* __restrict__ msgs,
+                                                           uint8_t* __restrict__ out_bits, uint8_t* __restrict__ cw_bits,
+                                                           const uint32_t* __restrict__ crc_x,
+                                                           const uint32_t* __restrict__ gtab)
+{
+  __shared__ uint8_t cw[kGenMaxBytes];
+  __shared__ uint8_t lam[4 * kMaxZ];
+  __shared__ uint32_t red[16];
+  const gen_job jb = jobs[blockIdx.x];
+  if (jb.bg == 1) {
+    gen_codeblock<1, true>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits);
+  } else {
+    gen_codeblock<2, true>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits);
   }
 }
 
@@ -416,6 +469,14 @@ hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs
                                  const uint32_t* gtab, hipStream_t s)
 {
   hipLaunchKernelGGL(k_gen_codeblocks, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, llrs, msgs, crc_x, gtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
+                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_encode_rate_match, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits, crc_x,
+                     gtab);
   return hipGetLastError();
 }
 

@@ -103,8 +103,12 @@ struct alignas(16) gen_job {
   uint16_t F;
   uint8_t write_msg;
   uint8_t msg_input; // the message is read from msgs + msg_off (data, CRC and fillers)
-  uint32_t pad1[5];
-  rm_params rm;
+  uint8_t want_cw;   // encoder: also the codeword (one bit per byte) at cwb_off
+  uint8_t pad0;
+  uint16_t pad1;
+  uint64_t cwb_off;
+  uint32_t pad2[2];
+  rm_params rm;      // encoder: rm.cw_off is the packed rate-matched output
 };
 static_assert(sizeof(gen_job) == 128, "gen_job layout");
 

@@ -179,6 +179,13 @@ class GenDesc(C.Structure):
                 ("flags", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class EncodeDesc(C.Structure):
+    _fields_ = [("msg_offset", C.c_uint64), ("out_offset", C.c_uint64), ("cw_offset", C.c_uint64),
+                ("rm_length", C.c_uint32), ("base_graph", C.c_uint32), ("lifting_size", C.c_uint32),
+                ("rv", C.c_uint32), ("modulation_order", C.c_uint32), ("Nref", C.c_uint32),
+                ("nof_filler_bits", C.c_uint32), ("pad", C.c_uint32)]
+
+
 LH_GEN_MSG_INPUT = 1
 
 
@@ -226,6 +233,7 @@ class TrainDesc(C.Structure):
 
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
+            "lh_ldpc_encode_rate_match",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
@@ -283,6 +291,8 @@ def lib():
         L.lh_server_poll.argtypes = [C.c_void_p, C.POINTER(TbResult), C.c_uint, C.POINTER(C.c_uint), C.c_int]
         L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.lh_crc_calculate.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
@@ -353,6 +363,13 @@ class Context:
         arr = (TbDesc * len(tbs))(*tbs)
         _lh(lib().lh_tb_finalize(self.handle, arr, len(tbs), d_cb_bits.ptr, d_cb_results.ptr, d_tb.ptr, d_status.ptr,
                                  stream.handle if stream else None), "lh_tb_finalize")
+
+    def encode_rate_match(self, descs, d_msgs, d_out, d_codewords=None, stream=None):
+        """LDPC encoding + rate matching (lh_ldpc_encode_rate_match)."""
+        arr = (EncodeDesc * len(descs))(*descs)
+        _lh(lib().lh_ldpc_encode_rate_match(self.handle, arr, len(descs), d_msgs.ptr, d_out.ptr,
+                                            d_codewords.ptr if d_codewords else None,
+                                            stream.handle if stream else None), "lh_ldpc_encode_rate_match")
 
     def gen_codeblocks(self, descs, d_llrs, d_msgs=None, stream=None):
         arr = (GenDesc * len(descs))(*descs)
