@@ -263,16 +263,26 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
     }
     uint8_t* ro = out_bits + rp.cw_off;
     const uint32_t qm = rp.Qm;
+    // Rank of e_{q Kq} for each q (reduced mod P once), then bit by bit: rank = base[q] + s,
+    // wrapped by subtraction (E / P + 1 times at most).
+    uint32_t base[8];
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) {
+      base[q] = q < qm ? (uint32_t)(((uint64_t)rp.start + (uint64_t)q * rp.Kq) % rp.P) : 0u;
+    }
     for (uint32_t b = (uint32_t)t; 8 * b < rp.E; b += (uint32_t)NT) {
       uint32_t v = 0;
-      for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t j = 8 * b + k;
-        if (j < rp.E) {
-          const uint32_t s = j / qm, q = j - s * qm;
-          const uint32_t i = q * rp.Kq + s;
-          const uint32_t rank = (uint32_t)(((uint64_t)rp.start + i) % rp.P);
-          const uint32_t p = rank < rp.info ? rank : rank + (rp.nsys - rp.info);
-          v |= (uint32_t)d[p] << (7 - k);
+      uint32_t s = (8 * b) / qm, q = 8 * b - s * qm;
+      for (uint32_t k = 0; k < 8 && 8 * b + k < rp.E; ++k) {
+        uint32_t rank = base[q] + s;
+        while (rank >= rp.P) {
+          rank -= rp.P;
+        }
+        const uint32_t p = rank < rp.info ? rank : rank + (rp.nsys - rp.info);
+        v |= (uint32_t)d[p] << (7 - k);
+        if (++q == qm) {
+          q = 0;
+          ++s;
         }
       }
       ro[b] = (uint8_t)v;
