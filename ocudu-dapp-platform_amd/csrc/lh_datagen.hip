@@ -121,9 +121,26 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   // 1. Message: data bits, CRC, zero fillers; the parity part starts zeroed.  With
   //    msg_input the whole message (data, CRC, fillers) comes from msgs + msg_off.
   if (jb.msg_input) {
+    // One 4-byte load per thread step (the message is read once, not once per bit) where
+    // the message is 4-byte aligned and the word lies inside it; bytes otherwise.
     const uint8_t* src = msgs + jb.msg_off;
-    for (int j = t; j < (K + M) * Z; j += NT) {
-      cw[j] = j < K * Z ? (uint8_t)((src[j >> 3] >> (7 - (j & 7))) & 1u) : (uint8_t)0;
+    const int kz = K * Z, nbytes = (kz + 7) / 8;
+    const bool al4 = (jb.msg_off & 3u) == 0;
+    for (int w = t; 4 * w < nbytes; w += NT) {
+      uint32_t x = 0;
+      if (al4 && 4 * w + 4 <= nbytes) {
+        x = reinterpret_cast<const uint32_t*>(src)[w];
+      } else {
+        for (int k = 0; k < 4 && 4 * w + k < nbytes; ++k) {
+          x |= (uint32_t)src[4 * w + k] << (8 * k);
+        }
+      }
+      for (int k = 0; k < 32 && 32 * w + k < kz; ++k) {
+        cw[32 * w + k] = (uint8_t)((x >> (8 * (k >> 3) + 7 - (k & 7))) & 1u);
+      }
+    }
+    for (int j = kz + t; j < (K + M) * Z; j += NT) {
+      cw[j] = 0;
     }
     __syncthreads();
   }
