@@ -342,11 +342,16 @@ int lh_segment_rx(unsigned base_graph, unsigned tbs_bits, unsigned modulation_or
  * launches (decode + TB reassembly) on up to max_slots_in_flight streams whose work
  * overlaps on the device, and completed asynchronously.  A TB whose CRC passes releases its
  * HARQ buffers; a failed one keeps them for a retransmission (lh_server_harq_release drops
- * them).  Single-threaded: one caller drives submit / step / poll. */
+ * them).  As the reference's rx_buffer, a HARQ process keeps each codeblock's message bits
+ * and CRC state: a codeblock whose CRC passed in an earlier transmission only combines the
+ * new LLRs and is not decoded again (pusch_decoder_impl.cpp:412-431), and all are decoded
+ * again after a TB CRC failure with every codeblock passed (:657-669).  Single-threaded:
+ * one caller drives submit / step / poll. */
 typedef struct lh_server lh_server;
 
 typedef struct lh_server_config {
-  size_t harq_budget_bytes;        /* device memory for HARQ soft buffers (N_short*Z per codeblock) */
+  size_t harq_budget_bytes;        /* device memory for HARQ processes (per codeblock N_short*Z soft bits + its
+                                      message bits, K_b*Z/8 bytes) */
   unsigned max_cbs_per_slot;       /* codeblocks per slot launch (<= the context capacity) */
   unsigned max_slots_in_flight;    /* 1..8 */
   unsigned nof_ldpc_iterations;

@@ -833,8 +833,11 @@ namespace {
 
 // Decoder jobs of a PUSCH batch (fused dematching + decoding, HARQ write-back).
 // new_data_per_cb (optional) overrides new_data per codeblock.
+// passed_iters (optional): per codeblock, 0 or the iteration count of an earlier
+// transmission whose CRC passed (combine only, report that count).
 int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data_all, int use_early_stop,
-               unsigned nof_ldpc_iterations, launch_plan& plan, const uint8_t* new_data_per_cb = nullptr)
+               unsigned nof_ldpc_iterations, launch_plan& plan, const uint8_t* new_data_per_cb = nullptr,
+               const uint8_t* passed_iters = nullptr)
 {
   if (nof_ldpc_iterations == 0 || nof_ldpc_iterations > 255) {
     return fail(LH_ERR_INVALID, "invalid number of LDPC iterations %u", nof_ldpc_iterations);
@@ -870,6 +873,10 @@ int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data_all, int us
     j.lbound = (uint16_t)layer_bound(b, Z, mp);
     j.gt_off = graph_offset(d.rm.base_graph, Z);
     j.out_idx = i;
+    if (passed_iters != nullptr && passed_iters[i] > 0 && !new_data) {
+      j.flags |= kFlagPassed;
+      j.preset = passed_iters[i];
+    }
     plan.jobs.push_back(j);
   }
   return LH_OK;
@@ -1522,11 +1529,27 @@ double now_us()
   return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A HARQ process on the device: the codeblocks' soft bits (N_short*Z each), then their
+// decoded message bits (one message per codeblock), and per codeblock the iteration count
+// of the transmission whose CRC passed, or 0 (rx_buffer's codeblock CRC flags and data
+// bits, rx_buffer.h:29-63).
 struct harq_entry {
   uint64_t off = 0, bytes = 0;
+  uint64_t msg_off = 0;
   uint32_t nof_cbs = 0, Z = 0, bg = 0, tbs = 0;
+  std::vector<uint8_t> cb_iters;
   bool busy = false;
 };
+
+uint64_t harq_bytes(uint32_t bg, uint32_t Z, uint32_t nof_cbs, uint64_t* msg_at = nullptr)
+{
+  const uint64_t soft = (uint64_t)nof_cbs * (bg == 1 ? 66u : 50u) * Z;
+  const uint64_t mb = ((bg == 1 ? 22u : 10u) * Z + 7) / 8;
+  if (msg_at != nullptr) {
+    *msg_at = soft;
+  }
+  return soft + (uint64_t)nof_cbs * mb;
+}
 
 struct srv_request {
   lh_tb_request req{};
@@ -1543,7 +1566,6 @@ struct srv_request {
 struct srv_slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  uint8_t* d_bits = nullptr;
   int32_t* d_res = nullptr;
   uint32_t* d_counter = nullptr;
   uint8_t* h_out = nullptr;
@@ -1660,7 +1682,8 @@ void admit(lh_server* sv)
   for (auto it = sv->pending.begin(); it != sv->pending.end();) {
     srv_request& r = *it;
     auto h = sv->harq.find(r.req.harq_process);
-    const uint64_t bytes = (uint64_t)r.seg.nof_cbs * (r.req.base_graph == 1 ? 66u : 50u) * r.seg.lifting_size;
+    uint64_t msg_at = 0;
+    const uint64_t bytes = harq_bytes(r.req.base_graph, r.seg.lifting_size, r.seg.nof_cbs, &msg_at);
     if (h != sv->harq.end() && h->second.busy) {
       ++it;
       continue;
@@ -1685,6 +1708,8 @@ void admit(lh_server* sv)
       harq_entry e;
       e.off = off;
       e.bytes = bytes;
+      e.msg_off = off + msg_at;
+      e.cb_iters.assign(r.seg.nof_cbs, 0);
       e.nof_cbs = r.seg.nof_cbs;
       e.Z = r.seg.lifting_size;
       e.bg = r.req.base_graph;
@@ -1723,6 +1748,17 @@ void retire_slot(lh_server* sv, srv_slot& sl)
       if (st == LH_TB_CRC_OK) {
         sv->arena.release(h->second.off, h->second.bytes);
         sv->harq.erase(h);
+      } else if (st == LH_TB_CRC_FAILED) {
+        // Every codeblock passed but the TB CRC did not: all are decoded again next time
+        // (pusch_decoder_impl.cpp:657-669, reset_codeblocks_crc).
+        std::fill(h->second.cb_iters.begin(), h->second.cb_iters.end(), (uint8_t)0);
+      } else {
+        for (uint32_t c = 0; c < r.seg.nof_cbs && c < h->second.cb_iters.size(); ++c) {
+          const int32_t x = sl.h_res[sl.first_cb[k] + c];
+          if (x > 0) {
+            h->second.cb_iters[c] = (uint8_t)std::min(x, 255);
+          }
+        }
       }
     }
     push_result(sv, r, st, ok, its);
@@ -1754,8 +1790,6 @@ int lh_server_create(lh_ctx* ctx, const lh_server_config* cfg, lh_server** out)
   for (auto& sl : sv->slots) {
     if (e == hipSuccess) e = hipStreamCreate(&sl.stream);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_bits, ncb * kMaxMsgBytes);
-    if (e == hipSuccess) e = hipMemset(sl.d_bits, 0, ncb * kMaxMsgBytes);
     if (e == hipSuccess) e = hipMalloc(&sl.d_res, ncb * 4);
     if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 4);
     if (e == hipSuccess) e = hipMemset(sl.d_counter, 0, 4);
@@ -1792,7 +1826,6 @@ int lh_server_destroy(lh_server* sv)
   for (auto& sl : sv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    if (sl.d_bits) (void)hipFree(sl.d_bits);
     if (sl.d_res) (void)hipFree(sl.d_res);
     if (sl.d_counter) (void)hipFree(sl.d_counter);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
@@ -1896,21 +1929,27 @@ int lh_server_step(lh_server* sv)
     cw_base = std::min(cw_base, r.req.d_llrs);
   }
   std::vector<lh_pusch_cb_desc> cbs;
-  std::vector<uint8_t> nd;
+  std::vector<uint8_t> nd, passed;
   std::vector<lh_tb_desc> tbs;
   sl->first_cb.clear();
   sl->tb_off.clear();
-  uint64_t bits = 0, tbo = 0;
+  uint64_t tbo = 0;
   for (auto& r : sl->reqs) {
     const lh_tb_request& q = r.req;
     const unsigned C = r.seg.nof_cbs, Z = r.seg.lifting_size, N = (q.base_graph == 1 ? 66u : 50u) * Z;
     const unsigned mb = ((q.base_graph == 1 ? 22u : 10u) * Z + 7) / 8;
     const uint32_t poly = C > 1 ? LH_CRC24B : (q.tbs_bits <= 3824 ? LH_CRC16 : LH_CRC24A);
+    // The codeblocks' message bits live in the HARQ process (kept across transmissions).
+    harq_entry& he = sv->harq.at(q.harq_process);
+    if (q.new_data) {
+      std::fill(he.cb_iters.begin(), he.cb_iters.end(), (uint8_t)0);
+    }
+    const uint64_t msg_base = he.msg_off;
     sl->first_cb.push_back((uint32_t)cbs.size());
     sl->tb_off.push_back((uint32_t)tbo);
     lh_tb_desc t{};
     t.tb_offset = tbo;
-    t.cb_bits_offset = bits;
+    t.cb_bits_offset = msg_base;
     t.cb_bits_stride = mb;
     t.tbs_bits = q.tbs_bits;
     t.first_cb = (uint32_t)cbs.size();
@@ -1928,17 +1967,18 @@ int lh_server_step(lh_server* sv)
       d.rm.modulation_order = q.modulation_order;
       d.rm.Nref = q.Nref;
       d.rm.nof_filler_bits = r.seg.nof_filler_bits;
-      d.bits_offset = bits + (uint64_t)c * mb;
+      d.bits_offset = msg_base + (uint64_t)c * mb;
       d.nof_crc_bits = r.seg.nof_cb_crc_bits;
       d.crc_poly = poly;
       cbs.push_back(d);
       nd.push_back(q.new_data ? 1 : 0);
+      passed.push_back(c < he.cb_iters.size() ? he.cb_iters[c] : (uint8_t)0);
     }
-    bits += (uint64_t)C * mb;
     tbo += ((q.tbs_bits / 8) + 3) & ~3u; // word-aligned TB offsets (the finalize writes words)
   }
   launch_plan plan;
-  int rc = pusch_jobs(cbs.data(), (unsigned)cbs.size(), 1, 1, sv->cfg.nof_ldpc_iterations, plan, nd.data());
+  int rc = pusch_jobs(cbs.data(), (unsigned)cbs.size(), 1, 1, sv->cfg.nof_ldpc_iterations, plan, nd.data(),
+                      passed.data());
   if (rc == LH_OK) {
     rc = validate_tbs(tbs.data(), (unsigned)tbs.size());
   }
@@ -1962,11 +2002,12 @@ int lh_server_step(lh_server* sv)
   slot* s = acquire_slot(ctx);
   const size_t tb_desc_bytes = align256(tbs.size() * sizeof(lh_tb_desc));
   std::memcpy(s->host, tbs.data(), tbs.size() * sizeof(lh_tb_desc));
-  rc = run_plan(ctx, plan, sv->d_arena, cw_base, sl->d_bits, sl->d_res, sl->stream, s, tb_desc_bytes,
+  uint8_t* arena_bytes = reinterpret_cast<uint8_t*>(sv->d_arena);
+  rc = run_plan(ctx, plan, sv->d_arena, cw_base, arena_bytes, sl->d_res, sl->stream, s, tb_desc_bytes,
                 sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0);
   hipError_t e = hipSuccess;
   if (rc == LH_OK) {
-    e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), (unsigned)tbs.size(), sl->d_bits,
+    e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), (unsigned)tbs.size(), arena_bytes,
                            sl->d_res, sl->h_out_dev + tb_at, reinterpret_cast<int32_t*>(sl->h_out_dev), ctx->d_tbcrc,
                            ctx->d_tbcrc + 256, sl->stream, &sig);
     retire(s, sl->stream);

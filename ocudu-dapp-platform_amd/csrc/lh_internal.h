@@ -21,6 +21,8 @@ constexpr uint8_t kFlagNewData = 8;    // rate dematch: copy instead of combine
 constexpr uint8_t kFlagFused = 16;     // decoder dematches its own input from the codeword LLRs
 constexpr uint8_t kFlagHarqWrite = 32; // fused: write the combined soft bits back (HARQ buffer)
 constexpr uint8_t kFlagSkipPassed = 64; // fallback pass: leave codeblocks whose result is > 0
+constexpr uint8_t kFlagPassed = 128;    // CRC passed in an earlier transmission: combine the new LLRs into
+                                        // the HARQ buffer only; the result is `preset`, the bits stay
 
 // Learned normalisation tables: kLearnedIters rows of M words (A | B << 16) per base
 // graph, BG1 (46 layers) first, then BG2 (42).
@@ -60,7 +62,7 @@ struct alignas(16) dec_job {
   uint8_t crc_idx;   // 0 none, 1 CRC24A, 2 CRC24B, 3 CRC16
   uint8_t flags;
   uint16_t lbound;   // upper bound of the layer count for this job (4..M)
-  uint16_t pad0;
+  uint16_t preset;   // kFlagPassed: the result reported (the earlier iteration count)
   uint32_t out_idx;  // index of this codeblock's result
   rm_params rm;      // fused dematching parameters
 };

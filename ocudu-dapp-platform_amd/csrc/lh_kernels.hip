@@ -795,7 +795,11 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   }
   // Fallback pass: codeblocks that an earlier pass decoded are left alone.
   const bool skip = active && (jb.flags & kFlagSkipPassed) && results[jb.out_idx] > 0;
-  const bool work = active && !skip; // groups that load and decode
+  const bool work = active && !skip; // groups that load (and decode unless `passed`)
+  // CRC passed in an earlier transmission (pusch_decoder_impl.cpp:412-431): the new LLRs
+  // are combined into the HARQ buffer, the codeblock is not decoded again and its bits
+  // are left as they are.
+  const bool passed = work && (jb.flags & kFlagPassed);
 
   // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
   const int soft_stride = ((K + lmax) * Z + 15) & ~15;
@@ -932,6 +936,9 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   if (active) {
     if (skip) {
       done = true;
+    } else if (passed) {
+      done = true;
+      result = (int)jb.preset;
     } else if (input_size < msg && !(jb.flags & kFlagForce)) {
       // Without a CRC calculator the output is set to ones (ldpc_decoder_impl.cpp:75-81).
       if (jb.crc_idx == 0 || (jb.flags & kFlagCrcAfter)) {
@@ -1157,7 +1164,7 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   }
 
   if (active && !skip) {
-    if (result >= 0 && !(input_size < msg && !(jb.flags & kFlagForce))) {
+    if (result >= 0 && !passed && !(input_size < msg && !(jb.flags & kFlagForce))) {
       write_hard_bits<PAIR>(soft, out, msg, false, (int)r, NT, Z);
     }
     if (r == 0) {
