@@ -10,7 +10,9 @@ layered min-sum decoding (lh_pusch_decode_batch).  AWGN Es/N0 is swept over 20..
 8424 info bits / step time) at the lowest swept Es/N0 whose BLER is <= 10%.  The learned
 decoder (tables/nms_bg1.json, lh_pusch_decode_batch_learned) is measured on the same
 inputs there, and both again at the operating point: the lowest Es/N0 on a 0.05 dB grid
-where the plain decoder's BLER is <= 10%.
+where the plain decoder's BLER is <= 10%.  Also reported: one-slot latency, the transmit
+side (LDPC encoder + rate matcher) on the same messages, the uneven batch, and the
+slot-deadline serving loop.
 
 Prints one JSON line (see the fields below).  --gpus N with N > 1 exits: the build
 targets a single GPU.
