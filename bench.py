@@ -11,7 +11,8 @@ layered min-sum decoding (lh_pusch_decode_batch).  AWGN Es/N0 is swept over 20..
 decoder (tables/nms_bg1.json, lh_pusch_decode_batch_learned) is measured on the same
 inputs there, and both again at the operating point: the lowest Es/N0 on a 0.05 dB grid
 where the plain decoder's BLER is <= 10%.  Also reported: one-slot latency, the transmit
-side (LDPC encoder + rate matcher) on the same messages, the uneven batch, and the
+side (LDPC encoder + rate matcher) on the same messages, the step before the path (soft
+demodulation + descrambling of one full 4-layer 256-QAM slot), the uneven batch, and the
 slot-deadline serving loop.
 
 Prints one JSON line (see the fields below).  --gpus N with N > 1 exits: the build
@@ -227,6 +228,12 @@ def main():
     print(f"# encoder + rate matcher: {json.dumps(encoder)}", file=sys.stderr, flush=True)
     d_enc.free()
 
+    # Step before the path: soft demodulation + descrambling of one full uplink slot of
+    # equalized symbols (273 PRB x 12 subcarriers x 12 data symbols x 4 layers, 256-QAM, one
+    # block per OFDM symbol as the PUSCH demodulator calls it).
+    demod = demod_slot(ctx, stream, ev0, ev1)
+    print(f"# demodulation + descrambling: {json.dumps(demod)}", file=sys.stderr, flush=True)
+
     uneven = uneven_batch(ctx, a) if a.uneven_ues else None
     serving = serving_runs(ctx, a) if a.serve_slots else None
 
@@ -258,6 +265,7 @@ def main():
         "learned": learned,
         "operating_point": op,
         "encoder": encoder,
+        "demodulation": demod,
         "uneven_batch": uneven,
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -339,6 +347,37 @@ def measured_traffic():
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1].strip())
     return None
+
+
+def demod_slot(ctx, stream, ev0, ev1):
+    """lh_pusch_demodulate over one full slot: 12 blocks (one per OFDM symbol) of 13104
+    symbols (273 PRB x 12 subcarriers x 4 layers), 256-QAM, each block continuing the
+    codeword's scrambling sequence.  HBM bytes per launch: 8 (symbol) + 4 (noise variance)
+    + 8 (LLRs) per symbol."""
+    import ocudu_hip as H
+
+    rng = np.random.default_rng(11)
+    per, nblk, qm = 273 * 12 * 4, 12, 8
+    nsym = per * nblk
+    iq = (rng.standard_normal(2 * nsym) * 0.7).astype(np.float32)
+    nv = np.full(nsym, 0.05, np.float32)
+    d_iq, d_nv = H.DeviceBuffer.from_array(iq), H.DeviceBuffer.from_array(nv)
+    d_llr = H.DeviceBuffer(nsym * qm)
+    descs = [H.DemodDesc(b * per, b * per * qm, per, qm, 0x1234 * 2**15 + 1, b * per * qm)
+             for b in range(nblk)]
+    ctx.pusch_demodulate(descs, d_iq, d_nv, d_llr, stream=stream)
+    ms = float("inf")
+    for _ in range(20):
+        ev0.record(stream)
+        ctx.pusch_demodulate(descs, d_iq, d_nv, d_llr, stream=stream)
+        ev1.record(stream)
+        ev1.sync()
+        ms = min(ms, ev0.elapsed_ms(ev1))
+    for d in (d_iq, d_nv, d_llr):
+        d.free()
+    gbytes = nsym * (8 + 4 + qm) / 1e9
+    return {"symbols": nsym, "llrs": nsym * qm, "blocks": nblk, "ms": round(ms, 4),
+            "llr_gbps": round(nsym * qm / (ms * 1e-3) / 1e9, 1), "hbm_gb_s": round(gbytes / (ms * 1e-3), 1)}
 
 
 def cpu_baseline(d_rm, n_cb, N, info_bits, max_iter, cpu_seconds, snr):

@@ -292,6 +292,38 @@ typedef struct lh_encode_desc {
 int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned n, const uint8_t* d_msgs,
                               uint8_t* d_out, uint8_t* d_codewords, void* stream);
 
+/* Modulation of a demodulated block: the reference's modulation_scheme values
+ * (include/ocudu/ran/sch/modulation_scheme.h:21-35), i.e. bits per symbol, 0 for pi/2-BPSK. */
+#define LH_MOD_PI_2_BPSK 0
+#define LH_MOD_BPSK 1
+#define LH_MOD_QPSK 2
+#define LH_MOD_QAM16 4
+#define LH_MOD_QAM64 6
+#define LH_MOD_QAM256 8
+
+/* One block of lh_pusch_demodulate: a span of equalized symbols of one codeword, as one
+ * demodulation_mapper::demodulate_soft call of the PUSCH demodulator. */
+typedef struct lh_demod_desc {
+  uint64_t symbol_offset; /* first symbol in d_symbols (complex float, re then im) and its noise
+                             variance in d_noise_vars */
+  uint64_t llr_offset;    /* first LLR in d_llrs: nof_symbols * bits per symbol LLRs */
+  uint32_t nof_symbols;
+  uint32_t modulation;    /* LH_MOD_* */
+  uint32_t c_init;        /* scrambling initialisation: RNTI * 2^15 + n_ID (TS38.211 6.3.1.1) */
+  uint32_t seq_offset;    /* position of the block's first LLR in the codeword's scrambling
+                             sequence (blocks after the first continue the sequence) */
+} lh_demod_desc;
+
+/* Soft demodulation and descrambling of a batch of blocks: demodulation_mapper::demodulate_soft
+ * (include/ocudu/phy/upper/channel_modulation/demodulation_mapper.h:48) followed by the
+ * PUSCH descrambling of pusch_demodulator_impl.cpp:400-414 (pseudo_random_generator::init
+ * with c_init and generate, pseudo_random_generator.h:36,88, then revert_scrambling:
+ * an LLR is negated where the sequence bit is 1).  LLRs are bit-exact with the reference's
+ * scalar demappers for finite symbols.  Blocks up to 2^25 sequence bits
+ * (seq_offset + nof_symbols * bits per symbol). */
+int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, const float* d_symbols,
+                        const float* d_noise_vars, int8_t* d_llrs, void* stream);
+
 /* One transport block of lh_gen_transport_blocks. */
 typedef struct lh_gen_tb_desc {
   uint64_t tb_offset;      /* TB bytes (tbs_bits / 8) written to d_tb */

@@ -36,6 +36,8 @@ hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned max
                                hipStream_t s);
 hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs, uint8_t* msgs, const uint32_t* crc_x,
                                  const uint32_t* gtab, hipStream_t s);
+hipError_t launch_demod(const demod_tile* tiles, unsigned nof_tiles, const demod_job* jobs, const uint32_t* tab,
+                        const float* symbols, const float* noise, int8_t* llrs, hipStream_t stream);
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
                                     uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
 size_t nms_train_lds_bytes(int bg, int Z, int L);
@@ -156,6 +158,7 @@ struct lh_ctx {
   size_t slot_bytes = 0;
   uint32_t* d_crc_x = nullptr;
   uint32_t* d_gtab = nullptr;
+  uint32_t* d_demod = nullptr; // LFSR jump matrices + demapper tables (kDemodTableWords)
   uint32_t* d_tbcrc = nullptr; // CRC24A byte table (256) + x^(2^k) mod G (32)
   uint32_t* d_scale = nullptr; // learned normalisation tables (kScaleTableWords)
   tune_entry tune[2][kNofLiftingSizes] = {}; // per (BG, Z): 0 = default
@@ -676,6 +679,14 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   if (e == hipSuccess) {
     e = hipMemcpy(c->d_gtab, gt.data(), gt.size() * 4, hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess) {
+    std::vector<uint32_t> dt(kDemodTableWords);
+    build_demod_tables(dt.data());
+    e = hipMalloc(&c->d_demod, dt.size() * 4);
+    if (e == hipSuccess) {
+      e = hipMemcpy(c->d_demod, dt.data(), dt.size() * 4, hipMemcpyHostToDevice);
+    }
+  }
   for (int i = 0; i < kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(&c->ring[i].host, c->slot_bytes, hipHostMallocDefault);
     if (e == hipSuccess) {
@@ -727,6 +738,9 @@ int lh_ctx_destroy(lh_ctx* c)
   }
   if (c->d_crc_x) {
     (void)hipFree(c->d_crc_x);
+  }
+  if (c->d_demod) {
+    (void)hipFree(c->d_demod);
   }
   if (c->d_gtab) {
     (void)hipFree(c->d_gtab);
@@ -1312,6 +1326,64 @@ int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned
                                           d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, st);
   retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "encoder launch");
+}
+
+int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, const float* d_symbols,
+                        const float* d_noise_vars, int8_t* d_llrs, void* stream)
+{
+  if (ctx == nullptr || (n > 0 && (descs == nullptr || d_symbols == nullptr || d_noise_vars == nullptr ||
+                                   d_llrs == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_pusch_demodulate");
+  }
+  size_t nof_tiles = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_demod_desc& d = descs[i];
+    const uint32_t m = d.modulation;
+    if (m != 0 && m != 1 && m != 2 && m != 4 && m != 6 && m != 8) {
+      return fail(LH_ERR_INVALID, "block %u: modulation %u is not 0, 1, 2, 4, 6 or 8", i, m);
+    }
+    const uint64_t bits = (uint64_t)d.nof_symbols * (m == 0 ? 1 : m);
+    if ((uint64_t)d.seq_offset + bits > (1ull << 25)) {
+      return fail(LH_ERR_INVALID, "block %u: sequence positions beyond 2^25", i);
+    }
+    nof_tiles += (d.nof_symbols + 1023) / 1024;
+  }
+  const size_t bytes = (size_t)n * sizeof(demod_job) + nof_tiles * sizeof(demod_tile);
+  if (bytes > ctx->slot_bytes) {
+    return fail(LH_ERR_CAPACITY, "%u blocks of %zu tiles exceed the context's staging (%zu bytes)", n, nof_tiles,
+                ctx->slot_bytes);
+  }
+  if (nof_tiles == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  demod_job* jobs = reinterpret_cast<demod_job*>(s->host);
+  demod_tile* tiles = reinterpret_cast<demod_tile*>(s->host + (size_t)n * sizeof(demod_job));
+  size_t t = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    const lh_demod_desc& d = descs[i];
+    demod_job& j = jobs[i];
+    j.sym_off = d.symbol_offset;
+    j.llr_off = d.llr_offset;
+    j.nof_symbols = d.nof_symbols;
+    j.modulation = d.modulation;
+    demod_start_states(d.c_init, d.seq_offset, &j.x1, &j.x2);
+    for (uint32_t k = 0; k * 1024u < d.nof_symbols; ++k) {
+      tiles[t++] = demod_tile{i, k};
+    }
+  }
+  int rc = upload(ctx, s, bytes, st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  const uint8_t* dv = reinterpret_cast<const uint8_t*>(s->dev);
+  hipError_t e = launch_demod(reinterpret_cast<const demod_tile*>(dv + (size_t)n * sizeof(demod_job)),
+                              (unsigned)nof_tiles, reinterpret_cast<const demod_job*>(dv), ctx->d_demod, d_symbols,
+                              d_noise_vars, d_llrs, st);
+  retire(s, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "demodulator launch");
 }
 
 } // extern "C"

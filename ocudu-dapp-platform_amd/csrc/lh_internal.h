@@ -169,6 +169,25 @@ struct tb_signal {
   uint32_t pad;
 };
 
+// Demodulation + descrambling: one block (a span of symbols with its scrambling
+// sequence start) and one tile of 1024 of its symbols per workgroup.
+constexpr int kDemodJumps = 20;   // A^(32 * 2^j), j < 20: blocks up to 2^25 sequence bits
+constexpr int kDemodFloats = 163; // slope/intercept tables + 1/sqrt(42), 1/sqrt(170), 1/sqrt(10)
+constexpr int kDemodTableWords = 2 * kDemodJumps * 31 + kDemodFloats;
+struct demod_job {
+  uint64_t sym_off;     // first symbol (and noise variance)
+  uint64_t llr_off;     // first LLR
+  uint32_t nof_symbols;
+  uint32_t modulation;  // 0 pi/2-BPSK, 1 BPSK, 2 QPSK, 4 16-QAM, 6 64-QAM, 8 256-QAM
+  uint32_t x1, x2;      // LFSR states at the block's first sequence bit (Nc included)
+};
+struct demod_tile {
+  uint32_t job;
+  uint32_t tile;
+};
+void demod_start_states(uint32_t c_init, uint64_t pos, uint32_t* x1, uint32_t* x2);
+void build_demod_tables(uint32_t* out);
+
 inline int crc_index(uint32_t poly)
 {
   return poly == 0x1864cfbu ? 1 : poly == 0x1800063u ? 2 : poly == 0x11021u ? 3 : (poly == 0 ? 0 : -1);

@@ -186,6 +186,14 @@ class EncodeDesc(C.Structure):
                 ("nof_filler_bits", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class DemodDesc(C.Structure):
+    """One block of lh_pusch_demodulate (lh_demod_desc)."""
+    _fields_ = [("symbol_offset", C.c_uint64), ("llr_offset", C.c_uint64), ("nof_symbols", C.c_uint32),
+                ("modulation", C.c_uint32), ("c_init", C.c_uint32), ("seq_offset", C.c_uint32)]
+
+
+LH_MOD_PI_2_BPSK, LH_MOD_BPSK, LH_MOD_QPSK, LH_MOD_QAM16, LH_MOD_QAM64, LH_MOD_QAM256 = 0, 1, 2, 4, 6, 8
+
 LH_GEN_MSG_INPUT = 1
 
 
@@ -234,6 +242,7 @@ class TrainDesc(C.Structure):
 EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode", "lh_rate_dematch",
             "lh_pusch_decode_batch", "lh_crc_calculate", "lh_tb_finalize", "lh_gen_codeblocks",
             "lh_ldpc_encode_rate_match",
+            "lh_pusch_demodulate",
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
@@ -291,6 +300,8 @@ def lib():
         L.lh_server_poll.argtypes = [C.c_void_p, C.POINTER(TbResult), C.c_uint, C.POINTER(C.c_uint), C.c_int]
         L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.lh_pusch_demodulate.argtypes = [C.c_void_p, C.POINTER(DemodDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]
         L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                                 C.c_void_p, C.c_void_p]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
@@ -370,6 +381,12 @@ class Context:
         _lh(lib().lh_ldpc_encode_rate_match(self.handle, arr, len(descs), d_msgs.ptr, d_out.ptr,
                                             d_codewords.ptr if d_codewords else None,
                                             stream.handle if stream else None), "lh_ldpc_encode_rate_match")
+
+    def pusch_demodulate(self, descs, d_symbols, d_noise_vars, d_llrs, stream=None):
+        """Soft demodulation + descrambling of blocks of equalized symbols (lh_pusch_demodulate)."""
+        arr = (DemodDesc * len(descs))(*descs)
+        _lh(lib().lh_pusch_demodulate(self.handle, arr, len(descs), d_symbols.ptr, d_noise_vars.ptr, d_llrs.ptr,
+                                      stream.handle if stream else None), "lh_pusch_demodulate")
 
     def gen_codeblocks(self, descs, d_llrs, d_msgs=None, stream=None):
         arr = (GenDesc * len(descs))(*descs)

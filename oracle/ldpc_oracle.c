@@ -1068,3 +1068,164 @@ uint32_t orc_tb_concat_crc(uint8_t* tb, unsigned tbs_bits, const uint8_t* cb_bit
   }
   return orc_crc_packed(ORC_CRC24A, tb, tbs_bits) ^ checksum;
 }
+
+/* ================================================================================= */
+/* f1: PUSCH soft demodulation + descrambling (the step before the rate dematcher)   */
+/* ================================================================================= */
+
+/* Gold sequence c(n) of TS38.211 5.2.1, bit by bit: x1(0) = 1, x2 from c_init, Nc = 1600
+ * (pseudo_random_generator_impl.cpp; the reference test's own expectation,
+ * pseudo_random_generator_test.cpp:80-100, is this textbook recursion).  c[i] =
+ * c(offset + i), one bit per byte. */
+void orc_scrambling_sequence(uint8_t* c, uint32_t c_init, unsigned offset, unsigned n)
+{
+  uint32_t x1 = 1u, x2 = c_init & 0x7fffffffu; /* bit i = x(m + i) */
+  const unsigned start = 1600u + offset;
+  for (unsigned m = 0; m < start + n; ++m) {
+    if (m >= start) {
+      c[m - start] = (uint8_t)((x1 ^ x2) & 1u);
+    }
+    const uint32_t f1 = ((x1 >> 3) ^ x1) & 1u;
+    const uint32_t f2 = ((x2 >> 3) ^ (x2 >> 2) ^ (x2 >> 1) ^ x2) & 1u;
+    x1 = (x1 >> 1) | (f1 << 30);
+    x2 = (x2 >> 1) | (f2 << 30);
+  }
+}
+
+/* Gray-mapped PAM amplitude (units of 1/sqrt(N)) of m bits, b[0] the sign bit: TS38.211
+ * 5.1.4-5.1.6, e.g. 64-QAM (1-2b0)(4-(1-2b2)(2-(1-2b4))). */
+static int orc_pam(int m, const int* b)
+{
+  int a = 1;
+  for (int l = m - 1; l >= 1; --l) {
+    a = (1 << (m - l)) - (1 - 2 * b[l]) * a;
+  }
+  return (1 - 2 * b[0]) * a;
+}
+
+/* Slope and intercept of the max-log LLR of bit `level` over interval idx of width
+ * w_units / sqrt(N): nearest amplitudes a0 (bit 0) and a1 (bit 1) at the interval's
+ * centre; LLR * noise_var = 2 (a0 - a1) x / sqrt(N) + (a1^2 - a0^2) / N.  In float as
+ * k * (1/sqrt(N)) and p / q (the values of the reference's tables in
+ * demodulation_mapper_qam64.cpp:25-60 and demodulation_mapper_qam256.cpp:25-144). */
+static void orc_pam_line(int m, int level, int nint, int w_units, int idx, int N, float* slope, float* icpt)
+{
+  const double u = ((double)idx - nint / 2 + 0.5) * w_units;
+  int a_best[2] = {0, 0};
+  double d_best[2] = {1e300, 1e300};
+  for (int v = 0; v < (1 << m); ++v) {
+    int b[4];
+    for (int l = 0; l < m; ++l) {
+      b[l] = (v >> l) & 1;
+    }
+    const int a = orc_pam(m, b);
+    const double d = fabs(u - a);
+    if (d < d_best[b[level]]) {
+      d_best[b[level]] = d;
+      a_best[b[level]] = a;
+    }
+  }
+  const float rsq = 1.0F / sqrtf((float)N);
+  *slope = (float)(2 * (a_best[0] - a_best[1])) * rsq;
+  *icpt = (float)(a_best[1] * a_best[1] - a_best[0] * a_best[0]) / (float)N;
+}
+
+/* interval_function of demodulation_mapper_intervals.h:13-40 then quantize (range 20). */
+static int8_t orc_interval_llr(float x, float rcp_noise, int m, int level, int nint, int w_units, int N)
+{
+  const float width = (float)w_units * (1.0F / sqrtf((float)N));
+  int idx = (int)floorf(x / width) + nint / 2;
+  idx = idx < 0 ? 0 : (idx > nint - 1 ? nint - 1 : idx);
+  float slope, icpt;
+  orc_pam_line(m, level, nint, w_units, idx, N, &slope, &icpt);
+  float l = slope * x + icpt;
+  l *= rcp_noise;
+  return orc_llr_quantize(l, 20.0F);
+}
+
+static int8_t orc_bpsk(float re, float im, float nv)
+{
+  if (!(nv > 0)) {
+    return 0;
+  }
+  const float k = 2.0F * 1.41421356237309504880F;
+  return orc_llr_quantize(k * (re + im) / nv, 24.0F);
+}
+
+/* demodulation_mapper_impl::demodulate_soft (demodulation_mapper_impl.cpp:13-88) with the
+ * scalar symbol functions of demodulation_mapper_{qpsk,qam16,qam64,qam256}.cpp:
+ * QPSK 2 sqrt(2) x / nv (range 24); 16-QAM 4x/sqrt(10), doubled minus 0.8 sign(x) beyond
+ * 2/sqrt(10), and 0.8 - 4|x|/sqrt(10), over nv (range 20); 64/256-QAM the interval
+ * functions times 1/nv (range 20); a symbol with |z|^2 < 1e-9 gives zero LLRs (16/64/256-QAM).
+ * modulation: 0 pi/2-BPSK, 1 BPSK, 2, 4, 6, 8 bits per symbol. */
+void orc_demodulate_soft(int8_t* llrs, const float* iq, const float* nv, unsigned nsym, int modulation)
+{
+  const float r10 = 1.0F / sqrtf(10.0F);
+  for (unsigned s = 0; s < nsym; ++s) {
+    const float re = iq[2 * s], im = iq[2 * s + 1], n = nv[s];
+    if (modulation == 0) {
+      llrs[s] = (s & 1u) ? orc_bpsk(im, -re, n) : orc_bpsk(re, im, n);
+      continue;
+    }
+    if (modulation == 1) {
+      llrs[s] = orc_bpsk(re, im, n);
+      continue;
+    }
+    int8_t* o = llrs + (size_t)s * (unsigned)modulation;
+    if (modulation == 2) {
+      const float k = 2.0F * 1.41421356237309504880F;
+      o[0] = n > 0 ? orc_llr_quantize(k * re / n, 24.0F) : 0;
+      o[1] = n > 0 ? orc_llr_quantize(k * im / n, 24.0F) : 0;
+      continue;
+    }
+    if (re * re + im * im < 1e-9F) {
+      memset(o, 0, (size_t)modulation);
+      continue;
+    }
+    if (modulation == 4) {
+      const float x[2] = {re, im};
+      for (int c = 0; c < 2; ++c) {
+        if (!(n > 0)) {
+          o[c] = 0;
+          o[2 + c] = 0;
+          continue;
+        }
+        float l = 4 * r10 * x[c];
+        if (fabsf(x[c]) > 2 * r10) {
+          l = 2 * l - copysignf(0.8F, x[c]);
+        }
+        o[c] = orc_llr_quantize(l / n, 20.0F);
+        o[2 + c] = orc_llr_quantize((0.8F - 4 * r10 * fabsf(x[c])) / n, 20.0F);
+      }
+      continue;
+    }
+    const float rcp = n > 0 ? 1 / n : 0.0F;
+    const int m = modulation / 2;
+    const int N = modulation == 6 ? 42 : 170;
+    for (int level = 0; level < m; ++level) {
+      const int last = level == m - 1;
+      const int nint = last ? (1 << (m - 1)) : (1 << m);
+      const int w = last ? 4 : 2;
+      o[2 * level] = orc_interval_llr(re, rcp, m, level, nint, w, N);
+      o[2 * level + 1] = orc_interval_llr(im, rcp, m, level, nint, w, N);
+    }
+  }
+}
+
+/* The PUSCH demodulator's per-block demapping and descrambling
+ * (pusch_demodulator_impl.cpp:262-271, 400-414, revert_scrambling :22-80): demodulate,
+ * then negate the LLRs whose sequence bit c(seq_offset + i) is 1. */
+void orc_pusch_demodulate(int8_t* llrs, const float* iq, const float* nv, unsigned nsym, int modulation,
+                          uint32_t c_init, unsigned seq_offset)
+{
+  const unsigned qm = modulation == 0 ? 1u : (unsigned)modulation;
+  orc_demodulate_soft(llrs, iq, nv, nsym, modulation);
+  uint8_t* c = (uint8_t*)malloc((size_t)nsym * qm + 1);
+  orc_scrambling_sequence(c, c_init, seq_offset, nsym * qm);
+  for (unsigned i = 0; i < nsym * qm; ++i) {
+    if (c[i]) {
+      llrs[i] = (int8_t)(-llrs[i]);
+    }
+  }
+  free(c);
+}

@@ -139,6 +139,12 @@ int orc_gen_codeblock(int bg, unsigned Z, unsigned F, uint32_t crc_poly, unsigne
                       unsigned Nref, double n0, uint64_t seed_msg, uint64_t seed_noise, uint8_t* msg_packed,
                       int8_t* llrs);
 
+/* f1: PUSCH soft demodulation and descrambling. */
+void orc_scrambling_sequence(uint8_t* c, uint32_t c_init, unsigned offset, unsigned n);
+void orc_demodulate_soft(int8_t* llrs, const float* iq, const float* nv, unsigned nsym, int modulation);
+void orc_pusch_demodulate(int8_t* llrs, const float* iq, const float* nv, unsigned nsym, int modulation,
+                          uint32_t c_init, unsigned seq_offset);
+
 #ifdef __cplusplus
 }
 #endif

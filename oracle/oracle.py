@@ -51,6 +51,10 @@ def lib():
             raise RuntimeError(f"{path} is not built (make -f oracle/Makefile)")
         L = C.CDLL(str(path))
         L.orc_llr_add.restype = C.c_int8
+        _fp = C.POINTER(C.c_float)
+        L.orc_demodulate_soft.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int]
+        L.orc_scrambling_sequence.argtypes = [_u8p, C.c_uint32, C.c_uint, C.c_uint]
+        L.orc_pusch_demodulate.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int, C.c_uint32, C.c_uint]
         L.orc_llr_add.argtypes = [C.c_int8, C.c_int8]
         L.orc_llr_promotion_sum.restype = C.c_int8
         L.orc_llr_promotion_sum.argtypes = [C.c_int8, C.c_int8]
@@ -355,3 +359,65 @@ def nms_forward(llrs, msg_packed, bg, Z, L, alpha, beta, w, tau, F=0, double=Tru
     f = lib().orc_nms_forward_d if double else lib().orc_nms_forward_f
     return float(f(bg, Z, F, L, T, _p(llrs, _i8p), llrs.size, _p(msg, _u8p), a.ctypes.data_as(dp),
                    b.ctypes.data_as(dp), wv.ctypes.data_as(dp), tau))
+
+
+# --- f1: PUSCH soft demodulation and descrambling --------------------------------------
+
+_ref_demod = None
+
+
+def ref_demod():
+    """The reference's scalar demappers and pseudo-random generator built from source
+    (_ref/libref_demod.so), or None when not built."""
+    global _ref_demod
+    if _ref_demod is None:
+        path = HERE / "_ref" / "libref_demod.so"
+        if not path.exists():
+            return None
+        R = C.CDLL(str(path))
+        _fp = C.POINTER(C.c_float)
+        R.ref_demodulate_soft.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int]
+        R.ref_scrambling_sequence.argtypes = [_u8p, C.c_uint32, C.c_uint, C.c_uint]
+        _ref_demod = R
+    return _ref_demod
+
+
+def _demod_args(L, iq, nv):
+    iq = np.ascontiguousarray(iq, dtype=np.float32).reshape(-1)
+    nv = np.ascontiguousarray(nv, dtype=np.float32).reshape(-1)
+    assert iq.size == 2 * nv.size
+    return iq, nv
+
+
+def bits_per_symbol(modulation):
+    return 1 if modulation == 0 else modulation
+
+
+def demodulate_soft(iq, nv, modulation, use_ref=False):
+    """int8 LLRs of symbols iq (re, im interleaved float32) with noise variances nv."""
+    L = ref_demod() if use_ref else lib()
+    iq, nv = _demod_args(L, iq, nv)
+    out = np.zeros(nv.size * bits_per_symbol(modulation), np.int8)
+    fn = L.ref_demodulate_soft if use_ref else L.orc_demodulate_soft
+    fn(_p(out, _i8p), _p(iq, C.POINTER(C.c_float)), _p(nv, C.POINTER(C.c_float)), nv.size, modulation)
+    return out
+
+
+def scrambling_sequence(c_init, n, offset=0, use_ref=False):
+    """c(offset .. offset + n - 1) of TS38.211 5.2.1, one bit per byte."""
+    out = np.zeros(max(n, 1), np.uint8)
+    if use_ref:
+        ref_demod().ref_scrambling_sequence(_p(out, _u8p), c_init, offset, n)
+    else:
+        lib().orc_scrambling_sequence(_p(out, _u8p), c_init, offset, n)
+    return out[:n]
+
+
+def pusch_demodulate(iq, nv, modulation, c_init, seq_offset=0):
+    """Demodulated and descrambled LLRs of one block (the PUSCH demodulator's order)."""
+    L = lib()
+    iq, nv = _demod_args(L, iq, nv)
+    out = np.zeros(nv.size * bits_per_symbol(modulation), np.int8)
+    L.orc_pusch_demodulate(_p(out, _i8p), _p(iq, C.POINTER(C.c_float)), _p(nv, C.POINTER(C.c_float)), nv.size,
+                           modulation, c_init, seq_offset)
+    return out

@@ -36,6 +36,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(H.GenDesc) == 5 * 8 + 10 * 4
     assert ctypes.sizeof(H.GenTbDesc) == 48
     assert ctypes.sizeof(H.EncodeDesc) == 3 * 8 + 8 * 4
+    assert ctypes.sizeof(H.DemodDesc) == 2 * 8 + 4 * 4
     assert ctypes.sizeof(H.Segmentation) == 32
     assert ctypes.sizeof(H.TbRequest) == 8 + 8 + 10 * 4
     assert ctypes.sizeof(H.TbResult) == 8 + 6 * 4 + 8
