@@ -1346,7 +1346,7 @@ int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, con
     if ((uint64_t)d.seq_offset + bits > (1ull << 25)) {
       return fail(LH_ERR_INVALID, "block %u: sequence positions beyond 2^25", i);
     }
-    nof_tiles += (d.nof_symbols + 1023) / 1024;
+    nof_tiles += (d.nof_symbols + 255) / 256;
   }
   const size_t bytes = (size_t)n * sizeof(demod_job) + nof_tiles * sizeof(demod_tile);
   if (bytes > ctx->slot_bytes) {
@@ -1369,9 +1369,11 @@ int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, con
     j.llr_off = d.llr_offset;
     j.nof_symbols = d.nof_symbols;
     j.modulation = d.modulation;
-    demod_start_states(d.c_init, d.seq_offset, &j.x1, &j.x2);
-    for (uint32_t k = 0; k * 1024u < d.nof_symbols; ++k) {
-      tiles[t++] = demod_tile{i, k};
+    const uint32_t nt = (d.nof_symbols + 255) / 256;
+    std::vector<uint32_t> s1(nt), s2(nt);
+    demod_tile_states(d.c_init, d.seq_offset, d.modulation == 0 ? 1 : (int)d.modulation, nt, s1.data(), s2.data());
+    for (uint32_t k = 0; k < nt; ++k) {
+      tiles[t++] = demod_tile{i, k, s1[k], s2[k]};
     }
   }
   int rc = upload(ctx, s, bytes, st);

@@ -15,13 +15,13 @@
 // The float arithmetic is the reference's, operation by operation, with no contraction
 // into fused multiply-adds, so the LLRs are bit-exact with the reference's scalar path.
 //
-// Work split: one workgroup per tile of 1024 symbols of one block.  The tile's
-// 1024 * Qm sequence bits are made first, one 32-bit word per thread: the two LFSR states
-// at the tile's position come from the block's start states (computed on the host) by a
-// jump, a product with the precomputed transition matrices A^(32 * 2^j) over GF(2) for
-// the set bits of the word offset; then each thread demaps 4 symbols (strided by 256, so
-// the symbol loads and LLR stores of a wave are contiguous) and applies the sequence
-// bits from LDS.
+// Work split: one workgroup per tile of 256 symbols of one block, one symbol per thread.
+// The host gives each tile the two LFSR states at its first sequence bit (the block's
+// start by a jump, then tile to tile by one product with A^(256 Qm) over GF(2), as byte
+// tables).  The tile's 256 * Qm sequence bits are made first, one 32-bit word per thread
+// for the first 8 Qm threads: word w by a jump of 32 w bits, products with the matrices
+// A^(32 * 2^j) for the set bits of w (at most 6); then each thread demaps its symbol and
+// applies the sequence bits from LDS.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,7 +33,7 @@ namespace lh {
 
 namespace {
 
-constexpr int kTileSymbols = 1024;
+constexpr int kTileSymbols = 256;
 constexpr int kDemodThreads = 256;
 constexpr unsigned kNc = 1600;
 
@@ -140,12 +140,49 @@ void pam_table(int m, int level, int nof_intervals, int width_units, float rsq, 
   }
 }
 
+// A^(256 Qm) of both LFSRs as byte tables (entry [b][v]: image of byte b of the state
+// holding v), for the tile-to-tile step.
+struct tile_step {
+  uint32_t t[2][4][256];
+};
+
+tile_step make_tile_step(int qm)
+{
+  tile_step ts;
+  for (int lf = 0; lf < 2; ++lf) {
+    gf2m m;
+    for (int i = 0; i < 31; ++i) {
+      m.col[i] = jump(lf == 1, 1u << i, (uint64_t)kTileSymbols * qm);
+    }
+    for (int b = 0; b < 4; ++b) {
+      for (uint32_t v = 0; v < 256; ++v) {
+        ts.t[lf][b][v] = apply(m, (v << (8 * b)) & 0x7fffffffu);
+      }
+    }
+  }
+  return ts;
+}
+
+const tile_step& tile_steps(int qm)
+{
+  static const tile_step t1 = make_tile_step(1), t2 = make_tile_step(2), t4 = make_tile_step(4),
+                         t6 = make_tile_step(6), t8 = make_tile_step(8);
+  return qm == 1 ? t1 : qm == 2 ? t2 : qm == 4 ? t4 : qm == 6 ? t6 : t8;
+}
+
 } // namespace
 
-void demod_start_states(uint32_t c_init, uint64_t pos, uint32_t* x1, uint32_t* x2)
+void demod_tile_states(uint32_t c_init, uint64_t pos, int qm, uint32_t nof_tiles, uint32_t* x1, uint32_t* x2)
 {
-  *x1 = jump(false, 1u, kNc + pos);
-  *x2 = jump(true, c_init & 0x7fffffffu, kNc + pos);
+  uint32_t s1 = jump(false, 1u, kNc + pos);
+  uint32_t s2 = jump(true, c_init & 0x7fffffffu, kNc + pos);
+  const tile_step& ts = tile_steps(qm);
+  for (uint32_t k = 0; k < nof_tiles; ++k) {
+    x1[k] = s1;
+    x2[k] = s2;
+    s1 = ts.t[0][0][s1 & 255u] ^ ts.t[0][1][(s1 >> 8) & 255u] ^ ts.t[0][2][(s1 >> 16) & 255u] ^ ts.t[0][3][s1 >> 24];
+    s2 = ts.t[1][0][s2 & 255u] ^ ts.t[1][1][(s2 >> 8) & 255u] ^ ts.t[1][2][(s2 >> 16) & 255u] ^ ts.t[1][3][s2 >> 24];
+  }
 }
 
 void build_demod_tables(uint32_t* out)
@@ -311,13 +348,13 @@ __global__ void __launch_bounds__(kDemodThreads)
   for (uint32_t i = tid; i < kDemodFloats; i += kDemodThreads) {
     ft[i] = gft[i];
   }
-  // Sequence words of the tile: word w starts at bit tile * 1024 * Qm + 32 w of the block.
-  const uint32_t nwords = (uint32_t)(32 * qm);
+  // Sequence words of the tile: word w starts at bit 32 w of the tile.
+  const uint32_t nwords = (uint32_t)(kTileSymbols / 32 * qm);
   if (tid < nwords) {
-    uint32_t off = t.tile * (uint32_t)(32 * qm) + tid; // in words
-    uint32_t s1 = jb.x1, s2 = jb.x2;
-    for (int k = 0; k < kDemodJumps && off != 0; ++k, off >>= 1) {
-      if (off & 1u) {
+    uint32_t s1 = t.x1, s2 = t.x2;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if ((tid >> k) & 1u) {
         s1 = gf2_apply(tab + k * 31, s1);
         s2 = gf2_apply(tab + (kDemodJumps + k) * 31, s2);
       }
@@ -330,24 +367,19 @@ __global__ void __launch_bounds__(kDemodThreads)
     seq[nwords] = 0;
   }
   __syncthreads();
-  const uint32_t base = t.tile * kTileSymbols;
-#pragma unroll
-  for (int k = 0; k < kTileSymbols / kDemodThreads; ++k) {
-    const uint32_t ls = tid + k * kDemodThreads; // symbol in the tile
-    const uint32_t sym = base + ls;
-    if (sym < jb.nof_symbols) {
-      const float2 z = symbols[jb.sym_off + sym];
-      const float nv = noise[jb.sym_off + sym];
-      int l[8];
-      demap_symbol(mod, z.x, z.y, nv, (sym & 1u) != 0, ft, l);
-      const uint32_t pos = ls * (uint32_t)qm;
-      const uint64_t win = (uint64_t)seq[pos >> 5] | ((uint64_t)seq[(pos >> 5) + 1] << 32);
-      const uint32_t c = (uint32_t)(win >> (pos & 31u));
-      int8_t* out = llrs + jb.llr_off + (uint64_t)sym * qm;
-      for (int b = 0; b < qm; ++b) {
-        const int v = ((c >> b) & 1u) ? -l[b] : l[b];
-        out[b] = (int8_t)v;
-      }
+  const uint32_t sym = t.tile * kTileSymbols + tid;
+  if (sym < jb.nof_symbols) {
+    const float2 z = symbols[jb.sym_off + sym];
+    const float nv = noise[jb.sym_off + sym];
+    int l[8];
+    demap_symbol(mod, z.x, z.y, nv, (sym & 1u) != 0, ft, l);
+    const uint32_t pos = tid * (uint32_t)qm;
+    const uint64_t win = (uint64_t)seq[pos >> 5] | ((uint64_t)seq[(pos >> 5) + 1] << 32);
+    const uint32_t c = (uint32_t)(win >> (pos & 31u));
+    int8_t* out = llrs + jb.llr_off + (uint64_t)sym * qm;
+    for (int b = 0; b < qm; ++b) {
+      const int v = ((c >> b) & 1u) ? -l[b] : l[b];
+      out[b] = (int8_t)v;
     }
   }
 }

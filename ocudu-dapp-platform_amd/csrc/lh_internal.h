@@ -170,7 +170,7 @@ struct tb_signal {
 };
 
 // Demodulation + descrambling: one block (a span of symbols with its scrambling
-// sequence start) and one tile of 1024 of its symbols per workgroup.
+// sequence start) and one tile of 256 of its symbols per workgroup.
 constexpr int kDemodJumps = 20;   // A^(32 * 2^j), j < 20: blocks up to 2^25 sequence bits
 constexpr int kDemodFloats = 163; // slope/intercept tables + 1/sqrt(42), 1/sqrt(170), 1/sqrt(10)
 constexpr int kDemodTableWords = 2 * kDemodJumps * 31 + kDemodFloats;
@@ -179,13 +179,15 @@ struct demod_job {
   uint64_t llr_off;     // first LLR
   uint32_t nof_symbols;
   uint32_t modulation;  // 0 pi/2-BPSK, 1 BPSK, 2 QPSK, 4 16-QAM, 6 64-QAM, 8 256-QAM
-  uint32_t x1, x2;      // LFSR states at the block's first sequence bit (Nc included)
+  uint32_t pad[2];
 };
 struct demod_tile {
   uint32_t job;
   uint32_t tile;
+  uint32_t x1, x2; // LFSR states at the tile's first sequence bit (Nc included)
 };
-void demod_start_states(uint32_t c_init, uint64_t pos, uint32_t* x1, uint32_t* x2);
+// LFSR states at the first sequence bit of each of a block's tiles.
+void demod_tile_states(uint32_t c_init, uint64_t pos, int qm, uint32_t nof_tiles, uint32_t* x1, uint32_t* x2);
 void build_demod_tables(uint32_t* out);
 
 inline int crc_index(uint32_t poly)
