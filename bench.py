@@ -231,7 +231,7 @@ def main():
     # Step before the path: soft demodulation + descrambling of one full uplink slot of
     # equalized symbols (273 PRB x 12 subcarriers x 12 data symbols x 4 layers, 256-QAM, one
     # block per OFDM symbol as the PUSCH demodulator calls it).
-    demod = demod_slot(ctx, stream, ev0, ev1)
+    demod = demod_slot(ctx, stream, ev0, ev1, cpu=not a.no_cpu_baseline)
     print(f"# demodulation + descrambling: {json.dumps(demod)}", file=sys.stderr, flush=True)
 
     uneven = uneven_batch(ctx, a) if a.uneven_ues else None
@@ -349,11 +349,13 @@ def measured_traffic():
     return None
 
 
-def demod_slot(ctx, stream, ev0, ev1):
+def demod_slot(ctx, stream, ev0, ev1, cpu=True):
     """lh_pusch_demodulate over one full slot: 12 blocks (one per OFDM symbol) of 13104
     symbols (273 PRB x 12 subcarriers x 4 layers), 256-QAM, each block continuing the
     codeword's scrambling sequence.  HBM bytes per launch: 8 (symbol) + 4 (noise variance)
-    + 8 (LLRs) per symbol."""
+    + 8 (LLRs) per symbol.  With cpu: the reference's own scalar demapper and
+    pseudo-random generator (oracle/_ref/libref_demod.so, built from its sources) on one
+    host thread over the same slot, timed, and its LLRs compared with the device's."""
     import ocudu_hip as H
 
     rng = np.random.default_rng(11)
@@ -373,11 +375,30 @@ def demod_slot(ctx, stream, ev0, ev1):
         ev1.record(stream)
         ev1.sync()
         ms = min(ms, ev0.elapsed_ms(ev1))
+    ref = None
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        if O.ref_demod() is not None:
+            got = d_llr.download(np.int8, nsym * qm)
+            t0 = time.perf_counter()
+            want = np.empty(nsym * qm, np.int8)
+            for b, d in enumerate(descs):
+                sl = slice(b * per, (b + 1) * per)
+                llr = O.demodulate_soft(iq[2 * sl.start:2 * sl.stop], nv[sl], qm, use_ref=True)
+                c = O.scrambling_sequence(d.c_init, per * qm, d.seq_offset, use_ref=True)
+                want[sl.start * qm:sl.stop * qm] = np.where(c == 1, -llr, llr)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            ref = {"kind": "reference", "cores": 1, "ms": round(cpu_ms, 2),
+                   "sample": "the same slot, reference scalar demapper + pseudo-random generator",
+                   "bit_exact": bool(np.array_equal(got, want))}
     for d in (d_iq, d_nv, d_llr):
         d.free()
     gbytes = nsym * (8 + 4 + qm) / 1e9
     return {"symbols": nsym, "llrs": nsym * qm, "blocks": nblk, "ms": round(ms, 4),
-            "llr_gbps": round(nsym * qm / (ms * 1e-3) / 1e9, 1), "hbm_gb_s": round(gbytes / (ms * 1e-3), 1)}
+            "llr_gbps": round(nsym * qm / (ms * 1e-3) / 1e9, 1), "hbm_gb_s": round(gbytes / (ms * 1e-3), 1),
+            "cpu_reference": ref}
 
 
 def cpu_baseline(d_rm, n_cb, N, info_bits, max_iter, cpu_seconds, snr):
