@@ -7,7 +7,8 @@ inputs (rate-matched int8 LLRs + dematching parameters) and the reference output
 (rate-dematched buffer checksum, hard bits, iteration count / CRC verdict).  The
 fixtures are data only; the tests compare the oracle and the HIP path against them.
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py              # every fixture
+    python tests/golden/gen_golden.py high_rate    # only high_rate_l4.npz
 """
 import sys
 import zlib
@@ -29,11 +30,57 @@ def run_case(dec, bg, Z, E, Qm, snr, seed, rv=0, F=0, max_iter=6):
     return le[0], rm, poly, cb, r, bits
 
 
+def _pack(rows, llr_blob, bits_blob, path, columns):
+    meta = np.array(rows, np.int64)
+    offs = np.cumsum([0] + [a.size for a in llr_blob])
+    boffs = np.cumsum([0] + [a.size for a in bits_blob])
+    np.savez_compressed(path, meta=meta, llr=np.concatenate(llr_blob), llr_off=offs, bits=np.concatenate(bits_blob),
+                        bits_off=boffs, columns=np.array(columns.split()))
+
+
+# Es/N0 (dB) near the 8-iteration decoding threshold at code rate ~0.93, per base graph and
+# bits per symbol (calibrated on the oracle's generator).
+HIGH_RATE_SNR = {1: {2: 8.6, 4: 14.3, 6: 20.0, 8: 25.6}, 2: {2: 6.4, 4: 12.2, 6: 17.6, 8: 23.4}}
+
+
+def high_rate(dec):
+    """The high-rate regime: rv0 codeblocks whose soft bits end before (K_b + 2) Z, so the
+    decoder runs the (K_b + 4) Z floor of four layers (ldpc_decoder_impl.cpp:90-99), with
+    8 iterations near threshold.  First the headline slot shape (BG1 Z=384, 256-QAM,
+    E=9088) at Es/N0 24.9..25.4 dB, then both base graphs over every third lifting size and
+    every modulation, E within (K_b + 2) Z, with and without fillers."""
+    rows, llr_blob, bits_blob = [], [], []
+
+    def add(bg, Z, E, Qm, F, snr, seed):
+        le, rm, poly, cb, r, b = run_case(dec, bg, Z, E, Qm, snr, seed, F=F, max_iter=8)
+        rows.append((bg, Z, E, Qm, 0, F, poly, cb, r, zlib.crc32(rm.tobytes()), int(round(snr * 100))))
+        llr_blob.append(le)
+        bits_blob.append(b)
+
+    for k, snr in enumerate((24.9, 25.0, 25.1, 25.2, 25.3, 25.4)):
+        for c in range(8):
+            add(1, 384, 9088, 8, 0, snr, 7000 + 8 * k + c)
+    rng = np.random.default_rng(20260103)
+    for bg in (1, 2):
+        for Z in O.LIFTING_SIZES[::3] + [384]:
+            nsys = (O.K_b(bg) - 2) * Z
+            for Qm in (2, 4, 6, 8):
+                F = int(rng.choice([0, 8, 16])) if nsys > 64 else 0
+                E = int((O.K_b(bg) + 2) * Z * rng.uniform(0.93, 1.0) - F) // Qm * Qm
+                snr = HIGH_RATE_SNR[bg][Qm] + float(rng.choice([-0.4, 0.0, 0.4]))
+                add(bg, Z, E, Qm, F, snr, int(rng.integers(0, 2 ** 31)))
+    _pack(rows, llr_blob, bits_blob, HERE / "high_rate_l4.npz", "bg Z E Qm rv F crc_poly crc_bits iters rm_crc32 snr_cdb")
+
+
 def main():
     R = O.ref()
     if R is None:
         raise SystemExit("build oracle/_ref first (make -f oracle/Makefile.ref)")
     dec = R.ref_decoder_create(0, 0, 0)  # generic, force_decoding=false, early_stop_syndrome=false
+    if sys.argv[1:] == ["high_rate"]:
+        high_rate(dec)
+        R.ref_decoder_destroy(dec)
+        return
 
     # cfg1: 64 CBs BG1 Z=384, E=16896, QPSK, Es/N0 2 dB, rv0, 6 iterations, CRC24B.
     msgs, llrs = O.make_codeblocks(64, 1, 384, 16896, 2, 2.0, seed=20260101)
@@ -67,12 +114,9 @@ def main():
                     rows.append((bg, Z, E, Qm, rv, F, poly, cb, r, zlib.crc32(rm.tobytes())))
                     llr_blob.append(le)
                     bits_blob.append(b)
-    meta = np.array([r[:10] for r in rows], np.int64)
-    offs = np.cumsum([0] + [a.size for a in llr_blob])
-    boffs = np.cumsum([0] + [a.size for a in bits_blob])
-    np.savez_compressed(HERE / "sweep_bg_z.npz", meta=meta, llr=np.concatenate(llr_blob), llr_off=offs,
-                        bits=np.concatenate(bits_blob), bits_off=boffs,
-                        columns=np.array("bg Z E Qm rv F crc_poly crc_bits iters rm_crc32".split()))
+    _pack([r[:10] for r in rows], llr_blob, bits_blob, HERE / "sweep_bg_z.npz",
+          "bg Z E Qm rv F crc_poly crc_bits iters rm_crc32")
+    high_rate(dec)
     R.ref_decoder_destroy(dec)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 

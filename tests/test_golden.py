@@ -12,11 +12,11 @@ import oracle as O
 G = Path(__file__).resolve().parent / "golden"
 
 
-def _sweep():
-    d = np.load(G / "sweep_bg_z.npz")
+def _sweep(name="sweep_bg_z.npz"):
+    d = np.load(G / name)
     out = []
     for k, row in enumerate(d["meta"]):
-        bg, Z, E, Qm, rv, F, poly, cb, iters, rmcrc = (int(x) for x in row)
+        bg, Z, E, Qm, rv, F, poly, cb, iters, rmcrc = (int(x) for x in row[:10])
         llr = d["llr"][d["llr_off"][k]:d["llr_off"][k + 1]]
         bits = d["bits"][d["bits_off"][k]:d["bits_off"][k + 1]]
         out.append(dict(bg=bg, Z=Z, E=E, Qm=Qm, rv=rv, F=F, poly=poly, cb=cb, iters=iters, rmcrc=rmcrc, llr=llr,
@@ -42,14 +42,26 @@ def test_oracle_sweep():
         assert r == c["iters"] and np.array_equal(b, c["bits"]), (k, c["bg"], c["Z"])
 
 
-@pytest.mark.gpu
-def test_gpu_sweep_fused():
-    import ocudu_hip as H
-    from gpu_util import require_gpu
+def test_oracle_high_rate():
+    """Fixtures of the four-layer regime (soft bits end before (K_b + 2) Z: the (K_b + 4) Z
+    floor of ldpc_decoder_impl.cpp:90-99), 8 iterations near threshold, incl. the headline
+    slot shape BG1 Z=384 256-QAM E=9088."""
+    cases = _sweep("high_rate_l4.npz")
+    assert sum(c["iters"] == 0 for c in cases) >= 40 and sum(c["iters"] > 0 for c in cases) >= 40
+    for k, c in enumerate(cases):
+        rm = O.rate_dematch(c["llr"], c["bg"], c["Z"], rv=c["rv"], Qm=c["Qm"], F=c["F"])
+        assert zlib.crc32(rm.tobytes()) == c["rmcrc"], k
+        assert np.flatnonzero(rm).max() < (O.K_b(c["bg"]) + 2) * c["Z"], k  # four layers
+        r, b = O.decode(rm, c["bg"], c["Z"], F=c["F"], crc_poly=c["poly"], max_iter=8, crc_bits=c["cb"])
+        assert r == c["iters"] and np.array_equal(b, c["bits"]), (k, c["bg"], c["Z"])
 
-    require_gpu()
-    ctx = H.Context(0, 1024)
-    cases = _sweep()
+
+def _gpu_fused(name, max_iter, rows):
+    import ocudu_hip as H
+    from gpu_util import make_ctx
+
+    ctx = make_ctx(1024, rows)
+    cases = _sweep(name)
     cbs, parts = [], []
     cw_off = rm_off = bo = 0
     for c in cases:
@@ -67,7 +79,7 @@ def test_gpu_sweep_fused():
     d_cw = H.DeviceBuffer.from_array(cw)
     d_rm, d_bits, d_res = H.DeviceBuffer(rm_off), H.DeviceBuffer(bo), H.DeviceBuffer(4 * len(cases))
     d_bits.zero()
-    ctx.pusch_decode_batch(cbs, d_cw, d_rm, d_bits, d_res, nof_ldpc_iterations=6)
+    ctx.pusch_decode_batch(cbs, d_cw, d_rm, d_bits, d_res, nof_ldpc_iterations=max_iter)
     res = d_res.download(np.int32, len(cases))
     rm = d_rm.download(np.int8, rm_off)
     bits = d_bits.download(np.uint8, bo)
@@ -76,6 +88,20 @@ def test_gpu_sweep_fused():
         assert zlib.crc32(rm[ro:ro + N].tobytes()) == c["rmcrc"], k
         assert res[k] == c["iters"] and np.array_equal(bits[b0:b0 + nb], c["bits"]), (k, c["bg"], c["Z"])
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", ["auto", "pairs"])
+def test_gpu_sweep_fused(rows):
+    print(f"case golden sweep rows={rows}", flush=True)
+    _gpu_fused("sweep_bg_z.npz", 6, rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", ["auto", "pairs"])
+def test_gpu_high_rate_fused(rows):
+    print(f"case golden high rate rows={rows}", flush=True)
+    _gpu_fused("high_rate_l4.npz", 8, rows)
 
 
 @pytest.mark.gpu
