@@ -172,17 +172,19 @@ struct lh_ctx {
 
 namespace {
 
-// Returns a staging slot whose previous upload has completed.  If the slot's event can no
-// longer be waited on (its stream was destroyed since), the device is drained instead and
-// the runtime's error state cleared, so the failure does not surface in a later launch.
+// Returns a staging slot whose previous upload has completed, or nullptr (error set) when
+// waiting for it fails: a device error is reported, not cleared.  Slot events never refer
+// to destroyed streams (rebind_ring runs before a server destroys its streams).
 slot* acquire_slot(lh_ctx* ctx)
 {
   slot* s = &ctx->ring[ctx->next];
   ctx->next = (ctx->next + 1) % kRing;
-  if (s->used && hipEventSynchronize(s->done) != hipSuccess) {
-    (void)hipGetLastError();
-    (void)hipDeviceSynchronize();
-    (void)hipGetLastError();
+  if (s->used) {
+    const hipError_t e = hipEventSynchronize(s->done);
+    if (e != hipSuccess) {
+      hip_fail(e, "wait for a descriptor staging slot");
+      return nullptr;
+    }
   }
   return s;
 }
@@ -393,6 +395,7 @@ bool pair_rows(const lh_ctx* ctx, const dec_job& j, size_t n)
 // and the launch shape of each row mode (index 1: row pairs).
 struct plan_block {
   std::vector<uint8_t> bytes;
+  std::vector<dec_item> items, tmp; // work items while packing
   size_t o_items = 0, o_jobs = 0;
   size_t n_items[2] = {0, 0};
   int nt[2] = {64, 64};
@@ -441,7 +444,9 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
   }
   nt[0] = std::min(nt[0], kMaxThreads);
   nt[1] = std::min(nt[1], kMaxThreads);
-  std::stable_sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
+  // Total order (ties by result index), so an unstable sort, which needs no temporary
+  // buffer, gives the same queue as a stable one.
+  std::sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
     const int ka = (int)a.Z * (a.lbound + 22), kb = (int)b.Z * (b.lbound + 22);
     const int bga = a.gt_off < kNofLiftingSizes * 316 ? 1 : 2, bgb = b.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
     if (ka != kb) {
@@ -450,9 +455,13 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     if (bga != bgb) {
       return bga < bgb;
     }
-    return a.Z > b.Z;
+    if (a.Z != b.Z) {
+      return a.Z > b.Z;
+    }
+    return a.out_idx < b.out_idx;
   });
-  std::vector<dec_item> items;
+  std::vector<dec_item>& items = pb.items;
+  items.clear();
   size_t lds[2] = {0, 0};
   for (size_t i = 0; i < jobs.size();) {
     const int bg = jobs[i].gt_off < kNofLiftingSizes * 316 ? 1 : 2;
@@ -497,11 +506,19 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     i = k;
   }
   // Row-pair items first (each mode keeps its largest-first order).
-  std::stable_partition(items.begin(), items.end(), [](const dec_item& it) { return it.rows2 != 0; });
-  size_t n_pair = 0;
-  while (n_pair < items.size() && items[n_pair].rows2) {
-    ++n_pair;
+  pb.tmp.clear();
+  for (const dec_item& it : items) {
+    if (it.rows2) {
+      pb.tmp.push_back(it);
+    }
   }
+  const size_t n_pair = pb.tmp.size();
+  for (const dec_item& it : items) {
+    if (!it.rows2) {
+      pb.tmp.push_back(it);
+    }
+  }
+  items.swap(pb.tmp);
   launch_header hdr[4]{};
   for (int pass = 0; pass < 2; ++pass) {
     hdr[2 * pass].n_items = (uint32_t)n_pair;
@@ -803,6 +820,9 @@ int lh_ldpc_decode(lh_ctx* ctx, const lh_decode_desc* descs, unsigned n, const i
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   return run_plan(ctx, plan, const_cast<int8_t*>(d_llrs), nullptr, d_bits, d_results, st, s, 0);
 }
 
@@ -831,6 +851,9 @@ int lh_rate_dematch(lh_ctx* ctx, const lh_rm_desc* descs, unsigned n, const int8
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   std::memcpy(s->host, jobs.data(), n * sizeof(rm_params));
   int rc = upload(ctx, s, n * sizeof(rm_params), st);
   if (rc != LH_OK) {
@@ -929,6 +952,9 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0);
 }
 
@@ -988,6 +1014,9 @@ int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0, ctx->d_scale, nof_fallback_iterations);
 }
 
@@ -1191,6 +1220,9 @@ int lh_tb_finalize(lh_ctx* ctx, const lh_tb_desc* tbs, unsigned n, const uint8_t
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   std::memcpy(s->host, tbs, n * sizeof(lh_tb_desc));
   int rc = upload(ctx, s, n * sizeof(lh_tb_desc), st);
   if (rc != LH_OK) {
@@ -1216,6 +1248,9 @@ int lh_gen_codeblocks(lh_ctx* ctx, const lh_gen_desc* descs, unsigned n, int8_t*
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   gen_job* jobs = reinterpret_cast<gen_job*>(s->host);
   for (unsigned i = 0; i < n; ++i) {
     const lh_gen_desc& d = descs[i];
@@ -1290,6 +1325,9 @@ int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   gen_job* jobs = reinterpret_cast<gen_job*>(s->host);
   for (unsigned i = 0; i < n; ++i) {
     const lh_encode_desc& d = descs[i];
@@ -1359,6 +1397,9 @@ int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, con
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   demod_job* jobs = reinterpret_cast<demod_job*>(s->host);
   demod_tile* tiles = reinterpret_cast<demod_tile*>(s->host + (size_t)n * sizeof(demod_job));
   size_t t = 0;
@@ -1459,6 +1500,9 @@ int lh_nms_train_step(lh_ctx* ctx, unsigned base_graph, const lh_train_desc* des
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   train_job* jobs = reinterpret_cast<train_job*>(s->host);
   size_t ws = 0;
   int maxZ = 0;
@@ -1544,6 +1588,9 @@ int lh_gen_transport_blocks(lh_ctx* ctx, const lh_gen_tb_desc* descs, unsigned n
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   std::memcpy(s->host, descs, n * sizeof(lh_gen_tb_desc));
   int rc = upload(ctx, s, n * sizeof(lh_gen_tb_desc), st);
   if (rc != LH_OK) {
@@ -1577,6 +1624,9 @@ int lh_crc_calculate(lh_ctx* ctx, uint32_t poly, const uint64_t* offsets, const 
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
   std::memcpy(s->host, offsets, n * 8);
   std::memcpy(s->host + n * 8, nbits, n * 4);
   int rc = upload(ctx, s, n * 12, st);

@@ -1,5 +1,6 @@
 // Shared host/device definitions of the HIP LDPC receive chain (not part of the C ABI).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 namespace lh {
