@@ -137,6 +137,20 @@ int lh_pusch_decode_batch_learned(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
                                   int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
                                   unsigned nof_ldpc_iterations, unsigned nof_fallback_iterations, void* stream);
 
+/* lh_pusch_decode_batch without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * as lh_pusch_decode_batch; the launch's descriptor block (work-queue headers and
+ * counters, work items, codeblock jobs) goes to a static device array of the library
+ * (one per GPU, held until the process ends) instead of the context's staging buffers,
+ * and the host packs it in workspaces the context reserved at creation.  At most 4096
+ * codeblocks per call (LH_ERR_CAPACITY beyond).  Every call on one GPU shares that array:
+ * calls on one stream are ordered, but calls that overlap on one GPU (other streams,
+ * other contexts of the same device) race. */
+int lh_pusch_decode_batch_noalloc(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw,
+                                  int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                                  unsigned nof_ldpc_iterations, void* stream);
+
 /* ---- Training of the learned decoder (no counterpart in the reference: its decoders
  * use a fixed scaling factor).  The model is the layered normalised-offset min-sum
  * decoder unrolled over `iters` iterations in FP32, with per-(iteration, layer) scale

@@ -52,6 +52,7 @@ hipError_t launch_gen_tb(const gen_tb_job* d_jobs, unsigned n, uint8_t* tb, uint
                          const uint32_t* lut24a, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
                       const uint32_t* xtab, int order, uint32_t* out, hipStream_t s);
+hipError_t scratch_address(uint8_t** ptr);
 } // namespace lh
 
 using namespace lh;
@@ -168,6 +169,13 @@ struct lh_ctx {
   // Descriptor uploads run here, so that a call's upload overlaps the kernels of the
   // previous call on the caller's stream (default priority, non-blocking).
   hipStream_t copy = nullptr;
+  // This device's copy of the static scratch (lh_scratch.hip), for the allocation-free
+  // entry point, and the host workspaces that entry point packs its launch in (reserved
+  // for kNoAllocMaxCbs codeblocks at creation, so packing does not reallocate).
+  uint8_t* d_scratch = nullptr;
+  std::vector<dec_job> ws_jobs;
+  std::vector<uint8_t> ws_bytes;
+  std::vector<dec_item> ws_items, ws_tmp;
 };
 
 namespace {
@@ -723,6 +731,17 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
     lh_ctx_destroy(c);
     return hip_fail(e, "context allocation");
   }
+  // With the device current: its address of the static scratch.
+  e = scratch_address(&c->d_scratch);
+  if (e != hipSuccess) {
+    lh_ctx_destroy(c);
+    return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  }
+  const size_t ws_cbs = std::min<size_t>(max_codeblocks, kNoAllocMaxCbs);
+  c->ws_jobs.reserve(ws_cbs);
+  c->ws_items.reserve(ws_cbs);
+  c->ws_tmp.reserve(ws_cbs);
+  c->ws_bytes.reserve(kScratchBytes);
   *out = c;
   return LH_OK;
 }
@@ -956,6 +975,72 @@ int lh_pusch_decode_batch(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, 
     return LH_ERR_HIP;
   }
   return run_plan(ctx, plan, d_rm, d_cw, d_bits, d_results, st, s, 0);
+}
+
+int lh_pusch_decode_batch_noalloc(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, const int8_t* d_cw,
+                                  int8_t* d_rm, uint8_t* d_bits, int32_t* d_results, int new_data, int use_early_stop,
+                                  unsigned nof_ldpc_iterations, void* stream)
+{
+  int rc = check_pusch_args(ctx, cbs, n, d_cw, d_rm, d_bits, d_results, "lh_pusch_decode_batch_noalloc");
+  if (rc != LH_OK) {
+    return rc;
+  }
+  if (n > kNoAllocMaxCbs) {
+    return fail(LH_ERR_CAPACITY, "lh_pusch_decode_batch_noalloc: %u codeblocks exceed the static scratch maximum of %u",
+                n, kNoAllocMaxCbs);
+  }
+  if (ctx->d_scratch == nullptr) {
+    return fail(LH_ERR_INVALID, "lh_pusch_decode_batch_noalloc: no scratch address for device %d", ctx->device);
+  }
+  // The launch is packed in the context's reserved host vectors (swapped in and back out,
+  // never reallocated) and its descriptor block goes to the static scratch.
+  launch_plan plan;
+  plan_block pb;
+  struct swap_guard {
+    lh_ctx* c;
+    launch_plan& pl;
+    plan_block& b;
+    swap_guard(lh_ctx* c_, launch_plan& pl_, plan_block& b_) : c(c_), pl(pl_), b(b_) { swap_all(); }
+    ~swap_guard() { swap_all(); }
+    void swap_all()
+    {
+      pl.jobs.swap(c->ws_jobs);
+      b.bytes.swap(c->ws_bytes);
+      b.items.swap(c->ws_items);
+      b.tmp.swap(c->ws_tmp);
+    }
+  } guard(ctx, plan, pb);
+  plan.jobs.clear();
+  rc = pusch_jobs(cbs, n, new_data, use_early_stop, nof_ldpc_iterations, plan);
+  if (rc != LH_OK || n == 0) {
+    return rc;
+  }
+  rc = build_block(ctx, plan, pb);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  if (pb.bytes.size() > kScratchBytes) {
+    return fail(LH_ERR_CAPACITY, "lh_pusch_decode_batch_noalloc: descriptor block of %zu bytes exceeds the scratch",
+                pb.bytes.size());
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
+  // Staged through the slot's pinned host half; the copy is ordered on the caller's
+  // stream behind the previous call's kernels, which read the same scratch.  The work-queue
+  // counters arrive as zeros with the block.
+  std::memcpy(s->host, pb.bytes.data(), pb.bytes.size());
+  hipError_t e = hipMemcpyAsync(ctx->d_scratch, s->host, pb.bytes.size(), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    return hip_fail(e, "descriptor upload to the scratch");
+  }
+  s->used = true;
+  e = launch_block(ctx, pb, ctx->d_scratch, d_rm, d_cw, d_bits, d_results, st, nullptr, 0);
+  retire(s, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
 
 int lh_set_scale_table(lh_ctx* ctx, unsigned base_graph, const uint16_t* alpha_q, const int16_t* beta_q,

@@ -92,6 +92,12 @@ struct alignas(16) launch_header {
   uint32_t pad;
 };
 
+// Static scratch of the allocation-free decode entry point: one descriptor block (four
+// headers, at most one item and one job per codeblock) of up to kNoAllocMaxCbs codeblocks.
+constexpr unsigned kNoAllocMaxCbs = 4096;
+constexpr size_t kScratchBytes =
+    ((4 * sizeof(launch_header) + kNoAllocMaxCbs * (sizeof(dec_item) + sizeof(dec_job))) + 255) & ~(size_t)255;
+
 // One synthetic codeblock of the generator (128 bytes): message bits from seed_msg,
 // noise from seed_noise, rate matching by `rm` (cw_off = LLR output offset).
 struct alignas(16) gen_job {

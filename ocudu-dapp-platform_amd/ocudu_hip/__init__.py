@@ -246,7 +246,8 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_set_scale_table", "lh_pusch_decode_batch_learned", "lh_nms_workspace_bytes", "lh_nms_train_step",
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
-            "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch"]
+            "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch",
+            "lh_pusch_decode_batch_noalloc"]
 
 
 def lib():
@@ -265,6 +266,7 @@ def lib():
                                       C.c_void_p]
         L.lh_pusch_decode_batch.argtypes = [C.c_void_p, C.POINTER(PuschCbDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint, C.c_void_p]
+        L.lh_pusch_decode_batch_noalloc.argtypes = L.lh_pusch_decode_batch.argtypes
         L.lh_tb_finalize.argtypes = [C.c_void_p, C.POINTER(TbDesc), C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]
         L.lh_set_scale_table.argtypes = [C.c_void_p, C.c_uint, C.POINTER(C.c_uint16), C.POINTER(C.c_int16),
