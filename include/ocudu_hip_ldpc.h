@@ -391,8 +391,10 @@ int lh_segment_rx(unsigned base_graph, unsigned tbs_bits, unsigned modulation_or
  * them).  As the reference's rx_buffer, a HARQ process keeps each codeblock's message bits
  * and CRC state: a codeblock whose CRC passed in an earlier transmission only combines the
  * new LLRs and is not decoded again (pusch_decoder_impl.cpp:412-431), and all are decoded
- * again after a TB CRC failure with every codeblock passed (:657-669).  Single-threaded:
- * one caller drives submit / step / poll. */
+ * again after a TB CRC failure with every codeblock passed (:657-669).  As rx_buffer_pool
+ * (lib/phy/upper/rx_buffer_pool_impl.cpp:79-125), a process expires expire_timeout_slots
+ * after the slot of its last reservation (lh_server_run_slot), unless it is in use, which
+ * renews its expiry.  Single-threaded: one caller drives submit / step / poll / run_slot. */
 typedef struct lh_server lh_server;
 
 typedef struct lh_server_config {
@@ -403,6 +405,8 @@ typedef struct lh_server_config {
   unsigned nof_ldpc_iterations;
   unsigned nof_fallback_iterations;/* learned decoder: plain fallback iterations (0 = none) */
   int learned;                     /* decode with the learned tables */
+  unsigned expire_timeout_slots;   /* HARQ process lifetime in slots after its last reservation (0 = no
+                                      expiry): rx_buffer_pool_config::expire_timeout_slots */
 } lh_server_config;
 
 typedef struct lh_tb_request {
@@ -417,7 +421,9 @@ typedef struct lh_tb_request {
   uint32_t nof_ch_symbols;
   uint32_t rv;
   uint32_t Nref;
-  uint32_t pad;
+  uint32_t slot;            /* slot of the transmission (a monotonic counter, as lh_server_run_slot's): its HARQ
+                               process expires expire_timeout_slots after it, or after the slot it is admitted in
+                               when it waited for HARQ memory past its own slot */
 } lh_tb_request;
 
 #define LH_TB_REJECTED (-2) /* a retransmission without matching HARQ buffers */
@@ -455,6 +461,12 @@ int lh_server_step(lh_server* server);
 int lh_server_poll(lh_server* server, lh_tb_result* out, unsigned max, unsigned* n_out, int wait);
 int lh_server_harq_release(lh_server* server, uint32_t harq_process);
 int lh_server_stats(lh_server* server, lh_server_stats_t* stats);
+/* Housekeeping of the HARQ processes at `slot` (rx_buffer_pool::run_slot,
+ * include/ocudu/phy/upper/rx_buffer_pool.h:68, lib/phy/upper/rx_buffer_pool_impl.cpp:88-125):
+ * a process in use (admitted, not yet completed) has its expiry renewed to slot +
+ * expire_timeout_slots; any other whose expiry is at or before `slot` is released, so a
+ * later retransmission for it is rejected.  Slots compare modulo 2^32. */
+int lh_server_run_slot(lh_server* server, uint32_t slot);
 
 #ifdef __cplusplus
 }

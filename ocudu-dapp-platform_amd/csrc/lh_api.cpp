@@ -1748,6 +1748,7 @@ struct harq_entry {
   uint32_t nof_cbs = 0, Z = 0, bg = 0, tbs = 0;
   std::vector<uint8_t> cb_iters;
   bool busy = false;
+  uint32_t expiry = 0; // slot at which it expires when not in use (expire_timeout_slots > 0)
 };
 
 uint64_t harq_bytes(uint32_t bg, uint32_t Z, uint32_t nof_cbs, uint64_t* msg_at = nullptr)
@@ -1861,6 +1862,8 @@ struct lh_server {
   std::deque<lh_tb_result> results;
   std::unordered_map<uint64_t, uint8_t*> tb_dest; // host TB destinations by request id
   unsigned next_slot = 0;
+  uint32_t cur_slot = 0; // last lh_server_run_slot slot
+  bool have_slot = false;
 };
 
 namespace {
@@ -1926,6 +1929,10 @@ void admit(lh_server* sv)
       h = sv->harq.emplace(r.req.harq_process, e).first;
     }
     h->second.busy = true;
+    // Reserved now (rx_buffer_pool_impl.cpp:79-80): at the request's slot, or at the current
+    // slot when the request waited for memory past it.
+    const uint32_t at = (sv->have_slot && (int32_t)(sv->cur_slot - r.req.slot) > 0) ? sv->cur_slot : r.req.slot;
+    h->second.expiry = at + sv->cfg.expire_timeout_slots;
     r.harq_off = h->second.off;
     sv->ready.push_back(std::move(r));
     it = sv->pending.erase(it);
@@ -1976,6 +1983,27 @@ void retire_slot(lh_server* sv, srv_slot& sl)
   sl.busy = false;
 }
 
+// A slot whose launch failed or was never made: nothing of it will complete.  Its requests
+// are reported as rejected and leave no state behind (destinations, busy processes); a new
+// transmission's HARQ process is dropped, since its soft bits were never written.
+void reject_slot(lh_server* sv, srv_slot& sl)
+{
+  for (auto& r : sl.reqs) {
+    auto h = sv->harq.find(r.req.harq_process);
+    if (h != sv->harq.end()) {
+      h->second.busy = false;
+      if (r.req.new_data) {
+        sv->arena.release(h->second.off, h->second.bytes);
+        sv->harq.erase(h);
+      }
+    }
+    sv->tb_dest.erase(r.req.id);
+    push_result(sv, r, LH_TB_REJECTED, 0, 0);
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
 } // namespace
 
 extern "C" {
@@ -1984,7 +2012,7 @@ int lh_server_create(lh_ctx* ctx, const lh_server_config* cfg, lh_server** out)
 {
   if (ctx == nullptr || cfg == nullptr || out == nullptr || cfg->max_cbs_per_slot == 0 ||
       cfg->max_cbs_per_slot > ctx->max_cb || cfg->max_slots_in_flight == 0 || cfg->max_slots_in_flight > 8 ||
-      cfg->harq_budget_bytes < 25344 || cfg->nof_ldpc_iterations == 0 || cfg->nof_ldpc_iterations > 255 ||
+      cfg->harq_budget_bytes < align256(harq_bytes(1, 384, 1)) || cfg->nof_ldpc_iterations == 0 || cfg->nof_ldpc_iterations > 255 ||
       cfg->nof_fallback_iterations > 255) {
     return fail(LH_ERR_INVALID, "invalid arguments to lh_server_create");
   }
@@ -2065,10 +2093,29 @@ int submit_one(lh_server* sv, const lh_tb_request* req, uint8_t* tb_out, double 
   }
   r.rm.resize(r.seg.nof_cbs);
   r.off.resize(r.seg.nof_cbs);
-  const uint64_t bytes = (uint64_t)r.seg.nof_cbs * (req->base_graph == 1 ? 66u : 50u) * r.seg.lifting_size;
-  if (r.seg.nof_cbs > sv->cfg.max_cbs_per_slot || bytes > sv->cfg.harq_budget_bytes || req->rv > 3) {
-    return fail(LH_ERR_CAPACITY, "request %llu: %u codeblocks cannot be served", (unsigned long long)req->id,
-                r.seg.nof_cbs);
+  // Every codeblock's dematching parameters are checked here, so that a launch never
+  // fails on them (rate-matched length, N_ref, RV, modulation order).
+  for (uint32_t c = 0; c < r.seg.nof_cbs; ++c) {
+    lh_rm_desc d{};
+    d.rm_length = r.rm[c];
+    d.base_graph = req->base_graph;
+    d.lifting_size = r.seg.lifting_size;
+    d.rv = req->rv;
+    d.modulation_order = req->modulation_order;
+    d.Nref = req->Nref;
+    d.nof_filler_bits = r.seg.nof_filler_bits;
+    rm_params j;
+    uint64_t mp = 0;
+    rc = make_rm_params(d, req->new_data != 0, j, mp, c);
+    if (rc != LH_OK) {
+      return fail(LH_ERR_INVALID, "request %llu: %s", (unsigned long long)req->id, lh_last_error());
+    }
+  }
+  // The HARQ process's whole footprint (soft bits and message bits, as the arena places it).
+  const uint64_t bytes = align256(harq_bytes(req->base_graph, r.seg.lifting_size, r.seg.nof_cbs));
+  if (r.seg.nof_cbs > sv->cfg.max_cbs_per_slot || bytes > sv->cfg.harq_budget_bytes) {
+    return fail(LH_ERR_CAPACITY, "request %llu: %u codeblocks (%llu HARQ bytes) cannot be served",
+                (unsigned long long)req->id, r.seg.nof_cbs, (unsigned long long)bytes);
   }
   r.t_submit = t_submit;
   sv->tb_dest[req->id] = tb_out;
@@ -2192,6 +2239,7 @@ int lh_server_step(lh_server* sv)
     rc = validate_tbs(tbs.data(), (unsigned)tbs.size());
   }
   if (rc != LH_OK) {
+    reject_slot(sv, *sl);
     return rc;
   }
   lh_ctx* ctx = sv->ctx;
@@ -2209,6 +2257,10 @@ int lh_server_step(lh_server* sv)
   // One upload carries the TB descriptors (at the start of the staging slot) and the
   // decoder's descriptor block behind them, ahead of the decode launch.
   slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    reject_slot(sv, *sl);
+    return LH_ERR_HIP;
+  }
   const size_t tb_desc_bytes = align256(tbs.size() * sizeof(lh_tb_desc));
   std::memcpy(s->host, tbs.data(), tbs.size() * sizeof(lh_tb_desc));
   uint8_t* arena_bytes = reinterpret_cast<uint8_t*>(sv->d_arena);
@@ -2223,17 +2275,11 @@ int lh_server_step(lh_server* sv)
     if (e == hipSuccess) e = hipEventRecord(sl->done, sl->stream);
   }
   if (rc != LH_OK || e != hipSuccess) {
-    // Nothing of this slot will complete: report its requests as rejected.
-    for (auto& r : sl->reqs) {
-      auto h = sv->harq.find(r.req.harq_process);
-      if (h != sv->harq.end()) {
-        h->second.busy = false;
-      }
-      sv->tb_dest.erase(r.req.id);
-      push_result(sv, r, LH_TB_REJECTED, 0, 0);
+    if (rc == LH_OK) {
+      rc = hip_fail(e, "server slot launch");
     }
-    sl->reqs.clear();
-    return rc != LH_OK ? rc : hip_fail(e, "server slot launch");
+    reject_slot(sv, *sl);
+    return rc;
   }
   sl->busy = true;
   return LH_OK;
@@ -2295,6 +2341,32 @@ int lh_server_harq_release(lh_server* sv, uint32_t harq_process)
   }
   sv->arena.release(h->second.off, h->second.bytes);
   sv->harq.erase(h);
+  return LH_OK;
+}
+
+int lh_server_run_slot(lh_server* sv, uint32_t slot)
+{
+  if (sv == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid server");
+  }
+  sv->cur_slot = slot;
+  sv->have_slot = true;
+  if (sv->cfg.expire_timeout_slots == 0) {
+    return LH_OK;
+  }
+  for (auto it = sv->harq.begin(); it != sv->harq.end();) {
+    harq_entry& h = it->second;
+    if (h.busy) {
+      h.expiry = slot + sv->cfg.expire_timeout_slots; // in use: renewed (rx_buffer_pool_impl.cpp:100-105)
+      ++it;
+    } else if ((int32_t)(slot - h.expiry) >= 0) {
+      sv->arena.release(h.off, h.bytes);
+      it = sv->harq.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  admit(sv); // memory freed by expiry admits waiting requests
   return LH_OK;
 }
 

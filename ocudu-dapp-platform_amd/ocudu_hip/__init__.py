@@ -205,14 +205,15 @@ class GenTbDesc(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("harq_budget_bytes", C.c_size_t), ("max_cbs_per_slot", C.c_uint), ("max_slots_in_flight", C.c_uint),
-                ("nof_ldpc_iterations", C.c_uint), ("nof_fallback_iterations", C.c_uint), ("learned", C.c_int)]
+                ("nof_ldpc_iterations", C.c_uint), ("nof_fallback_iterations", C.c_uint), ("learned", C.c_int),
+                ("expire_timeout_slots", C.c_uint)]
 
 
 class TbRequest(C.Structure):
     _fields_ = [("id", C.c_uint64), ("d_llrs", C.c_void_p), ("harq_process", C.c_uint32), ("new_data", C.c_int32),
                 ("base_graph", C.c_uint32), ("tbs_bits", C.c_uint32), ("modulation_order", C.c_uint32),
                 ("nof_layers", C.c_uint32), ("nof_ch_symbols", C.c_uint32), ("rv", C.c_uint32), ("Nref", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("slot", C.c_uint32)]
 
 
 class TbResult(C.Structure):
@@ -227,6 +228,11 @@ class ServerStats(C.Structure):
 
 
 LH_TB_CRC_OK, LH_TB_CB_FAILED, LH_TB_CRC_FAILED, LH_TB_REJECTED = 1, 0, -1, -2
+
+# The 51 NR lifting sizes (TS38.212 Table 5.3.2-1).
+LIFTING_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40,
+                 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240,
+                 256, 288, 320, 352, 384]
 
 
 class Segmentation(C.Structure):
@@ -247,7 +253,7 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
             "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch",
-            "lh_pusch_decode_batch_noalloc"]
+            "lh_pusch_decode_batch_noalloc", "lh_server_run_slot"]
 
 
 def lib():
@@ -302,6 +308,7 @@ def lib():
         L.lh_server_poll.argtypes = [C.c_void_p, C.POINTER(TbResult), C.c_uint, C.POINTER(C.c_uint), C.c_int]
         L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.lh_server_run_slot.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_pusch_demodulate.argtypes = [C.c_void_p, C.POINTER(DemodDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]
         L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
@@ -476,9 +483,9 @@ class Server:
     """The slot-deadline serving runtime (lh_server_*)."""
 
     def __init__(self, ctx, harq_budget_bytes, max_cbs_per_slot=1024, max_slots_in_flight=2, nof_ldpc_iterations=8,
-                 learned=False, nof_fallback_iterations=8):
+                 learned=False, nof_fallback_iterations=8, expire_timeout_slots=0):
         cfg = ServerConfig(harq_budget_bytes, max_cbs_per_slot, max_slots_in_flight, nof_ldpc_iterations,
-                           nof_fallback_iterations if learned else 0, int(learned))
+                           nof_fallback_iterations if learned else 0, int(learned), expire_timeout_slots)
         h = C.c_void_p()
         _lh(lib().lh_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "lh_server_create")
         self.handle = h.value
@@ -501,6 +508,9 @@ class Server:
 
     def harq_release(self, process):
         _lh(lib().lh_server_harq_release(self.handle, process), "lh_server_harq_release")
+
+    def run_slot(self, slot):
+        _lh(lib().lh_server_run_slot(self.handle, slot), "lh_server_run_slot")
 
     def stats(self):
         st = ServerStats()

@@ -19,7 +19,7 @@ from pathlib import Path
 import numpy as np
 
 from . import Context
-from .datagen import LIFTING_SIZES
+from . import LIFTING_SIZES
 from .nms import DeviceBatch, rm_length
 
 MAX_THREADS = 384

@@ -38,6 +38,8 @@ TBS_TABLE = [24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 104, 112, 120, 128, 136, 14
              2664, 2728, 2792, 2856, 2976, 3104, 3240, 3368, 3496, 3624, 3752, 3824]
 RE_PER_PRB = 144
 HARQ_RTT_SLOTS = 4
+# HARQ process lifetime after its last transmission (the reference's PUSCH tests use 10 slots).
+HARQ_EXPIRE_SLOTS = 10
 # Maximum LDPC iterations of the PUSCH decoder (the reference's default).
 PUSCH_LDPC_ITERATIONS = 6
 
@@ -145,19 +147,21 @@ class Trace:
         d_msg.free()
         self.tb_host = self.d_tb.download(np.uint8, max(1, tb_off))
 
-    def request(self, i, tx):
+    def request(self, i, tx, slot=None):
+        """Transmission tx of TB i, sent in `slot` (default: the TB's own slot)."""
         t = self.tbs[i]
         return TbRequest(id=2 * i + tx, d_llrs=self.d_llr.ptr + t["llr_off"][tx], harq_process=i, new_data=int(tx == 0),
                          base_graph=t["bg"], tbs_bits=t["tbs"], modulation_order=t["qm"], nof_layers=t["layers"],
-                         nof_ch_symbols=t["nsym"], rv=2 * tx, Nref=0)
+                         nof_ch_symbols=t["nsym"], rv=2 * tx, Nref=0, slot=t["slot"] if slot is None else slot)
 
 
 def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs_per_slot=4096, slots_in_flight=3,
-          iters=PUSCH_LDPC_ITERATIONS, realtime=True, log=None, timeout_s=600.0):
+          iters=PUSCH_LDPC_ITERATIONS, realtime=True, log=None, timeout_s=600.0, expire_slots=HARQ_EXPIRE_SLOTS):
     """Runs the trace through a server: returns per-transmission results and per-slot
-    latencies.  realtime: slots are released every slot_us; else as fast as possible."""
+    latencies.  realtime: slots are released every slot_us; else as fast as possible.
+    HARQ processes expire expire_slots after their last transmission's slot (0: never)."""
     srv = Server(ctx, harq_budget, max_cbs_per_slot, slots_in_flight, iters, learned=learned,
-                 nof_fallback_iterations=iters)
+                 nof_fallback_iterations=iters, expire_timeout_slots=expire_slots)
     by_slot = {}
     for i, t in enumerate(trace.tbs):
         by_slot.setdefault(t["slot"], []).append(i)
@@ -213,7 +217,7 @@ def _serve_loop(srv, trace, prepared, by_slot, out_bufs, tx_slot, slot_us, realt
             if due_slot < last_slot:
                 # The retransmission's request and TB buffer are made now, not at its slot.
                 out_bufs[2 * i + 1] = np.zeros(trace.tbs[i]["tbs"] // 8, np.uint8)
-                retx.setdefault(due_slot, []).append((i, trace.request(i, 1)))
+                retx.setdefault(due_slot, []).append((i, trace.request(i, 1, slot=due_slot)))
             else:
                 srv.harq_release(i)  # past the run: no retransmission
         elif tx == 1 and r.tb_crc_ok != LH_TB_CRC_OK:
@@ -237,6 +241,7 @@ def _serve_loop(srv, trace, prepared, by_slot, out_bufs, tx_slot, slot_us, realt
             srv.close()
             raise RuntimeError(f"serving loop exceeded {timeout_s} s at slot {k}")
         if k < last_slot and ((realtime and now >= t0 + k * slot_us * 1e-6) or (not realtime and due)):
+            srv.run_slot(k)  # HARQ expiry at the slot boundary
             new = prepared.pop(k, None)
             again = retx.pop(k, [])
             if new is not None or again:

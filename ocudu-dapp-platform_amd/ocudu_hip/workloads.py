@@ -7,7 +7,7 @@ Es/N0 where the plain decoder's BLER is about 10% for that UE's configuration.
 """
 import numpy as np
 
-from .datagen import LIFTING_SIZES
+from . import LIFTING_SIZES
 from .nms import MixedBatch, bler, rm_length
 
 QMS = (2, 4, 6, 8)

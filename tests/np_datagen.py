@@ -1,4 +1,4 @@
-"""Vectorised synthetic-codeblock generator (numpy): CRC attach, LDPC encoding, rate
+"""Test infrastructure: a vectorised synthetic-codeblock generator (numpy): CRC attach, LDPC encoding, rate
 matching, QAM mapping, AWGN and max-log soft demapping to int8 LLRs.
 
 Follows the reference's transmit/receive conventions so that its decoder sees the LLRs
@@ -18,7 +18,7 @@ from pathlib import Path
 
 import numpy as np
 
-TABLES = Path(__file__).resolve().parent.parent / "csrc" / "bg_tables.inc"
+TABLES = Path(__file__).resolve().parent.parent / "ocudu-dapp-platform_amd" / "csrc" / "bg_tables.inc"
 LIFTING_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40,
                  44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240,
                  256, 288, 320, 352, 384]

@@ -40,7 +40,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(H.Segmentation) == 32
     assert ctypes.sizeof(H.TbRequest) == 8 + 8 + 10 * 4
     assert ctypes.sizeof(H.TbResult) == 8 + 6 * 4 + 8
-    assert ctypes.sizeof(H.ServerConfig) == 8 + 5 * 4 + 4
+    assert ctypes.sizeof(H.ServerConfig) == 8 + 4 * 4 + 4 + 4
     assert ctypes.sizeof(H.TrainDesc) == 2 * 8 + 4 * 4
 
 

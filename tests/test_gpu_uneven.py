@@ -1,7 +1,10 @@
 """Uneven slot batch in one persistent launch: 16 UEs x 16 codeblocks of random BG, Z,
 code rate and modulation (some with E > N, the wrap/combine branch) decode to the same
 results and bits in the mix as each UE alone, with default and with autotuned launch
-shapes; a sample of codeblocks matches the oracle bit-exactly."""
+shapes; every codeblock of the mix (result, hard bits, HARQ buffer) matches the oracle
+bit-exactly."""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -53,13 +56,22 @@ def test_uneven_mix_equals_alone(ctx):
     assert np.array_equal(res_t, res_mix) and np.array_equal(bits_t, bits_mix)
     for bg, Z, *_ in ues:
         ctx.tune_set(bg, Z, 0, 0)
-    # Oracle sample.
+    # Every codeblock against the oracle: HARQ buffer, result, hard bits.
     cw = mix.d_cw.download(np.int8, mix.d_cw.nbytes)
-    for i in range(0, len(cbs), 37):
+    rm_dev = mix.d_rm.download(np.int8, mix.d_rm.nbytes)
+
+    def oracle(i):
         bg, Z, E, qm, _ = cbs[i]
         d = mix.pusch[i]
         rm = O.rate_dematch(cw[d.rm.cw_offset:d.rm.cw_offset + E], bg, Z, Qm=qm)
-        r, b = O.decode(rm, bg, Z, crc_poly=d.crc_poly, crc_bits=d.nof_crc_bits, max_iter=8)
+        return rm, O.decode(rm, bg, Z, crc_poly=d.crc_poly, crc_bits=d.nof_crc_bits, max_iter=8)
+
+    with ThreadPoolExecutor(8) as ex:
+        want = list(ex.map(oracle, range(len(cbs))))
+    for i, (rm, (r, b)) in enumerate(want):
+        d = mix.pusch[i]
+        assert np.array_equal(rm_dev[d.rm.rm_offset:d.rm.rm_offset + rm.size], rm), f"cb {i}: HARQ buffer differs"
         assert res_mix[i] == r, f"cb {i}: {res_mix[i]} vs oracle {r}"
         o = mix.msg_off[i]
         assert np.array_equal(bits_mix[o:o + b.size], b), f"cb {i}: bits differ from the oracle"
+    assert 0 < (res_mix > 0).sum() < len(cbs)

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from ocudu_hip import datagen as D
+import np_datagen as D
 
 # TS38.212 Table 5.3.2-2/-3 structure
 NNZ = {1: 316, 2: 197}

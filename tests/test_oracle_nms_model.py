@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from ocudu_hip import datagen as D
+import np_datagen as D
 
 
 def _np_forward(llrs, msg, bg, Z, L, alpha, beta, w, tau, F=0):

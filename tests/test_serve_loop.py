@@ -21,12 +21,14 @@ class FakeServer:
         self.ready, self.flight, self.log, self.released = [], [], [], []
         self.submitted = {}
         self.step_no = 0
+        self.slots = []
 
     def submit_batch(self, reqs, outs):
         assert len(reqs) == len(outs)
         for q, o in zip(reqs, outs):
             assert o, "a transmission without an output buffer"
             assert q.id not in self.submitted, "transmission submitted twice"
+            assert q.slot == self.slots[-1], "a request carries the slot it is sent in"
             self.submitted[q.id] = self.step_no
             self.ready.append(q)
 
@@ -50,6 +52,10 @@ class FakeServer:
     def harq_release(self, i):
         self.released.append(i)
 
+    def run_slot(self, k):
+        assert not self.slots or k > self.slots[-1], "slots run out of order"
+        self.slots.append(k)
+
     def stats(self):
         return types.SimpleNamespace(slots_in_flight=len(self.flight), ready=len(self.ready))
 
@@ -61,9 +67,10 @@ class FakeTrace:
     def __init__(self, n, per_slot=3):
         self.tbs = [dict(slot=i // per_slot, tbs=64) for i in range(n)]
 
-    def request(self, i, tx):
+    def request(self, i, tx, slot=None):
         return TbRequest(id=2 * i + tx, d_llrs=None, harq_process=i, new_data=int(tx == 0), base_graph=2,
-                         tbs_bits=64, modulation_order=2, nof_layers=1, nof_ch_symbols=576, rv=2 * tx, Nref=0)
+                         tbs_bits=64, modulation_order=2, nof_layers=1, nof_ch_symbols=576, rv=2 * tx, Nref=0,
+                         slot=self.tbs[i]["slot"] if slot is None else slot)
 
 
 @pytest.mark.parametrize("n", [1, 7, 30])
