@@ -8,14 +8,26 @@
 
 #include "ocudu_mirror.h"
 
+#include <cstdint>
 #include <string>
+#include <vector>
 
 namespace ocudu {
+
+// A learned min-sum normalisation table (lh_set_scale_table): iters x layers entries.
+struct hip_learned_table {
+  std::vector<uint16_t> alpha_q;
+  std::vector<int16_t> beta_q;
+  unsigned iters = 0;
+};
 
 struct hip_backend_config {
   int device = 0;
   // Capacity of one batch (pusch_decoder_capacity codeblocks).
   unsigned max_codeblocks = 512;
+  // Tables of the "learned" decoder algorithm (BG1, BG2); empty: that base graph decodes
+  // with the plain decoder.
+  hip_learned_table learned[2];
 };
 
 std::shared_ptr<ldpc_decoder_factory>

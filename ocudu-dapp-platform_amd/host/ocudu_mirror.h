@@ -1,21 +1,34 @@
-// Restatement of the reference interfaces that the HIP backend plugs into, so the
-// adapters compile stand-alone. Names, signatures and semantics follow the reference:
-//   span / bit_buffer          include/ocudu/adt/span.h, include/ocudu/adt/bit_buffer.h:98-221 (MSB-first)
-//   log_likelihood_ratio       include/ocudu/phy/upper/log_likelihood_ratio.h:26-236
+// The reference interfaces that the HIP backend plugs into.  Inside the reference tree
+// (OCUDU_HIP_REFERENCE_HEADERS defined, the reference's include/ on the include path) they
+// are the reference's own headers:
+//   span / bit_buffer          include/ocudu/adt/span.h, include/ocudu/adt/bit_buffer.h
+//   log_likelihood_ratio       include/ocudu/phy/upper/log_likelihood_ratio.h
 //   crc_calculator             include/ocudu/phy/upper/channel_coding/crc_calculator.h:17-81
 //   codeblock_metadata         include/ocudu/phy/upper/codeblock_metadata.h:24-144
 //   ldpc_decoder               include/ocudu/phy/upper/channel_coding/ldpc/ldpc_decoder.h:19-59
 //   ldpc_rate_dematcher        include/ocudu/phy/upper/channel_coding/ldpc/ldpc_rate_dematcher.h:17-40
 //   *_factory                  include/ocudu/phy/upper/channel_coding/channel_coding_factories.h:26-100
-//   pusch_resident_codeblock_decoder (batch subset)
-//                              include/ocudu/phy/upper/channel_processors/pusch/pusch_resident_codeblock_decoder.h:62-149
-// In the reference tree the adapters compile against the real headers instead.
+//   pusch_resident_codeblock_decoder
+//                              include/ocudu/phy/upper/channel_processors/pusch/pusch_resident_codeblock_decoder.h:20-149
+// Without it, the restatement below (same names, signatures and semantics) lets the
+// adapters and their chain test build stand-alone.
 #pragma once
+
+#ifdef OCUDU_HIP_REFERENCE_HEADERS
+
+#include "ocudu/phy/upper/channel_coding/channel_coding_factories.h"
+#include "ocudu/phy/upper/channel_coding/crc_calculator.h"
+#include "ocudu/phy/upper/channel_coding/ldpc/ldpc_decoder.h"
+#include "ocudu/phy/upper/channel_coding/ldpc/ldpc_rate_dematcher.h"
+#include "ocudu/phy/upper/channel_processors/pusch/pusch_resident_codeblock_decoder.h"
+
+#else
 
 #include <cstddef>
 #include <cstdint>
 #include <memory>
 #include <optional>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -99,11 +112,17 @@ enum class crc_generator_poly : uint32_t {
   CRC6 = 0x61,
 };
 
+using crc_calculator_checksum_t = unsigned;
+
 class crc_calculator
 {
 public:
   virtual ~crc_calculator() = default;
-  virtual uint32_t calculate(const bit_buffer& data) const = 0;
+  // 8 packed bits per byte, MSB first.
+  virtual crc_calculator_checksum_t calculate_byte(span<const uint8_t> data) const = 0;
+  // One bit per byte.
+  virtual crc_calculator_checksum_t calculate_bit(span<const uint8_t> data) const = 0;
+  virtual crc_calculator_checksum_t calculate(const bit_buffer& data) const = 0;
   virtual crc_generator_poly get_generator_poly() const = 0;
 };
 
@@ -185,13 +204,45 @@ public:
   virtual std::unique_ptr<ldpc_rate_dematcher> create() = 0;
 };
 
-// Batch subset of pusch_resident_codeblock_decoder (decode_batch, rate_dematch_only,
-// export_resident_softbits).
+struct pusch_resident_codeblock_decoder_timing_stats {
+  float h2d_transfer_us = 0.0f;
+  float deinterleave_us = 0.0f;
+  float rate_dematch_us = 0.0f;
+  float ldpc_decode_us = 0.0f;
+  float crc_check_us = 0.0f;
+  float d2h_transfer_us = 0.0f;
+  float decoder_config_us = 0.0f;
+  float rate_match_config_us = 0.0f;
+  float completion_wait_us = 0.0f;
+  float tb_output_copy_us = 0.0f;
+  float avg_iters_query_us = 0.0f;
+  float cpu_crc_us = 0.0f;
+  float total_backend_us = 0.0f;
+  float total_e2e_us = 0.0f;
+  unsigned nof_cbs = 0;
+  bool timing_enabled = false;
+  float extract_bits_us = 0.0f;
+  float grid_staging_us = 0.0f;
+  float demod_sync_us = 0.0f;
+};
+
 class pusch_resident_codeblock_decoder
 {
 public:
   struct codeblock_result {
     bool crc_ok = false;
+    unsigned nof_iterations = 0;
+  };
+  struct scrambling_config {
+    uint16_t n_rnti = 0;
+    uint16_t n_id = 0;
+    uint8_t q = 0;
+    uint8_t n_s = 0;
+    unsigned bit_offset = 0;
+  };
+  struct tb_decode_result {
+    bool tb_crc_ok = false;
+    bool all_cb_crcs_ok = false;
     unsigned nof_iterations = 0;
   };
   virtual ~pusch_resident_codeblock_decoder() = default;
@@ -207,6 +258,22 @@ public:
                                  bool                             new_data,
                                  const codeblock_metadata&        metadata) = 0;
   virtual bool export_resident_softbits(span<span<log_likelihood_ratio>> rm_buffers) = 0;
+  virtual tb_decode_result decode_resident_softbits(void*                                                resident_llrs,
+                                                    size_t                                               nof_softbits,
+                                                    const scrambling_config*                             scrambling_cfg,
+                                                    const codeblock_metadata::tb_common_metadata&        tb_common,
+                                                    span<const codeblock_metadata::cb_specific_metadata> cb_specific,
+                                                    span<uint8_t>                                        tb_output,
+                                                    crc_generator_poly                                   cb_crc_poly,
+                                                    unsigned nof_ldpc_iterations,
+                                                    void*    execution_context,
+                                                    int      buffer_index = -1) = 0;
+  virtual const pusch_resident_codeblock_decoder_timing_stats& get_last_timing_stats() const = 0;
+  virtual void set_demod_timing(float grid_staging_us, float demod_sync_us) = 0;
+  virtual void set_timing_enabled(bool enable) = 0;
+  virtual void set_ldpc_decoder_algorithm(const std::string& algorithm) = 0;
 };
 
 } // namespace ocudu
+
+#endif // OCUDU_HIP_REFERENCE_HEADERS

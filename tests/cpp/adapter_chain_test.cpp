@@ -4,14 +4,22 @@
 // encode, rate match, map to LLRs (noise-free +-10 and AWGN), then
 //   * ldpc_rate_dematcher "hip"  == oracle dematcher (new data and HARQ combine),
 //   * ldpc_decoder "hip"         == oracle decoder (bits, iterations/nullopt),
-//   * pusch batch decode_batch   == oracle per codeblock, HARQ buffers written back.
+//   * pusch batch decode_batch   == oracle per codeblock, HARQ buffers written back;
+//   * decode_resident_softbits   == oracle TB chain (TX segmentation statement, RX
+//     segmentation, dematch, decode, TB CRC) for C = 1 and C > 1 TBs, clean and noisy;
+//     export_resident_softbits == oracle dematched buffers, once per decode; invalid
+//     grants rejected without touching the output; timing statistics; the algorithm
+//     selection ("learned" without tables decodes as the plain decoder).
 // The oracle (oracle/liboracle.so) is the checker.  Exit code 0 = pass.
 #include "../../oracle/ldpc_oracle.h"
 #include "hip_ldpc_adapters.h"
 
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <random>
 #include <vector>
 
@@ -35,7 +43,15 @@ class oracle_crc : public crc_calculator
 {
 public:
   explicit oracle_crc(crc_generator_poly p) : poly(p) {}
-  uint32_t calculate(const bit_buffer& d) const override
+  crc_calculator_checksum_t calculate_byte(span<const uint8_t> d) const override
+  {
+    return orc_crc_packed(static_cast<uint32_t>(poly), d.data(), static_cast<unsigned>(8 * d.size()));
+  }
+  crc_calculator_checksum_t calculate_bit(span<const uint8_t> d) const override
+  {
+    return orc_crc_unpacked(static_cast<uint32_t>(poly), d.data(), static_cast<unsigned>(d.size()));
+  }
+  crc_calculator_checksum_t calculate(const bit_buffer& d) const override
   {
     return orc_crc_packed(static_cast<uint32_t>(poly), d.get_buffer().data(), static_cast<unsigned>(d.size()));
   }
@@ -87,6 +103,128 @@ modulation_scheme mod_of(unsigned qm)
 {
   return qm == 2 ? modulation_scheme::QPSK
                  : qm == 4 ? modulation_scheme::QAM16 : qm == 6 ? modulation_scheme::QAM64 : modulation_scheme::QAM256;
+}
+
+void append_crc(std::vector<uint8_t>& bits, uint32_t poly, unsigned order)
+{
+  const uint32_t r = orc_crc_unpacked(poly, bits.data(), static_cast<unsigned>(bits.size()));
+  for (unsigned i = 0; i != order; ++i) {
+    bits.push_back((r >> (order - 1 - i)) & 1);
+  }
+}
+
+// One transport block through the resident path against the oracle TB chain.
+void resident_case(pusch_resident_codeblock_decoder& batch, int bg, unsigned tbs, unsigned Qm, unsigned layers,
+                   unsigned nsym, double snr, std::mt19937_64& rng, bool expect_pass)
+{
+  orc_segmentation seg;
+  if (orc_segment_rx(&seg, bg, tbs, nsym, Qm, layers) != 0) {
+    EXPECT(false, "segmentation of TB %u", tbs);
+    return;
+  }
+  const unsigned C = seg.nof_segments, Z = seg.Z, K = seg.segment_length, F = seg.nof_filler_bits;
+  const unsigned N = (bg == 1 ? 66 : 50) * Z, M = bg == 1 ? 46 : 42;
+  // TX segmentation statement (TS38.212 5.1-5.2): TB + TB CRC, split, CB CRC24B, fillers.
+  std::vector<uint8_t> tb_bits(tbs);
+  for (auto& b : tb_bits) {
+    b = rng() & 1;
+  }
+  std::vector<uint8_t> b = tb_bits;
+  append_crc(b, tbs > 3824 ? ORC_CRC24A : ORC_CRC16, tbs > 3824 ? 24 : 16);
+  std::vector<std::vector<uint8_t>> msgs(C);
+  const unsigned D = K - F - (C > 1 ? 24 : 0);
+  b.resize(static_cast<size_t>(C) * D, 0);
+  for (unsigned c = 0; c != C; ++c) {
+    msgs[c].assign(b.begin() + static_cast<size_t>(c) * D, b.begin() + static_cast<size_t>(c + 1) * D);
+    if (C > 1) {
+      append_crc(msgs[c], ORC_CRC24B, 24);
+    }
+    msgs[c].resize(K, 0);
+  }
+  size_t total = 0;
+  for (unsigned c = 0; c != C; ++c) {
+    total += seg.rm_length[c];
+  }
+  std::vector<int8_t> llr(total);
+  std::vector<codeblock_metadata::cb_specific_metadata> cbs(C);
+  std::vector<std::vector<int8_t>> want_rm(C);
+  const uint32_t cb_poly = C > 1 ? ORC_CRC24B : (tbs > 3824 ? ORC_CRC24A : ORC_CRC16);
+  const unsigned cb_crc = C > 1 ? 24 : (tbs > 3824 ? 24 : 16);
+  unsigned passed = 0;
+  for (unsigned c = 0; c != C; ++c) {
+    std::vector<uint8_t> cw((K / Z + M) * Z), e(seg.rm_length[c]);
+    orc_ldpc_encode(bg, Z, msgs[c].data(), cw.data());
+    orc_rate_match(e.data(), seg.rm_length[c], cw.data() + 2 * Z, N, bg, Z, 0, Qm, 0, F);
+    orc_modulate_awgn_demap(llr.data() + seg.cw_offset[c], e.data(), seg.rm_length[c], Qm, std::pow(10.0, -snr / 10.0),
+                            rng());
+    cbs[c].full_length = N;
+    cbs[c].rm_length = seg.rm_length[c];
+    cbs[c].nof_filler_bits = F;
+    cbs[c].cw_offset = seg.cw_offset[c];
+    cbs[c].nof_crc_bits = cb_crc;
+    orc_rm_cfg rc{0, Qm, 0, F};
+    want_rm[c].assign(N, 0);
+    orc_rate_dematch(want_rm[c].data(), N, llr.data() + seg.cw_offset[c], seg.rm_length[c], 1, &rc);
+    orc_dec_cfg oc{bg, Z, F, cb_crc, 6};
+    std::vector<uint8_t> bits((K + 7) / 8);
+    passed += orc_ldpc_decode(&oc, 0, 0, cb_poly, want_rm[c].data(), N, bits.data()) > 0 ? 1 : 0;
+  }
+  codeblock_metadata::tb_common_metadata common;
+  common.base_graph = bg == 1 ? ldpc_base_graph_type::BG1 : ldpc_base_graph_type::BG2;
+  common.lifting_size = static_cast<ldpc::lifting_size_t>(Z);
+  common.rv = 0;
+  common.mod = mod_of(Qm);
+  common.Nref = 0;
+  common.cw_length = static_cast<unsigned>(total);
+  void* d_llr = nullptr;
+  if (hipMalloc(&d_llr, total) != hipSuccess || hipMemcpy(d_llr, llr.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+    EXPECT(false, "TB %u: device LLR buffer", tbs);
+    return;
+  }
+  std::vector<uint8_t> out(tbs / 8, 0xA5);
+  const auto poly = static_cast<crc_generator_poly>(cb_poly);
+  // Rejections first: nothing is touched and nothing can be exported.
+  auto rej = batch.decode_resident_softbits(d_llr, total - 1, nullptr, common, cbs, out, poly, 6, nullptr);
+  pusch_resident_codeblock_decoder::scrambling_config scr;
+  auto rej2 = batch.decode_resident_softbits(d_llr, total, &scr, common, cbs, out, poly, 6, nullptr);
+  auto rej3 = batch.decode_resident_softbits(nullptr, total, nullptr, common, cbs, out, poly, 6, nullptr);
+  EXPECT(!rej.tb_crc_ok && !rej2.tb_crc_ok && !rej3.tb_crc_ok && rej.nof_iterations == 0, "TB %u: rejection", tbs);
+  EXPECT(std::all_of(out.begin(), out.end(), [](uint8_t x) { return x == 0xA5; }), "TB %u: rejected output touched", tbs);
+  batch.set_timing_enabled(true);
+  auto r = batch.decode_resident_softbits(d_llr, total, nullptr, common, cbs, out, poly, 6, nullptr);
+  const auto& st = batch.get_last_timing_stats();
+  EXPECT(st.timing_enabled && st.nof_cbs == C && st.ldpc_decode_us > 0.0f && st.total_e2e_us >= st.ldpc_decode_us,
+         "TB %u: timing statistics", tbs);
+  batch.set_timing_enabled(false);
+  // Oracle verdict: all codeblocks pass and the TB CRC holds (the data is what was sent).
+  const bool want_ok = passed == C;
+  EXPECT(r.all_cb_crcs_ok == want_ok && r.tb_crc_ok == want_ok, "TB %u: verdict %d/%d vs oracle %u of %u CBs", tbs,
+         r.tb_crc_ok, r.all_cb_crcs_ok, passed, C);
+  EXPECT(want_ok == expect_pass, "TB %u: case does not exercise what it should", tbs);
+  if (want_ok) {
+    std::vector<uint8_t> pk(tbs / 8, 0);
+    for (unsigned i = 0; i != tbs; ++i) {
+      pk[i / 8] |= tb_bits[i] << (7 - i % 8);
+    }
+    EXPECT(out == pk, "TB %u: TB bytes", tbs);
+  }
+  // Export: the oracle's dematched buffers, exactly once.
+  std::vector<std::vector<log_likelihood_ratio>> host(C, std::vector<log_likelihood_ratio>(N, log_likelihood_ratio(77)));
+  std::vector<span<log_likelihood_ratio>> spans;
+  for (auto& h : host) {
+    spans.emplace_back(h.data(), h.size());
+  }
+  EXPECT(batch.export_resident_softbits(spans), "TB %u: export", tbs);
+  for (unsigned c = 0; c != C; ++c) {
+    EXPECT(std::memcmp(host[c].data(), want_rm[c].data(), N) == 0, "TB %u: exported soft bits of CB %u", tbs, c);
+  }
+  EXPECT(!batch.export_resident_softbits(spans), "TB %u: second export", tbs);
+  // "learned" without tables: the plain decoder, same verdict.
+  batch.set_ldpc_decoder_algorithm("learned");
+  auto r2 = batch.decode_resident_softbits(d_llr, total, nullptr, common, cbs, out, poly, 6, nullptr);
+  batch.set_ldpc_decoder_algorithm("min_sum");
+  EXPECT(r2.tb_crc_ok == r.tb_crc_ok && r2.nof_iterations == r.nof_iterations, "TB %u: algorithm fallback", tbs);
+  EXPECT(hipFree(d_llr) == hipSuccess, "TB %u: hipFree", tbs);
 }
 
 } // namespace
@@ -232,6 +370,12 @@ int main()
       EXPECT(std::memcmp(harq[k].data(), expect_rm[k].data(), harq[k].size()) == 0, "batch case %zu harq", k);
     }
   }
+  // Resident path: C = 1 (CRC16, CRC24A) and C > 1 TBs, BG1 and BG2, passing and failing.
+  resident_case(*batch, 2, 1000, 2, 1, 1440, 8.0, rng, true);
+  resident_case(*batch, 1, 8192, 2, 1, 4608, 12.0, rng, true);
+  resident_case(*batch, 1, 42016, 4, 2, 17280, 14.0, rng, true);
+  resident_case(*batch, 2, 3368, 4, 2, 2880, 12.0, rng, true);
+  resident_case(*batch, 1, 42016, 4, 2, 17280, 3.0, rng, false);
   std::printf("%zu cases, %d failures\n", cases.size(), failures);
   return failures == 0 ? 0 : 1;
 }
