@@ -13,7 +13,10 @@ inputs there, and both again at the operating point: the lowest Es/N0 on a 0.05 
 where the plain decoder's BLER is <= 10%.  Also reported: one-slot latency, the transmit
 side (LDPC encoder + rate matcher) on the same messages, the step before the path (soft
 demodulation + descrambling of one full 4-layer 256-QAM slot), the uneven batch, and the
-slot-deadline serving loop.
+slot-deadline serving loop.  The roofline object carries the HBM figure (the declared
+bound) and the VALU fraction (the binding resource); the CPU baseline is the reference's
+own AVX-512 decoder on every usable host core, with one-thread AVX-512 and generic figures
+and the CPU model.
 
 Prints one JSON line (see the fields below).  --gpus N with N > 1 exits: the build
 targets a single GPU.
@@ -188,6 +191,18 @@ def main():
     achieved = dec_bytes / (dec_ms * 1e-3) / 1e9
     traffic = measured_traffic()
     run(1, 0)  # the HARQ buffers hold the headline inputs for the CPU baseline
+    # The binding resource: VALU.  Edge updates per launch = iterations executed x Z x the
+    # edges of the layers decoded (four layers, 76 edges, at this rate), at 10 lane
+    # operations each, against the int32 VALU peak (one wave64 instruction per compute
+    # unit and clock: 64 lane operations).
+    res_h = d_res.download(np.int32, n_cb)
+    iters_run = np.where(res_h > 0, res_h, a.max_iter)
+    edge_updates = float(iters_run.sum()) * Z * 76
+    cus, clk_khz = device_attr(63), device_attr(5)
+    valu_peak = cus * 64 * clk_khz * 1e3
+    valu = {"edge_updates_per_launch": int(edge_updates), "lane_ops_per_edge_update": 10,
+            "peak_lane_ops_per_s": valu_peak, "compute_units": cus, "clock_mhz": clk_khz / 1e3,
+            "fraction": round(edge_updates * 10 / (dec_ms * 1e-3) / valu_peak, 4)}
 
     # Slot latency: one slot (152 CBs) per call, launch -> host-visible CRC verdicts.
     slot_arrs = []
@@ -259,6 +274,7 @@ def main():
                    "global_batch": n_cb, "esn0_db": snr_h, "decoder": "plain (bit-exact generic min-sum)"},
         "bler": head["bler"],
         "mean_iters": head["mean_iters"],
+        "info_gbps_at_10pct_bler": op["plain"]["info_gbps"] if op else None,
         "slot_latency_us": {"p50": round(p50, 1), "p99": round(p99, 1), "slots": a.latency_slots,
                             "cbs_per_slot": a.cbs_per_slot, "budget_us": 500},
         "snr_sweep": sweep,
@@ -272,7 +288,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_ldpc_decode (fused dematch + layered min-sum, row-pair mode at this load)",
                      "kernel_ms": round(dec_ms, 4),
-                     "bytes_per_launch": dec_bytes},
+                     "bytes_per_launch": dec_bytes,
+                     "valu_fraction": valu["fraction"], "valu": valu},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -401,10 +418,32 @@ def demod_slot(ctx, stream, ev0, ev1, cpu=True):
             "cpu_reference": ref}
 
 
+def device_attr(attr):
+    """hipDeviceGetAttribute of device 0 (63: compute units, 5: peak clock in kHz)."""
+    import ctypes as C
+
+    import ocudu_hip as H
+    v = C.c_int(0)
+    H._chk(H.hip().hipDeviceGetAttribute(C.byref(v), attr, 0), "hipDeviceGetAttribute")
+    return v.value
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(d_rm, n_cb, N, info_bits, max_iter, cpu_seconds, snr):
-    """The reference decoder (AVX-512 / AVX2, built from its sources into oracle/_ref)
-    on the host cores, over the same dematched int8 soft bits; the plain-C oracle when
-    the reference build is absent."""
+    """The reference decoder, built from its sources into oracle/_ref, on the host over
+    the same dematched int8 soft bits: the best SIMD decoder (AVX-512, else AVX2) with one
+    decoder per thread on every core this process may use (at most 16, the host share of
+    one GPU), then one thread of the same decoder and one thread of the generic decoder.
+    The plain-C oracle when the reference build is absent."""
     import ctypes as C
 
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -413,19 +452,29 @@ def cpu_baseline(d_rm, n_cb, N, info_bits, max_iter, cpu_seconds, snr):
     sample = min(n_cb, 608)
     rm = d_rm.download(np.int8, sample * N).reshape(sample, N)
     R = O.ref()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, avail))
+    model = f"{cpu_model()} ({os.cpu_count()} logical CPUs on the host, {avail} usable by this process)"
     if R is not None:
+        def measure(kind, nthreads, secs_wall):
+            done, ok, its = C.c_ulong(), C.c_ulong(), C.c_ulong()
+            secs = R.ref_decode_throughput(kind, nthreads, O._p(rm, O._i8p), sample, N, 1, 384, 0, max_iter,
+                                           O.CRC24B, secs_wall, C.byref(done), C.byref(ok), C.byref(its))
+            return {"gbps": round(ok.value * info_bits / secs / 1e9, 4),
+                    "us_per_cb_per_core": round(secs * nthreads / max(done.value, 1) * 1e6, 2),
+                    "decodes": done.value, "seconds": round(secs, 2),
+                    "mean_iters": round(its.value / max(done.value, 1), 3)}, secs
+
         kind = R.ref_decoder_kind(2)
-        secs_wall = max(1.0, cpu_seconds / threads)
-        done, ok, its = C.c_ulong(), C.c_ulong(), C.c_ulong()
-        secs = R.ref_decode_throughput(2, threads, O._p(rm, O._i8p), sample, N, 1, 384, 0, max_iter, O.CRC24B,
-                                       secs_wall, C.byref(done), C.byref(ok), C.byref(its))
-        gbps = ok.value * info_bits / secs / 1e9
         name = {0: "generic", 1: "avx2", 2: "avx512"}[kind]
-        return {"value": round(gbps, 3), "unit": "Gbit/s", "cores": threads, "kind": "reference",
-                "sample": f"{done.value} decodes of {sample} distinct CBs (Es/N0 {snr} dB) in {secs:.1f} s by "
-                          f"the reference ldpc_decoder_{name}, one decoder per thread",
-                "us_per_cb_per_core": round(secs * threads / max(done.value, 1) * 1e6, 2)}
+        allc, secs = measure(kind, threads, max(1.0, cpu_seconds / threads))
+        one_simd, _ = measure(kind, 1, cpu_seconds / 4)
+        one_generic, _ = measure(0, 1, cpu_seconds / 4)
+        return {"value": allc["gbps"], "unit": "Gbit/s", "cores": threads, "kind": "reference",
+                "sample": f"{allc['decodes']} decodes of {sample} distinct CBs (Es/N0 {snr} dB) in {secs:.1f} s by "
+                          f"the reference ldpc_decoder_{name}, one decoder per thread on {threads} threads",
+                "cpu_model": model, "decoder": name, "us_per_cb_per_core": allc["us_per_cb_per_core"],
+                "single_thread": {name: one_simd, "generic": one_generic}}
     t0 = time.time()
     done = ok = 0
     while time.time() - t0 < cpu_seconds:
@@ -434,7 +483,7 @@ def cpu_baseline(d_rm, n_cb, N, info_bits, max_iter, cpu_seconds, snr):
         done += 1
     secs = time.time() - t0
     return {"value": round(ok * info_bits / secs / 1e9, 4), "unit": "Gbit/s", "cores": 1, "kind": "port",
-            "sample": f"{done} decodes by the C oracle in {secs:.1f} s"}
+            "sample": f"{done} decodes by the C oracle in {secs:.1f} s", "cpu_model": model}
 
 
 if __name__ == "__main__":

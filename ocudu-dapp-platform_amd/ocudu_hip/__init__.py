@@ -49,6 +49,7 @@ def hip():
         h.hipEventSynchronize.argtypes = [C.c_void_p]
         h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         h.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        h.hipDeviceGetAttribute.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
         h.hipSetDevice.argtypes = [C.c_int]
         h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
         h.hipHostFree.argtypes = [C.c_void_p]
