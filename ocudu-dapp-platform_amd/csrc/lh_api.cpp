@@ -29,6 +29,7 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
                          bool pairs, hipStream_t s);
 int decode_occupancy(int nt, size_t lds, bool learned, bool pairs);
+hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s,
                               const tb_signal* sig = nullptr);
@@ -47,7 +48,6 @@ hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt,
                             const uint32_t* gtab, int backward, hipStream_t s);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, int n_alpha, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t s);
-hipError_t launch_reset_headers(launch_header* hdr, hipStream_t s);
 hipError_t launch_gen_tb(const gen_tb_job* d_jobs, unsigned n, uint8_t* tb, uint8_t* msgs, const uint32_t* crc_x,
                          const uint32_t* lut24a, const uint32_t* pow2, hipStream_t s);
 hipError_t launch_crc(const uint64_t* d_offs, const uint32_t* d_nbits, unsigned n, const uint8_t* bits,
@@ -405,7 +405,9 @@ struct plan_block {
   std::vector<uint8_t> bytes;
   std::vector<dec_item> items, tmp; // work items while packing
   size_t o_items = 0, o_jobs = 0;
+  size_t ctr_words = 0;             // sub-queue counters behind the headers
   size_t n_items[2] = {0, 0};
+  unsigned grid[2] = {0, 0};        // persistent workgroups per row mode
   int nt[2] = {64, 64};
   size_t lds[2] = {0, 0};
 };
@@ -527,21 +529,41 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     }
   }
   items.swap(pb.tmp);
+  pb.n_items[1] = n_pair;
+  pb.n_items[0] = items.size() - n_pair;
+  // Persistent workgroups per row mode: as many as are resident on the device at once (the
+  // plain and the learned instance alike), at most one per item.  Items beyond the first
+  // round form the sub-queues, whose counters follow the headers.
   launch_header hdr[4]{};
-  for (int pass = 0; pass < 2; ++pass) {
-    hdr[2 * pass].n_items = (uint32_t)n_pair;
-    hdr[2 * pass].first_item = 0;
-    hdr[2 * pass + 1].n_items = (uint32_t)(items.size() - n_pair);
-    hdr[2 * pass + 1].first_item = (uint32_t)n_pair;
+  size_t ctr_words = 0;
+  for (int m = 0; m < 2; ++m) {
+    pb.grid[m] = 0;
+    if (pb.n_items[m] == 0) {
+      continue;
+    }
+    const int per_cu = std::max(1, std::min(decode_occupancy(nt[m], lds[m], false, m == 1),
+                                             decode_occupancy(nt[m], lds[m], true, m == 1)));
+    pb.grid[m] = (unsigned)std::min<size_t>(pb.n_items[m], (size_t)ctx->nof_cus * per_cu);
   }
-  pb.o_items = sizeof(hdr);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int m = 1; m >= 0; --m) {
+      launch_header& h = hdr[2 * pass + (1 - m)];
+      h.n_items = (uint32_t)pb.n_items[m];
+      h.first_item = m == 1 ? 0u : (uint32_t)n_pair;
+      if (pb.n_items[m] > pb.grid[m]) {
+        h.nq = std::min<uint32_t>(kMaxSubQueues, pb.grid[m]);
+        h.ctr_off = (uint32_t)((sizeof(hdr) - (size_t)(2 * pass + (1 - m)) * sizeof(launch_header)) / 4 + ctr_words);
+        ctr_words += (size_t)h.nq * kCounterStride;
+      }
+    }
+  }
+  pb.ctr_words = ctr_words;
+  pb.o_items = sizeof(hdr) + 4 * ctr_words;
   pb.o_jobs = pb.o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
   pb.bytes.assign(pb.o_jobs + jobs.size() * sizeof(dec_job), 0);
   std::memcpy(pb.bytes.data(), hdr, sizeof(hdr));
   std::memcpy(pb.bytes.data() + pb.o_items, items.data(), items.size() * sizeof(dec_item));
   std::memcpy(pb.bytes.data() + pb.o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
-  pb.n_items[1] = n_pair;
-  pb.n_items[0] = items.size() - n_pair;
   pb.nt[0] = nt[0];
   pb.nt[1] = nt[1];
   pb.lds[0] = lds[0];
@@ -566,9 +588,7 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
       if (pb.n_items[m] == 0) {
         continue;
       }
-      const int per_cu = std::max(1, decode_occupancy(pb.nt[m], pb.lds[m], tab != nullptr, m == 1));
-      const unsigned grid = (unsigned)std::min<size_t>(pb.n_items[m], (size_t)ctx->nof_cus * per_cu);
-      e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, grid, pb.nt[m], pb.lds[m], rm, cw, bits,
+      e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, pb.grid[m], pb.nt[m], pb.lds[m], rm, cw, bits,
                         results, ctx->d_crc_x, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1, stream);
     }
   }
@@ -621,8 +641,9 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   lh_ctx* c = new lh_ctx();
   c->device = device;
   c->max_cb = max_codeblocks;
-  c->slot_bytes =
-      (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params) + sizeof(dec_item) + sizeof(lh_tb_desc)) + 4096;
+  // Per codeblock its descriptors; per launch the headers and sub-queue counters.
+  c->slot_bytes = (size_t)max_codeblocks * (sizeof(dec_job) + sizeof(rm_params) + sizeof(dec_item) + sizeof(lh_tb_desc)) +
+                  4096 + 4 * kMaxSubQueues * kCounterStride * 4;
   (void)hipDeviceGetAttribute(&c->nof_cus, hipDeviceAttributeMultiprocessorCount, device);
   std::vector<uint32_t> x(3 * (size_t)kCrcXLen);
   crc_x_table(LH_CRC24A, x.data());
@@ -1192,12 +1213,16 @@ int lh_slot_graph_build(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, co
   if (e == hipSuccess) {
     e = hipStreamCreate(&g->cap);
   }
-  // Graph: rewind the work queues, decode (+ fallback), reassemble the TBs.
+  // Graph: rewind the sub-queue counters, decode (+ fallback), reassemble the TBs.
   if (e == hipSuccess) {
     e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeRelaxed);
   }
   if (e == hipSuccess) {
-    hipError_t le = launch_reset_headers(reinterpret_cast<launch_header*>(g->d_block), g->cap);
+    // Every replay starts with zero sub-queue counters (a small kernel, no memset node).
+    hipError_t le = pb.ctr_words == 0
+                        ? hipSuccess
+                        : launch_reset_counters(reinterpret_cast<uint32_t*>(g->d_block + 4 * sizeof(launch_header)),
+                                                (unsigned)pb.ctr_words, g->cap);
     if (le == hipSuccess) {
       le = launch_block(ctx, pb, g->d_block, d_rm, d_cw, d_bits, d_results, g->cap, learned ? ctx->d_scale : nullptr,
                         learned ? nof_fallback_iterations : 0);

@@ -85,18 +85,24 @@ struct alignas(16) dec_item {
 static_assert(sizeof(dec_item) == 16, "dec_item layout");
 
 // Header of the device descriptor block of one decoder launch.
+// Items [0, grid) go to workgroup w = item (the first round, largest items first); the
+// rest are nq interleaved sub-queues (sub-queue q: items grid + q + k nq) that the
+// workgroups w = q (mod nq) claim through counter q, kCounterStride words apart from
+// ctr_off on (words from this header), zero at upload.
+constexpr uint32_t kMaxSubQueues = 16;
+constexpr uint32_t kCounterStride = 32; // one 128-byte line per counter
 struct alignas(16) launch_header {
-  uint32_t next_item;  // work-queue counter, zero at upload
+  uint32_t nq;         // sub-queues of the dynamic part (0: no dynamic part)
   uint32_t n_items;    // items of this queue: [first_item, first_item + n_items)
   uint32_t first_item;
-  uint32_t pad;
+  uint32_t ctr_off;
 };
 
 // Static scratch of the allocation-free decode entry point: one descriptor block (four
 // headers, at most one item and one job per codeblock) of up to kNoAllocMaxCbs codeblocks.
 constexpr unsigned kNoAllocMaxCbs = 4096;
-constexpr size_t kScratchBytes =
-    ((4 * sizeof(launch_header) + kNoAllocMaxCbs * (sizeof(dec_item) + sizeof(dec_job))) + 255) & ~(size_t)255;
+constexpr size_t kScratchBytes = ((4 * sizeof(launch_header) + 4 * kMaxSubQueues * kCounterStride * 4 +
+                                   kNoAllocMaxCbs * (sizeof(dec_item) + sizeof(dec_job))) + 255) & ~(size_t)255;
 
 // One synthetic codeblock of the generator (128 bytes): message bits from seed_msg,
 // noise from seed_noise, rate matching by `rm` (cw_off = LLR output offset).

@@ -590,44 +590,59 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
       tail = max(ql < rp.info ? ql + 1 : ql + F + 1, rp.nsys);
     }
     const int n_main = min(n_in, (int)((tail + 3) & ~3u));
-    for (int p0 = 4 * (int)r; p0 < n_main; p0 += 4 * (int)NT) {
-      // Branch-free: the four gathers go out together (clamped to a valid byte when the
-      // position takes no LLR) and one wait covers them.
-      int x[4];
-      uint32_t kind = 0; // bit k: filler, bit 4+k: LLR
+    const int step = 4 * (int)NT;
+    // Groups of four positions per thread, kGatherBatch groups at a time: their gathers
+    // all go out before the first is used (one memory latency per batch instead of one
+    // per group).  Branch-free: a position that takes no LLR, or a group past n_main,
+    // gathers a valid byte that is not used.
+    constexpr int kGatherBatch = 4;
+    for (int pb = 4 * (int)r; pb < n_main; pb += kGatherBatch * step) {
+      int x[kGatherBatch][4];
+      uint32_t kind[kGatherBatch]; // bit k: filler, bit 4+k: LLR
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t p = (uint32_t)(p0 + k);
-        const uint32_t q = p < rp.info ? p : p - F;
-        const uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start;
-        const bool fill = p - rp.info < F;
-        const bool llr = !fill && p < rp.Ncb && i < rp.E;
-        const uint32_t qt = (uint32_t)((float)i * rp.rcp_kq);
-        const uint32_t at = llr ? (i - qt * rp.Kq) * rp.Qm + qt : 0u;
-        x[k] = src[at];
-        kind |= ((uint32_t)fill << k) | ((uint32_t)llr << (4 + k));
-      }
-      uint32_t outw = 0;
+      for (int b = 0; b < kGatherBatch; ++b) {
+        const int p0 = pb + b * step;
+        kind[b] = 0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int p = p0 + k;
-        const int v = ((kind >> k) & 1u) ? 127 : (((kind >> (4 + k)) & 1u) ? x[k] : 0);
-        if (p < n_in) {
-          outw |= (uint32_t)(uint8_t)v << (8 * k);
-          last = v != 0 ? p : last;
-          if (p < cap) {
-            int j = j0 + k;
-            j = j >= Z ? j - Z : j;
-            soft2[soft_pos<PAIR>(p, j, Z)] = (int8_t)min(max(v, -64), 64);
-          }
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t p = (uint32_t)(p0 + k);
+          const uint32_t q = p < rp.info ? p : p - F;
+          const uint32_t i = q >= rp.start ? q - rp.start : q + rp.P - rp.start;
+          const bool fill = p - rp.info < F;
+          const bool llr = !fill && p < rp.Ncb && i < rp.E && p0 < n_main;
+          const uint32_t qt = (uint32_t)((float)i * rp.rcp_kq);
+          const uint32_t at = llr ? (i - qt * rp.Kq) * rp.Qm + qt : 0u;
+          x[b][k] = src[at];
+          kind[b] |= ((uint32_t)fill << k) | ((uint32_t)llr << (4 + k));
         }
       }
-      if (wr) {
-        if (al4 && p0 + 4 <= n_in) {
-          *reinterpret_cast<uint32_t*>(harq + p0) = outw;
-        } else {
-          for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
-            harq[p0 + k] = (int8_t)(outw >> (8 * k));
+#pragma unroll
+      for (int b = 0; b < kGatherBatch; ++b) {
+        const int p0 = pb + b * step;
+        if (p0 < n_main) {
+          uint32_t outw = 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int p = p0 + k;
+            const int v = ((kind[b] >> k) & 1u) ? 127 : (((kind[b] >> (4 + k)) & 1u) ? x[b][k] : 0);
+            if (p < n_in) {
+              outw |= (uint32_t)(uint8_t)v << (8 * k);
+              last = v != 0 ? p : last;
+              if (p < cap) {
+                int j = j0 + k;
+                j = j >= Z ? j - Z : j;
+                soft2[soft_pos<PAIR>(p, j, Z)] = (int8_t)min(max(v, -64), 64);
+              }
+            }
+          }
+          if (wr) {
+            if (al4 && p0 + 4 <= n_in) {
+              *reinterpret_cast<uint32_t*>(harq + p0) = outw;
+            } else {
+              for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
+                harq[p0 + k] = (int8_t)(outw >> (8 * k));
+              }
+            }
           }
         }
       }
@@ -749,9 +764,9 @@ struct grp_ctl {
 };
 
 constexpr int kCtlBytes = (int)sizeof(grp_ctl);
-// The last 16 bytes of the dynamic LDS hold the work-queue slot of the workgroup: with
-// no static allocation the soft bits start at LDS address 0, so a soft-bit index is the
-// LDS byte address itself (and a ^ 1 its row-pair partner, with no offset to fold).
+// The last 16 bytes of the dynamic LDS hold the workgroup's claimed item: with no static
+// allocation the soft bits start at LDS address 0, so a soft-bit index is the LDS byte
+// address itself (and a ^ 1 its row-pair partner, with no offset to fold).
 constexpr int kQueueBytes = 16;
 
 // Decodes one work item (G codeblocks of one (BG, Z)), behaviour of
@@ -888,21 +903,36 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
     // Row of the first position of a word (the same in every pass: the stride 4 NT is a
     // multiple of Z).
     const int jb = PAIR ? ((4 * (int)r - lead) % Z + Z) % Z : 0;
-    for (int b = (int)r; b < nblk; b += NT) {
-      const uint32_t w = blk[b];
-      int j = jb;
+    // kLoadBatch words per thread in flight together.
+    constexpr int kLoadBatch = 8;
+    for (int bb = (int)r; bb < nblk; bb += kLoadBatch * NT) {
+      uint32_t wv[kLoadBatch];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int p = 4 * b + k - lead;
-        const int x = (int)(int8_t)(w >> (8 * k));
-        if (p >= 0 && p < n_in) {
-          last = x != 0 ? p : last;
-          if (p < cap) {
-            soft[soft_pos<PAIR>(2 * Z + p, j, Z)] = (int8_t)(p < full ? min(max(x, -64), 64) : med3i(x, -121, 121));
+      for (int u = 0; u < kLoadBatch; ++u) {
+        const int b = bb + u * NT;
+        wv[u] = b < nblk ? blk[b] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kLoadBatch; ++u) {
+        const int b = bb + u * NT;
+        if (b < nblk) {
+          const uint32_t w = wv[u];
+          int j = jb;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int p = 4 * b + k - lead;
+            const int x = (int)(int8_t)(w >> (8 * k));
+            if (p >= 0 && p < n_in) {
+              last = x != 0 ? p : last;
+              if (p < cap) {
+                soft[soft_pos<PAIR>(2 * Z + p, j, Z)] =
+                    (int8_t)(p < full ? min(max(x, -64), 64) : med3i(x, -121, 121));
+              }
+            }
+            if (PAIR) {
+              j = j + 1 == Z ? 0 : j + 1;
+            }
           }
-        }
-        if (PAIR) {
-          j = j + 1 == Z ? 0 : j + 1;
         }
       }
     }
@@ -1174,11 +1204,14 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   __syncthreads(); // the next item reuses the LDS
 }
 
-// Persistent decoder: workgroups pull work items from the (BG, Z)-sorted queue until it
-// is empty.  Items were sorted largest first by the host, so the dynamic pull balances
-// uneven codeblocks.
+// Persistent decoder.  The host sorts the items largest first.  Workgroup w takes item w
+// (the first round needs no claim), then claims items of its sub-queue w mod nq through
+// that sub-queue's counter: the claims of a launch spread over nq device-scope counters
+// in separate cache lines instead of queueing on one, and each sub-queue still balances
+// uneven items (and early-stopping codeblocks) dynamically.  The claimed index goes to
+// the workgroup through a word at the end of the dynamic LDS (qslot).
 // One persistent decoder per row mode: the row-pair instance holds more state per thread
-// and runs at three waves per SIMD (168 VGPRs, no spills), the one-row instance at four.
+// and runs at three waves per SIMD (168 VGPRs), the one-row instance at four.
 template <bool LEARNED, bool PAIR>
 __global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(PAIR ? 3 : 4)))
 k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ items, const dec_job* __restrict__ jobs,
@@ -1187,18 +1220,11 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
               const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot)
 {
   extern __shared__ __align__(16) uint8_t lds[];
-  uint32_t& item_id = *reinterpret_cast<uint32_t*>(lds + qslot);
-  const uint32_t n_items = hdr->n_items;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      item_id = atomicAdd(&hdr->next_item, 1u);
-    }
-    __syncthreads();
-    const uint32_t id = item_id;
-    __syncthreads();
-    if (id >= n_items) {
-      break;
-    }
+  uint32_t& claimed = *reinterpret_cast<uint32_t*>(lds + qslot);
+  const uint32_t n_items = hdr->n_items, nq = hdr->nq;
+  const uint32_t W = gridDim.x, w = blockIdx.x, q = nq ? w % nq : 0u;
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(hdr) + hdr->ctr_off + q * kCounterStride;
+  for (uint32_t id = w; id < n_items;) {
     const dec_item item = items[hdr->first_item + id];
     if (item.bg == 1) {
       decode_item<1, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab,
@@ -1207,6 +1233,15 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
       decode_item<2, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
                                     LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
     }
+    if (nq == 0) {
+      break;
+    }
+    if (threadIdx.x == 0) {
+      claimed = atomicAdd(ctr, 1u);
+    }
+    __syncthreads();
+    id = W + q + nq * claimed;
+    __syncthreads();
   }
 }
 
@@ -1429,7 +1464,7 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
   if (e != hipSuccess) {
     return e;
   }
-  // lds (from decode_item_lds_bytes, a multiple of 4) ends with the work-queue slot.
+  // lds (from decode_item_lds_bytes, a multiple of 4) ends with the claimed-item word.
   const unsigned qslot = (unsigned)lds - kQueueBytes;
 #define LH_LAUNCH(L, P)                                                                                           \
   hipLaunchKernelGGL((k_ldpc_decode<L, P>), dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, \
@@ -1451,6 +1486,21 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
   return hipGetLastError();
 }
 
+// Zeroes the sub-queue counters of a resident descriptor block (graph replays):
+// `words` words from `ctr` on.
+__global__ void k_reset_counters(uint32_t* ctr, unsigned words)
+{
+  for (unsigned i = threadIdx.x; i < words; i += blockDim.x) {
+    ctr[i] = 0;
+  }
+}
+
+hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(256), 0, s, ctr, words);
+  return hipGetLastError();
+}
+
 int decode_occupancy(int nt, size_t lds, bool learned, bool pairs)
 {
   int n = 0;
@@ -1462,19 +1512,6 @@ int decode_occupancy(int nt, size_t lds, bool learned, bool pairs)
   return n;
 }
 
-// Rewinds the work-queue counters of a resident descriptor block (graph replays).
-__global__ void k_reset_headers(launch_header* hdr)
-{
-  if (threadIdx.x < 4) {
-    hdr[threadIdx.x].next_item = 0;
-  }
-}
-
-hipError_t launch_reset_headers(launch_header* hdr, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_reset_headers, dim3(1), dim3(64), 0, s, hdr);
-  return hipGetLastError();
-}
 
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s)
