@@ -158,6 +158,7 @@ struct lh_ctx {
   unsigned max_cb = 0;
   size_t slot_bytes = 0;
   uint32_t* d_crc_x = nullptr;
+  uint32_t* d_crc_xz = nullptr; // the decoder's zero-padded CRC weights (kCrcXZLen per polynomial)
   uint32_t* d_gtab = nullptr;
   uint32_t* d_demod = nullptr; // LFSR jump matrices + demapper tables (kDemodTableWords)
   uint32_t* d_tbcrc = nullptr; // CRC24A byte table (256) + x^(2^k) mod G (32)
@@ -589,7 +590,7 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
         continue;
       }
       e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, pb.grid[m], pb.nt[m], pb.lds[m], rm, cw, bits,
-                        results, ctx->d_crc_x, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1, stream);
+                        results, ctx->d_crc_xz, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1, stream);
     }
   }
   return e;
@@ -652,6 +653,21 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   e = hipMalloc(&c->d_crc_x, x.size() * 4);
   if (e == hipSuccess) {
     e = hipMemcpy(c->d_crc_x, x.data(), x.size() * 4, hipMemcpyHostToDevice);
+  }
+  {
+    std::vector<uint32_t> xz(3 * (size_t)kCrcXZLen, 0u);
+    for (int k = 0; k < 3; ++k) {
+      const int order = k == 2 ? 16 : 24;
+      for (int e2 = order; e2 < kCrcXLen; ++e2) {
+        xz[(size_t)k * kCrcXZLen + kCrcPad + e2] = x[(size_t)k * kCrcXLen + e2];
+      }
+    }
+    if (e == hipSuccess) {
+      e = hipMalloc(&c->d_crc_xz, xz.size() * 4);
+    }
+    if (e == hipSuccess) {
+      e = hipMemcpy(c->d_crc_xz, xz.data(), xz.size() * 4, hipMemcpyHostToDevice);
+    }
   }
   // Lifted edge tables: (col * Z) | ((V mod Z) << 16) per edge, rows in order
   // (ldpc_graph_impl.cpp:11-39, ldpc_luts_impl.cpp:4505-4523).
@@ -792,6 +808,9 @@ int lh_ctx_destroy(lh_ctx* c)
   if (c->copy) {
     (void)hipStreamSynchronize(c->copy);
     (void)hipStreamDestroy(c->copy);
+  }
+  if (c->d_crc_xz) {
+    (void)hipFree(c->d_crc_xz);
   }
   if (c->d_crc_x) {
     (void)hipFree(c->d_crc_x);

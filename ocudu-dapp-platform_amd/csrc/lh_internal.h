@@ -9,6 +9,11 @@ constexpr int kMaxZ = 384;
 constexpr int kMaxThreads = 384;
 // x^e mod G tables: e in [0, 22*384 + 24).
 constexpr int kCrcXLen = 22 * 384 + 24;
+// The decoder's CRC weights: per polynomial kCrcPad zeros, then x^e mod G for e in
+// [0, kCrcXLen) with the entries e < order zeroed, so that the weight index of a filler
+// bit (at least order - F > -kCrcPad) reads zero.
+constexpr int kCrcPad = 20 * 384;
+constexpr int kCrcXZLen = kCrcPad + kCrcXLen;
 constexpr uint32_t kMaxRmLength = 22 * 384 * 35; // MAX_CODEBLOCK_RM_SIZE
 constexpr int kNofLiftingSizes = 51;
 // Lifted edge tables of all graphs: BG1 graphs first (316 edges each), then BG2 (197).

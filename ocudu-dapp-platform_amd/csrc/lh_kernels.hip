@@ -1027,7 +1027,7 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   // (index clamped at 0 for bits beyond len, whose weights are masked off).
   const int len = msg - (int)jb.F;
   const int order = jb.crc_idx == 3 ? 16 : 24;
-  const uint32_t* __restrict__ xbase = crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXLen : 0);
+  const uint32_t* __restrict__ xbase = crc_x + (use_crc ? (jb.crc_idx - 1) * kCrcXZLen : 0) + kCrcPad;
   const int xi0 = (len - 1 + order) - (int)r; // weight index of bit r
   __syncthreads();
   const int Litem = ctl->lmax_item;
@@ -1118,34 +1118,34 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
     uint32_t x = 0, z = 0;
     if (check) {
       const bool want_crc = use_crc && (es_crc || last_it);
-      // Check rows of this thread: r, and r + Z/2 with row pairs, eight message columns at
-      // a time (loads in flight without holding all K in registers).  Row rr is byte rr of
-      // a column, or byte 2r + half with row pairs.
+      // Check rows of this thread: r, and r + Z/2 with row pairs; row rr is byte rr of a
+      // column, or byte 2r + half with row pairs.  The weight of bit n Z + rr is entry
+      // xi - n Z of the zero-padded table, zero for filler bits (no bounds test).
 #pragma unroll 1
       for (int half = 0; half < (PAIR ? 2 : 1); ++half) {
         const int rr = (int)r + half * NT;
-        // Opaque base and index: the weight addresses are formed at each check instead of
-        // being hoisted out of the iteration loop into registers.
-        const uint32_t* xt = xbase;
-        int xi = xi0 - half * NT;
-        asm volatile("" : "+v"(xt), "+v"(xi));
+        // Opaque base: the weight addresses are formed at each check instead of being
+        // hoisted out of the iteration loop into registers.
+        const uint32_t* xt = xbase + (xi0 - half * NT);
+        asm volatile("" : "+v"(xt));
         const int8_t* sc = soft0 + goff + (PAIR ? 2 * (int)r + half : rr);
-#pragma unroll 1
-        for (int n0 = 0; n0 < K; n0 += 8) {
-          int sv[8];
-          uint32_t xw[8];
+        uint32_t acc = 0, zz = 0;
+        if (want_crc) {
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const int n = n0 + k;
-            sv[k] = n < K ? sc[n * Z] : 1;
-            xw[k] = (n < K && want_crc && n * Z + rr < len) ? xt[max(xi - n * Z, 0)] : 0u;
+          for (int n = 0; n < K; ++n) {
+            const int sv = sc[n * Z];
+            const uint32_t w = xt[-n * Z];
+            acc ^= sv <= 0 ? w : 0u; // weight of a hard one
+            zz |= (uint32_t)(sv == 0);
           }
+        } else {
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            z |= (sv[k] == 0);
-            x ^= xw[k] & (uint32_t)((sv[k] - 1) >> 31); // weight of a hard one (soft <= 0)
+          for (int n = 0; n < K; ++n) {
+            zz |= (uint32_t)(sc[n * Z] == 0);
           }
         }
+        x ^= acc;
+        z |= zz;
       }
       if (!es_crc) {
         if (PAIR) {
