@@ -83,14 +83,33 @@ const unsigned kLiftingSizes[kNofLiftingSizes] = {
     44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352,
     384};
 
-int lifting_position(unsigned Z)
-{
-  for (int i = 0; i < kNofLiftingSizes; ++i) {
-    if (kLiftingSizes[i] == Z) {
-      return i;
+// Position among the 51 lifting sizes and lifting set index (TS38.212 Table 5.3.2-1) of
+// every Z <= 384, -1 where Z is not a lifting size: O(1) per codeblock on the host path.
+struct lifting_lut {
+  int8_t pos[kMaxZ + 1];
+  int8_t set[kMaxZ + 1];
+  lifting_lut()
+  {
+    for (int z = 0; z <= kMaxZ; ++z) {
+      pos[z] = -1;
+      set[z] = -1;
+    }
+    for (int i = 0; i < kNofLiftingSizes; ++i) {
+      pos[kLiftingSizes[i]] = (int8_t)i;
+    }
+    static const unsigned a_set[8] = {2, 3, 5, 7, 9, 11, 13, 15};
+    for (int i = 0; i < 8; ++i) {
+      for (unsigned v = a_set[i]; v <= (unsigned)kMaxZ; v *= 2) {
+        set[v] = (int8_t)i;
+      }
     }
   }
-  return -1;
+};
+const lifting_lut kLift;
+
+int lifting_position(unsigned Z)
+{
+  return Z <= (unsigned)kMaxZ ? kLift.pos[Z] : -1;
 }
 
 // Offset of the lifted edge table of graph (bg, Z) in the device table.
@@ -102,15 +121,7 @@ uint32_t graph_offset(unsigned bg, unsigned Z)
 
 int lifting_index(unsigned Z)
 {
-  static const unsigned a_set[8] = {2, 3, 5, 7, 9, 11, 13, 15};
-  for (int i = 0; i < 8; ++i) {
-    for (unsigned v = a_set[i]; v <= 384; v *= 2) {
-      if (v == Z) {
-        return i;
-      }
-    }
-  }
-  return -1;
+  return Z <= (unsigned)kMaxZ ? kLift.set[Z] : -1;
 }
 
 // x^e mod G for e in [0, kCrcXLen): the contribution of one message bit to the
@@ -174,6 +185,14 @@ struct lh_ctx {
   // entry point, and the host workspaces that entry point packs its launch in (reserved
   // for kNoAllocMaxCbs codeblocks at creation, so packing does not reallocate).
   uint8_t* d_scratch = nullptr;
+  // Resident workgroups per compute unit of the decoder instances, by launch shape.
+  struct occ_entry {
+    int nt = 0;
+    size_t lds = 0;
+    int key = -1; // learned | pairs << 1
+    int occ = 0;
+  } occ_cache[32];
+  int occ_next = 0;
   std::vector<dec_job> ws_jobs;
   std::vector<uint8_t> ws_bytes;
   std::vector<dec_item> ws_items, ws_tmp;
@@ -407,18 +426,39 @@ struct plan_block {
   std::vector<dec_item> items, tmp; // work items while packing
   size_t o_items = 0, o_jobs = 0;
   size_t ctr_words = 0;             // sub-queue counters behind the headers
+  size_t size = 0;                  // bytes of the packed block
   size_t n_items[2] = {0, 0};
   unsigned grid[2] = {0, 0};        // persistent workgroups per row mode
   int nt[2] = {64, 64};
   size_t lds[2] = {0, 0};
 };
 
+// decode_occupancy, remembered per launch shape (the runtime query costs microseconds).
+int occupancy(lh_ctx* ctx, int nt, size_t lds, bool learned, bool pairs)
+{
+  const int key = (learned ? 1 : 0) | (pairs ? 2 : 0);
+  for (const auto& c : ctx->occ_cache) {
+    if (c.key == key && c.nt == nt && c.lds == lds) {
+      return c.occ;
+    }
+  }
+  auto& c = ctx->occ_cache[ctx->occ_next];
+  ctx->occ_next = (ctx->occ_next + 1) % 32;
+  c.nt = nt;
+  c.lds = lds;
+  c.key = key;
+  c.occ = decode_occupancy(nt, lds, learned, pairs);
+  return c.occ;
+}
+
 // Sorts the jobs largest first (longest-processing-time order for the dynamic queue; equal
 // (BG, Z) stay adjacent so they pack into shared items), decides the row mode of every
 // (BG, Z, layer bound) class, packs items of up to G codeblocks of one class -- G from
 // the tuning table, else as many as fit the workgroup -- and decides LDS staging of the
 // codeword LLRs per item.
-int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
+// The block is written to dst (cap bytes, e.g. a pinned staging slot), or to pb.bytes
+// when dst is null.
+int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = nullptr, size_t cap = 0)
 {
   std::vector<dec_job>& jobs = plan.jobs;
   const size_t n_cbs = jobs.size();
@@ -456,8 +496,9 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
   nt[0] = std::min(nt[0], kMaxThreads);
   nt[1] = std::min(nt[1], kMaxThreads);
   // Total order (ties by result index), so an unstable sort, which needs no temporary
-  // buffer, gives the same queue as a stable one.
-  std::sort(jobs.begin(), jobs.end(), [](const dec_job& a, const dec_job& b) {
+  // buffer, gives the same queue as a stable one; a batch already in that order (a
+  // homogeneous slot) is not sorted again.
+  const auto by_work = [](const dec_job& a, const dec_job& b) {
     const int ka = (int)a.Z * (a.lbound + 22), kb = (int)b.Z * (b.lbound + 22);
     const int bga = a.gt_off < kNofLiftingSizes * 316 ? 1 : 2, bgb = b.gt_off < kNofLiftingSizes * 316 ? 1 : 2;
     if (ka != kb) {
@@ -470,7 +511,10 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
       return a.Z > b.Z;
     }
     return a.out_idx < b.out_idx;
-  });
+  };
+  if (!std::is_sorted(jobs.begin(), jobs.end(), by_work)) {
+    std::sort(jobs.begin(), jobs.end(), by_work);
+  }
   std::vector<dec_item>& items = pb.items;
   items.clear();
   size_t lds[2] = {0, 0};
@@ -542,8 +586,8 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
     if (pb.n_items[m] == 0) {
       continue;
     }
-    const int per_cu = std::max(1, std::min(decode_occupancy(nt[m], lds[m], false, m == 1),
-                                             decode_occupancy(nt[m], lds[m], true, m == 1)));
+    const int per_cu = std::max(1, std::min(occupancy(ctx, nt[m], lds[m], false, m == 1),
+                                             occupancy(ctx, nt[m], lds[m], true, m == 1)));
     pb.grid[m] = (unsigned)std::min<size_t>(pb.n_items[m], (size_t)ctx->nof_cus * per_cu);
   }
   for (int pass = 0; pass < 2; ++pass) {
@@ -561,10 +605,21 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb)
   pb.ctr_words = ctr_words;
   pb.o_items = sizeof(hdr) + 4 * ctr_words;
   pb.o_jobs = pb.o_items + ((items.size() * sizeof(dec_item) + 15) & ~(size_t)15);
-  pb.bytes.assign(pb.o_jobs + jobs.size() * sizeof(dec_job), 0);
-  std::memcpy(pb.bytes.data(), hdr, sizeof(hdr));
-  std::memcpy(pb.bytes.data() + pb.o_items, items.data(), items.size() * sizeof(dec_item));
-  std::memcpy(pb.bytes.data() + pb.o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
+  pb.size = pb.o_jobs + jobs.size() * sizeof(dec_job);
+  uint8_t* out = dst;
+  if (out == nullptr) {
+    pb.bytes.resize(pb.size);
+    out = pb.bytes.data();
+  } else if (pb.size > cap) {
+    return fail(LH_ERR_CAPACITY, "descriptor block of %zu bytes exceeds the staging capacity %zu", pb.size, cap);
+  }
+  // Headers, zeroed counters, items (zero padding up to the jobs), jobs.
+  std::memcpy(out, hdr, sizeof(hdr));
+  std::memset(out + sizeof(hdr), 0, pb.o_items - sizeof(hdr));
+  std::memcpy(out + pb.o_items, items.data(), items.size() * sizeof(dec_item));
+  std::memset(out + pb.o_items + items.size() * sizeof(dec_item), 0,
+              pb.o_jobs - pb.o_items - items.size() * sizeof(dec_item));
+  std::memcpy(out + pb.o_jobs, jobs.data(), jobs.size() * sizeof(dec_job));
   pb.nt[0] = nt[0];
   pb.nt[1] = nt[1];
   pb.lds[0] = lds[0];
@@ -604,15 +659,12 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
     return LH_OK;
   }
   plan_block pb;
-  int rc = build_block(ctx, plan, pb);
+  // Packed straight into the pinned staging slot.
+  int rc = build_block(ctx, plan, pb, s->host + off, ctx->slot_bytes - off);
   if (rc != LH_OK) {
     return rc;
   }
-  if (off + pb.bytes.size() > ctx->slot_bytes) {
-    return fail(LH_ERR_CAPACITY, "descriptor block exceeds the staging capacity");
-  }
-  std::memcpy(s->host + off, pb.bytes.data(), pb.bytes.size());
-  rc = upload(ctx, s, off + pb.bytes.size(), stream);
+  rc = upload(ctx, s, off + pb.size, stream);
   if (rc != LH_OK) {
     return rc;
   }
@@ -938,6 +990,7 @@ int pusch_jobs(const lh_pusch_cb_desc* cbs, unsigned n, int new_data_all, int us
   if (nof_ldpc_iterations == 0 || nof_ldpc_iterations > 255) {
     return fail(LH_ERR_INVALID, "invalid number of LDPC iterations %u", nof_ldpc_iterations);
   }
+  plan.jobs.reserve(plan.jobs.size() + n);
   for (unsigned i = 0; i < n; ++i) {
     const lh_pusch_cb_desc& d = cbs[i];
     const int new_data = new_data_per_cb ? new_data_per_cb[i] : new_data_all;
@@ -1055,25 +1108,20 @@ int lh_pusch_decode_batch_noalloc(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsi
   if (rc != LH_OK || n == 0) {
     return rc;
   }
-  rc = build_block(ctx, plan, pb);
-  if (rc != LH_OK) {
-    return rc;
-  }
-  if (pb.bytes.size() > kScratchBytes) {
-    return fail(LH_ERR_CAPACITY, "lh_pusch_decode_batch_noalloc: descriptor block of %zu bytes exceeds the scratch",
-                pb.bytes.size());
-  }
   (void)hipSetDevice(ctx->device);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   slot* s = acquire_slot(ctx);
   if (s == nullptr) {
     return LH_ERR_HIP;
   }
-  // Staged through the slot's pinned host half; the copy is ordered on the caller's
-  // stream behind the previous call's kernels, which read the same scratch.  The work-queue
-  // counters arrive as zeros with the block.
-  std::memcpy(s->host, pb.bytes.data(), pb.bytes.size());
-  hipError_t e = hipMemcpyAsync(ctx->d_scratch, s->host, pb.bytes.size(), hipMemcpyHostToDevice, st);
+  // Packed in the slot's pinned host half, then copied to the scratch on the caller's
+  // stream, behind the previous call's kernels, which read the same scratch.  The
+  // sub-queue counters arrive as zeros with the block.
+  rc = build_block(ctx, plan, pb, s->host, std::min(kScratchBytes, ctx->slot_bytes));
+  if (rc != LH_OK) {
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(ctx->d_scratch, s->host, pb.size, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) {
     return hip_fail(e, "descriptor upload to the scratch");
   }
@@ -1219,9 +1267,9 @@ int lh_slot_graph_build(lh_ctx* ctx, const lh_pusch_cb_desc* cbs, unsigned n, co
   (void)hipSetDevice(ctx->device);
   lh_slot_graph* g = new lh_slot_graph();
   g->ctx = ctx;
-  hipError_t e = hipMalloc(&g->d_block, pb.bytes.size());
+  hipError_t e = hipMalloc(&g->d_block, pb.size);
   if (e == hipSuccess) {
-    e = hipMemcpy(g->d_block, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice);
+    e = hipMemcpy(g->d_block, pb.bytes.data(), pb.size, hipMemcpyHostToDevice);
   }
   if (e == hipSuccess && n_tbs > 0) {
     e = hipMalloc(&g->d_tbs, n_tbs * sizeof(lh_tb_desc));
