@@ -330,10 +330,11 @@ def uneven_batch(ctx, a):
 
 
 def serving_runs(ctx, a):
-    """Slot-deadline serving loop (0.5 ms slots, Poisson(24) UEs per slot): at the specified
-    per-UE load (PRBs in [4, 273]) and at one carrier's load (PRBs in [4, 18]), plain and
-    learned decoders on the same traces, at most 6 LDPC iterations (the reference gNB's
-    PUSCH default)."""
+    """Slot-deadline serving loop (0.5 ms slots, the library's native slot loop): at the
+    specified per-UE load (Poisson(24) UEs per slot, PRBs in [4, 273] each, 1-4 layers) and
+    at one 100 MHz carrier with 4 layers per slot (Poisson(24) UEs sharing 273 PRBs, 4 layers
+    each), plain and learned decoders on the same traces, at most 6 LDPC iterations (the
+    reference gNB's PUSCH default)."""
     from ocudu_hip import nms, serve
     tables = ROOT / "ocudu-dapp-platform_amd" / "tables"
     for bg in (1, 2):
@@ -342,12 +343,15 @@ def serving_runs(ctx, a):
             ctx.set_scale_table(bg, aq, bq)
     thr = serve.mcs_thresholds(ctx)
     out = {"slots": a.serve_slots, "slot_us": 500, "mean_ues": 24, "ldpc_iterations": serve.PUSCH_LDPC_ITERATIONS}
-    for name, prb_max in (("spec_load", 273), ("carrier_load", 18)):
-        trace = serve.Trace(ctx, a.serve_slots, 24.0, thr, seed=a.seed, prb_max=prb_max)
-        entry = {"prb_max": prb_max, "tbs": len(trace.tbs),
-                 "codeblocks": int(sum(t["seg"].nof_cbs for t in trace.tbs))}
+    loads = (("spec_load", dict(prb_max=273, layers=(1, 4))),
+             ("carrier_100mhz_4layer", dict(carrier_prbs=273, layers=(4, 4))))
+    for name, kw in loads:
+        trace = serve.Trace(ctx, a.serve_slots, 24.0, thr, seed=a.seed, **kw)
+        entry = {"load": {k: list(v) if isinstance(v, tuple) else v for k, v in kw.items()}, "tbs": len(trace.tbs),
+                 "codeblocks": int(sum(t["seg"].nof_cbs for t in trace.tbs)),
+                 "codeblocks_per_slot": round(sum(t["seg"].nof_cbs for t in trace.tbs) / a.serve_slots, 1)}
         for dec in ("plain", "learned"):
-            entry[dec] = serve.summarize(trace, *serve.serve(ctx, trace, learned=dec == "learned"))
+            entry[dec] = serve.summarize(trace, *serve.serve(ctx, trace, learned=dec == "learned", native=True))
             print(f"# serving {name} {dec}: {json.dumps(entry[dec])}", file=sys.stderr, flush=True)
         out[name] = entry
         del trace

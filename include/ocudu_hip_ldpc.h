@@ -468,6 +468,41 @@ int lh_server_stats(lh_server* server, lh_server_stats_t* stats);
  * later retransmission for it is rejected.  Slots compare modulo 2^32. */
 int lh_server_run_slot(lh_server* server, uint32_t slot);
 
+/* ---- The slot loop, natively: the serving side of the PUSCH processor over a trace of
+ * transport blocks (each first transmitted in its slot, retransmitted with its second
+ * request harq_rtt_slots later when the first fails), driven by one host thread without
+ * an interpreter between the slot boundaries and the completions. ---- */
+#define LH_TB_NOT_SENT (-3) /* result of a retransmission that was not needed or fell past the run */
+
+typedef struct lh_serve_tb {
+  lh_tb_request tx[2]; /* new transmission, then its retransmission (same harq_process); ids are set by the loop
+                          to 2 i + tx and slots to the slot each is sent in */
+  uint8_t* tb_out[2];  /* host TB bytes of each transmission (may be NULL) */
+  uint32_t slot;       /* slot of the new transmission */
+  uint32_t pad;
+} lh_serve_tb;
+
+typedef struct lh_serve_config {
+  double slot_us;          /* slot duration */
+  double timeout_s;        /* the loop fails with LH_ERR_CAPACITY past this wall time */
+  unsigned nof_slots;      /* slots with new transmissions: 0 .. nof_slots - 1 */
+  unsigned harq_rtt_slots; /* slots from a failed transmission to its retransmission */
+  int realtime;            /* 1: slot k is released k * slot_us after the start; 0: as soon as the slots whose
+                              retransmissions fall in it have completed (deterministic) */
+  unsigned pad;
+} lh_serve_config;
+
+/* Runs the trace through the server: at each slot boundary lh_server_run_slot(k), then the
+ * slot's new transmissions and the retransmissions due in it are submitted and stepped;
+ * completions are polled continuously.  A retransmission is due max(slot + rtt, current
+ * slot) after its failed transmission; one that would fall past the run, and a failed
+ * retransmission, release their HARQ process.  results: 2 n_tbs entries, index 2 i + tx
+ * (LH_TB_NOT_SENT where nothing was sent); slot_latency_us: nof_slots + harq_rtt_slots + 2
+ * entries, submission of a slot to the completion of its last transmission (-1: nothing
+ * sent in it); wall_s: the loop's wall time. */
+int lh_server_serve(lh_server* server, const lh_serve_tb* tbs, unsigned n_tbs, const lh_serve_config* cfg,
+                    lh_tb_result* results, double* slot_latency_us, double* wall_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2462,6 +2462,130 @@ int lh_server_run_slot(lh_server* sv, uint32_t slot)
   return LH_OK;
 }
 
+int lh_server_serve(lh_server* sv, const lh_serve_tb* tbs, unsigned n_tbs, const lh_serve_config* cfg,
+                    lh_tb_result* results, double* slot_latency_us, double* wall_s)
+{
+  if (sv == nullptr || cfg == nullptr || results == nullptr || slot_latency_us == nullptr ||
+      (n_tbs > 0 && tbs == nullptr) || cfg->slot_us <= 0.0 || cfg->nof_slots == 0) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_server_serve");
+  }
+  const unsigned rtt = cfg->harq_rtt_slots;
+  const unsigned last_slot = cfg->nof_slots + rtt + 2;
+  std::vector<std::vector<uint32_t>> fresh(cfg->nof_slots), retx(last_slot);
+  for (unsigned i = 0; i < n_tbs; ++i) {
+    results[2 * i] = results[2 * i + 1] = lh_tb_result{};
+    results[2 * i].id = 2 * (uint64_t)i;
+    results[2 * i + 1].id = 2 * (uint64_t)i + 1;
+    results[2 * i].tb_crc_ok = results[2 * i + 1].tb_crc_ok = LH_TB_NOT_SENT;
+    if (tbs[i].slot < cfg->nof_slots) {
+      fresh[tbs[i].slot].push_back(i);
+    }
+  }
+  std::vector<double> start(last_slot, -1.0);
+  std::vector<int> outstanding(last_slot, 0);
+  std::vector<uint32_t> sent_in(2 * (size_t)n_tbs, 0);
+  for (unsigned k = 0; k < last_slot; ++k) {
+    slot_latency_us[k] = -1.0;
+  }
+  unsigned k = 0, first_open = 0;
+  size_t open = 0; // slots with outstanding transmissions
+  std::vector<lh_tb_result> got(256);
+  std::vector<lh_tb_request> reqs;
+  std::vector<uint8_t*> outs;
+  const double t0 = now_us();
+  int rc = LH_OK;
+  auto complete = [&](const lh_tb_result& r) {
+    if (r.id >= 2 * (uint64_t)n_tbs) {
+      return;
+    }
+    results[r.id] = r;
+    const uint32_t s = sent_in[r.id];
+    if (--outstanding[s] == 0) {
+      slot_latency_us[s] = now_us() - start[s];
+      --open;
+    }
+    const unsigned i = (unsigned)(r.id / 2), tx = (unsigned)(r.id % 2);
+    if (r.tb_crc_ok != LH_TB_CRC_OK) {
+      const unsigned due = std::max(s + rtt, k);
+      if (tx == 0 && due < last_slot) {
+        retx[due].push_back(i);
+      } else {
+        (void)lh_server_harq_release(sv, tbs[i].tx[0].harq_process);
+      }
+    }
+  };
+  auto drain = [&](int wait) {
+    unsigned n = 0;
+    int r = lh_server_poll(sv, got.data(), (unsigned)got.size(), &n, wait);
+    for (unsigned j = 0; j < n; ++j) {
+      complete(got[j]);
+    }
+    return r;
+  };
+  while (rc == LH_OK && (k < last_slot || open > 0)) {
+    if (cfg->realtime) {
+      rc = drain(0);
+      if (rc != LH_OK) {
+        break;
+      }
+    }
+    const double now = now_us();
+    while (first_open < k && outstanding[first_open] == 0) {
+      ++first_open;
+    }
+    // Deterministic release: slot k once the slots whose retransmissions fall in it are
+    // done; or when nothing is in flight or ready (requests wait for HARQ memory that only
+    // the run's progress frees).
+    bool due = cfg->realtime != 0 || open == 0 || first_open + rtt >= k + 1;
+    if (!cfg->realtime && !due) {
+      lh_server_stats_t st;
+      (void)lh_server_stats(sv, &st);
+      due = st.slots_in_flight == 0 && st.ready == 0;
+    }
+    if (now - t0 > cfg->timeout_s * 1e6) {
+      rc = fail(LH_ERR_CAPACITY, "serving loop exceeded %.1f s at slot %u", cfg->timeout_s, k);
+      break;
+    }
+    if (k < last_slot && ((cfg->realtime && now >= t0 + k * cfg->slot_us) || (!cfg->realtime && due))) {
+      (void)lh_server_run_slot(sv, k);
+      reqs.clear();
+      outs.clear();
+      const std::vector<uint32_t>* lists[2] = {k < cfg->nof_slots ? &fresh[k] : nullptr, &retx[k]};
+      for (int tx = 0; tx < 2; ++tx) {
+        if (lists[tx] == nullptr) {
+          continue;
+        }
+        for (uint32_t i : *lists[tx]) {
+          lh_tb_request q = tbs[i].tx[tx];
+          q.id = 2 * (uint64_t)i + tx;
+          q.slot = k;
+          sent_in[q.id] = k;
+          reqs.push_back(q);
+          outs.push_back(tbs[i].tb_out[tx]);
+        }
+      }
+      if (!reqs.empty()) {
+        start[k] = now_us();
+        outstanding[k] += (int)reqs.size();
+        ++open;
+        rc = lh_server_submit_batch(sv, reqs.data(), (unsigned)reqs.size(), outs.data());
+        if (rc != LH_OK) {
+          break;
+        }
+      }
+      ++k;
+    }
+    rc = lh_server_step(sv);
+    if (rc == LH_OK) {
+      rc = drain(!cfg->realtime && (k >= last_slot || !due) ? 1 : 0);
+    }
+  }
+  if (wall_s != nullptr) {
+    *wall_s = (now_us() - t0) * 1e-6;
+  }
+  return rc;
+}
+
 int lh_server_stats(lh_server* sv, lh_server_stats_t* st)
 {
   if (sv == nullptr || st == nullptr) {

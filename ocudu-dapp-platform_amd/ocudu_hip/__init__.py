@@ -223,12 +223,21 @@ class TbResult(C.Structure):
                 ("latency_us", C.c_double)]
 
 
+class ServeTb(C.Structure):
+    _fields_ = [("tx", TbRequest * 2), ("tb_out", C.c_void_p * 2), ("slot", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slot_us", C.c_double), ("timeout_s", C.c_double), ("nof_slots", C.c_uint),
+                ("harq_rtt_slots", C.c_uint), ("realtime", C.c_int), ("pad", C.c_uint)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("harq_free_bytes", C.c_uint64), ("harq_processes", C.c_uint32), ("pending", C.c_uint32),
                 ("ready", C.c_uint32), ("slots_in_flight", C.c_uint32)]
 
 
-LH_TB_CRC_OK, LH_TB_CB_FAILED, LH_TB_CRC_FAILED, LH_TB_REJECTED = 1, 0, -1, -2
+LH_TB_CRC_OK, LH_TB_CB_FAILED, LH_TB_CRC_FAILED, LH_TB_REJECTED, LH_TB_NOT_SENT = 1, 0, -1, -2, -3
 
 # The 51 NR lifting sizes (TS38.212 Table 5.3.2-1).
 LIFTING_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40,
@@ -254,7 +263,7 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
             "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch",
-            "lh_pusch_decode_batch_noalloc", "lh_server_run_slot"]
+            "lh_pusch_decode_batch_noalloc", "lh_server_run_slot", "lh_server_serve"]
 
 
 def lib():
@@ -310,6 +319,8 @@ def lib():
         L.lh_server_harq_release.argtypes = [C.c_void_p, C.c_uint32]
         L.lh_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
         L.lh_server_run_slot.argtypes = [C.c_void_p, C.c_uint32]
+        L.lh_server_serve.argtypes = [C.c_void_p, C.POINTER(ServeTb), C.c_uint, C.POINTER(ServeConfig),
+                                      C.POINTER(TbResult), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lh_pusch_demodulate.argtypes = [C.c_void_p, C.POINTER(DemodDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]
         L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
@@ -512,6 +523,17 @@ class Server:
 
     def run_slot(self, slot):
         _lh(lib().lh_server_run_slot(self.handle, slot), "lh_server_run_slot")
+
+    def serve(self, tbs, nof_slots, harq_rtt_slots, slot_us=500.0, realtime=True, timeout_s=600.0):
+        """The native slot loop over a ctypes array of ServeTb: (results, slot latencies in
+        us, wall seconds); results[2 i + tx] per transmission."""
+        cfg = ServeConfig(slot_us, timeout_s, nof_slots, harq_rtt_slots, int(realtime), 0)
+        res = (TbResult * (2 * len(tbs)))()
+        lat = (C.c_double * (nof_slots + harq_rtt_slots + 2))()
+        wall = C.c_double()
+        _lh(lib().lh_server_serve(self.handle, tbs, len(tbs), C.byref(cfg), res, lat, C.byref(wall)),
+            "lh_server_serve")
+        return list(res), list(lat), wall.value
 
     def stats(self):
         st = ServerStats()

@@ -23,8 +23,8 @@ import time
 
 import numpy as np
 
-from . import (LH_CRC16, LH_CRC24A, LH_CRC24B, LH_GEN_MSG_INPUT, LH_TB_CRC_OK, Context, DeviceBuffer, GenDesc,
-               GenTbDesc, Server, TbRequest, msg_bytes, segment_rx)
+from . import (LH_CRC16, LH_CRC24A, LH_CRC24B, LH_GEN_MSG_INPUT, LH_TB_CRC_OK, LH_TB_NOT_SENT, Context, DeviceBuffer,
+               GenDesc, GenTbDesc, ServeTb, Server, TbRequest, msg_bytes, segment_rx)
 
 # TS38.214 Table 5.1.3.1-2 (MCS index table 2, 256-QAM): (Qm, R x 1024).
 MCS_256QAM = [(2, 120), (2, 193), (2, 308), (2, 449), (2, 602), (4, 378), (4, 434), (4, 490), (4, 553), (4, 616),
@@ -91,21 +91,31 @@ class Trace:
     """The TBs of a run with all their transmissions' LLRs on the device."""
 
     def __init__(self, ctx, slots, mean_ues, thresholds, seed=0, snr_lo=-0.2, snr_hi=1.0, prb_max=273,
-                 max_cbs_per_tb=512):
+                 max_cbs_per_tb=512, carrier_prbs=0, layers=(1, 4)):
+        """carrier_prbs > 0: the slot's UEs share one carrier of that many PRBs (a random
+        split, at least 4 PRBs each, so fewer UEs when the carrier is full); otherwise each
+        UE draws its PRBs in [4, prb_max].  Layers uniform in the `layers` range."""
         rng = np.random.default_rng(seed)
         self.tbs = []  # dicts
         for k in range(slots):
-            for _ in range(int(rng.poisson(mean_ues))):
+            n_ue = int(rng.poisson(mean_ues))
+            if carrier_prbs:
+                n_ue = max(1, min(n_ue, carrier_prbs // 4))
+                cuts = np.sort(rng.choice(np.arange(1, carrier_prbs // 4), n_ue - 1, replace=False)) if n_ue > 1 else []
+                edges = np.concatenate([[0], cuts, [carrier_prbs // 4]]).astype(int)
+                prbs = [4 * int(b - a) for a, b in zip(edges[:-1], edges[1:])]
+                prbs[-1] += carrier_prbs % 4
+            for u in range(n_ue):
                 mcs = int(rng.integers(0, len(MCS_256QAM)))
                 qm, r1024 = MCS_256QAM[mcs]
-                prb = int(rng.integers(4, prb_max + 1))
-                layers = int(rng.integers(1, 5))
-                tbs = tbs_of(prb, qm, r1024, layers)
+                prb = prbs[u] if carrier_prbs else int(rng.integers(4, prb_max + 1))
+                nl = int(rng.integers(layers[0], layers[1] + 1))
+                tbs = tbs_of(prb, qm, r1024, nl)
                 bg = base_graph_of(tbs, r1024 / 1024.0)
-                nsym = RE_PER_PRB * prb * layers
-                seg, rm, off = segment_rx(bg, tbs, qm, layers, nsym, max_cbs_per_tb)
+                nsym = RE_PER_PRB * prb * nl
+                seg, rm, off = segment_rx(bg, tbs, qm, nl, nsym, max_cbs_per_tb)
                 snr = thresholds[mcs] + float(rng.uniform(snr_lo, snr_hi))
-                self.tbs.append(dict(slot=k, mcs=mcs, qm=qm, prb=prb, layers=layers, tbs=tbs, bg=bg, nsym=nsym,
+                self.tbs.append(dict(slot=k, mcs=mcs, qm=qm, prb=prb, layers=nl, tbs=tbs, bg=bg, nsym=nsym,
                                      seg=seg, rm=rm, off=off, snr=snr, seed=int(rng.integers(1, 2 ** 62)),
                                      noise=[int(rng.integers(1, 2 ** 62)) for _ in range(2)]))
         self._generate(ctx)
@@ -156,12 +166,19 @@ class Trace:
 
 
 def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs_per_slot=4096, slots_in_flight=3,
-          iters=PUSCH_LDPC_ITERATIONS, realtime=True, log=None, timeout_s=600.0, expire_slots=HARQ_EXPIRE_SLOTS):
+          iters=PUSCH_LDPC_ITERATIONS, realtime=True, log=None, timeout_s=600.0, expire_slots=HARQ_EXPIRE_SLOTS,
+          native=False):
     """Runs the trace through a server: returns per-transmission results and per-slot
     latencies.  realtime: slots are released every slot_us; else as fast as possible.
-    HARQ processes expire expire_slots after their last transmission's slot (0: never)."""
+    HARQ processes expire expire_slots after their last transmission's slot (0: never).
+    native: the slot loop runs in the library (lh_server_serve) instead of here."""
     srv = Server(ctx, harq_budget, max_cbs_per_slot, slots_in_flight, iters, learned=learned,
                  nof_fallback_iterations=iters, expire_timeout_slots=expire_slots)
+    if native:
+        try:
+            return _serve_native(srv, trace, slot_us, realtime, timeout_s)
+        finally:
+            srv.close()
     by_slot = {}
     for i, t in enumerate(trace.tbs):
         by_slot.setdefault(t["slot"], []).append(i)
@@ -192,6 +209,28 @@ def serve(ctx, trace, learned=False, slot_us=500.0, harq_budget=8 << 30, max_cbs
     srv.close()
     slot_lat = [(slot_done[s] - slot_start[s]) * 1e6 for s in slot_done if s in slot_start]
     return results, slot_lat, wall, out_bufs
+
+
+def _serve_native(srv, trace, slot_us, realtime, timeout_s):
+    """The same loop in the library: every transmission's request and TB buffer prepared
+    ahead, one call."""
+    n = len(trace.tbs)
+    bufs = [(np.zeros(t["tbs"] // 8, np.uint8), np.zeros(t["tbs"] // 8, np.uint8)) for t in trace.tbs]
+    arr = (ServeTb * n)()
+    for i, t in enumerate(trace.tbs):
+        arr[i].tx[0] = trace.request(i, 0)
+        arr[i].tx[1] = trace.request(i, 1)
+        arr[i].tb_out[0] = bufs[i][0].ctypes.data
+        arr[i].tb_out[1] = bufs[i][1].ctypes.data
+        arr[i].slot = t["slot"]
+    nslots = max(t["slot"] for t in trace.tbs) + 1 if n else 1
+    res, lat, wall = srv.serve(arr, nslots, HARQ_RTT_SLOTS, slot_us=slot_us, realtime=realtime, timeout_s=timeout_s)
+    results, out_bufs = {}, {}
+    for k, r in enumerate(res):
+        if r.tb_crc_ok != LH_TB_NOT_SENT:
+            results[k] = r
+            out_bufs[k] = bufs[k // 2][k % 2]
+    return results, [x for x in lat if x >= 0], wall, out_bufs
 
 
 def _serve_loop(srv, trace, prepared, by_slot, out_bufs, tx_slot, slot_us, realtime, timeout_s, nslots):

@@ -376,3 +376,27 @@ def test_harq_expiry(setup):
     r = _one(srv, req(2, 5, 1, 15))  # retransmission of the expired process
     assert r.tb_crc_ok == H.LH_TB_REJECTED
     srv.close()
+
+
+def test_native_loop_equals_python_loop(setup):
+    """The library's slot loop (lh_server_serve) against the Python loop over the same trace:
+    deterministic runs give the same transmissions with the same results and bytes, and a
+    real-time run gives the same result for every transmission both made."""
+    print("case native slot loop", flush=True)
+    from ocudu_hip import serve
+    ctx, trace = setup
+    py = serve.serve(ctx, trace, realtime=False, timeout_s=120)
+    nat = serve.serve(ctx, trace, realtime=False, timeout_s=120, native=True)
+    rt = serve.serve(ctx, trace, realtime=True, timeout_s=120, native=True)
+    assert set(py[0]) == set(nat[0])
+    for k, r in py[0].items():
+        m = nat[0][k]
+        assert (r.tb_crc_ok, r.nof_cbs_ok, r.max_iterations) == (m.tb_crc_ok, m.nof_cbs_ok, m.max_iterations), k
+        if r.tb_crc_ok == H.LH_TB_CRC_OK:
+            assert np.array_equal(py[3][k], nat[3][k])
+    common = set(py[0]) & set(rt[0])
+    assert len(common) >= len(trace.tbs)
+    for k in common:
+        r, m = py[0][k], rt[0][k]
+        assert (r.tb_crc_ok, r.nof_cbs_ok, r.max_iterations) == (m.tb_crc_ok, m.nof_cbs_ok, m.max_iterations), k
+    assert len(nat[1]) > 0 and all(x >= 0 for x in nat[1])
