@@ -338,6 +338,61 @@ typedef struct lh_demod_desc {
 int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, const float* d_symbols,
                         const float* d_noise_vars, int8_t* d_llrs, void* stream);
 
+/* One codeword of lh_ulsch_demultiplex: ulsch_demultiplex::configuration
+ * (include/ocudu/phy/upper/channel_processors/pusch/ulsch_demultiplex.h:35-66) plus the CSI
+ * Part 2 sizes of set_csi_part2 (:75) and the codeword's scrambling initialisation. */
+typedef struct lh_ulsch_demux_desc {
+  uint64_t llr_offset;       /* the descrambled codeword (lh_pusch_demodulate's output) in d_llrs */
+  uint64_t sch_offset;       /* UL-SCH soft bits written to d_sch */
+  uint64_t harq_ack_offset;  /* HARQ-ACK soft bits written to d_harq_ack */
+  uint64_t csi_part1_offset; /* CSI Part 1 soft bits written to d_csi_part1 */
+  uint64_t csi_part2_offset; /* CSI Part 2 soft bits written to d_csi_part2 */
+  uint32_t modulation;       /* LH_MOD_* */
+  uint32_t nof_layers;       /* 1..4 */
+  uint32_t nof_prb;          /* 1..275 */
+  uint32_t start_symbol_index;
+  uint32_t nof_symbols;      /* start_symbol_index + nof_symbols <= 14 */
+  uint32_t dmrs_symbol_mask; /* bit l set: OFDM symbol l of the slot carries DM-RS */
+  uint32_t dmrs_type;        /* 1 or 2 */
+  uint32_t nof_cdm_groups_without_data;
+  uint32_t nof_harq_ack_rvd;       /* G_rvd^ACK, bits */
+  uint32_t nof_harq_ack_bits;      /* O_ACK */
+  uint32_t nof_enc_harq_ack_bits;  /* G^ACK */
+  uint32_t nof_csi_part1_bits;     /* O_CSI-1 */
+  uint32_t nof_enc_csi_part1_bits; /* G^CSI-1 */
+  uint32_t nof_csi_part2_bits;     /* O_CSI-2 (0: no CSI Part 2) */
+  uint32_t nof_enc_csi_part2_bits; /* G^CSI-2 */
+  uint32_t c_init;                 /* the codeword's scrambling initialisation (lh_demod_desc.c_init) */
+} lh_ulsch_demux_desc;
+
+typedef struct lh_ulsch_demux_sizes {
+  uint32_t nof_llrs;      /* codeword soft bits read */
+  uint32_t nof_sch;       /* UL-SCH soft bits: the rate dematcher's codeword */
+  uint32_t nof_harq_ack;
+  uint32_t nof_csi_part1;
+  uint32_t nof_csi_part2;
+} lh_ulsch_demux_sizes;
+
+/* Sizes of one codeword's demultiplexed fields (host only, no context).  Rejects what the
+ * reference rejects with an assertion: a field whose encoded bits are not a multiple of
+ * the bits per RE, that does not fit its REs, or whose payload and encoded sizes are not
+ * both zero or both non-zero; an invalid DM-RS configuration. */
+int lh_ulsch_demux_get_sizes(const lh_ulsch_demux_desc* desc, lh_ulsch_demux_sizes* sizes);
+
+/* UL-SCH demultiplexing of TS38.212 6.2.7 on the device: ulsch_demultiplex_impl
+ * (lib/phy/upper/channel_processors/pusch/ulsch_demultiplex_impl.cpp:178-573) over whole
+ * codewords, CSI Part 2 known before the first OFDM symbol (set_csi_part2 right after
+ * demultiplex; the reference's resident index builder
+ * pusch_demodulator_gpu_impl.cpp:471-625 allocates the same).  Per OFDM symbol: reserved
+ * REs for a HARQ-ACK of up to 2 bits, HARQ-ACK of more than 2 bits, CSI Part 1, CSI Part 2;
+ * the UL-SCH soft bits are the rest, in RE order, with the REs of a 1- or 2-bit HARQ-ACK
+ * zeroed; UCI of 1 or 2 bits has its placeholder bits re-scrambled as
+ * on_uci_placeholder_1bit/_2bit (:86-176) do.  The outputs' sizes are those of
+ * lh_ulsch_demux_get_sizes; an output pointer may be NULL when no codeword has that field. */
+int lh_ulsch_demultiplex(lh_ctx* ctx, const lh_ulsch_demux_desc* descs, unsigned n, const int8_t* d_llrs,
+                         int8_t* d_sch, int8_t* d_harq_ack, int8_t* d_csi_part1, int8_t* d_csi_part2,
+                         void* stream);
+
 /* One transport block of lh_gen_transport_blocks. */
 typedef struct lh_gen_tb_desc {
   uint64_t tb_offset;      /* TB bytes (tbs_bits / 8) written to d_tb */

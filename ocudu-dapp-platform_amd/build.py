@@ -52,11 +52,17 @@ def build(verbose=True):
 
 
 def build_tests(oracle_lib):
-    """The adapter chain test (tests/cpp): links the adapters and the CPU oracle."""
+    """The C++ tests (tests/cpp): the adapter chain test, linking the adapters and the CPU
+    oracle, and the demultiplexer's host-side check."""
     out = ROOT / "tests" / "cpp" / "adapter_chain_test"
     run([HIPCC, *COMMON, f"-I{PKG / 'host'}", ROOT / "tests" / "cpp" / "adapter_chain_test.cpp", "-o", out,
          f"-L{LIB}", "-locudu_hip_adapters", "-locudu_hip_ldpc", f"-Wl,-rpath,{LIB}",
          f"-L{oracle_lib.parent}", "-loracle", f"-Wl,-rpath,{oracle_lib.parent}"])
+    # Host-side replay of the device demultiplexer's arithmetic against the oracle.
+    demux = ROOT / "tests" / "cpp" / "demux_rank_test"
+    run([HIPCC, *COMMON, ROOT / "tests" / "cpp" / "demux_rank_test.cpp", "-o", demux, f"-L{LIB}",
+         "-locudu_hip_ldpc", f"-Wl,-rpath,{LIB}", f"-L{oracle_lib.parent}", "-loracle",
+         f"-Wl,-rpath,{oracle_lib.parent}"])
     return out
 
 

@@ -39,6 +39,8 @@ hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs
                                  const uint32_t* gtab, hipStream_t s);
 hipError_t launch_demod(const demod_tile* tiles, unsigned nof_tiles, const demod_job* jobs, const uint32_t* tab,
                         const float* symbols, const float* noise, int8_t* llrs, hipStream_t stream);
+hipError_t launch_demux(const demux_tile* tiles, unsigned nof_tiles, const demux_sym* syms, const uint32_t* tab,
+                        const int8_t* llrs, int8_t* sch, int8_t* ack, int8_t* c1, int8_t* c2, hipStream_t stream);
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
                                     uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
 size_t nms_train_lds_bytes(int bg, int Z, int L);
@@ -1604,6 +1606,80 @@ int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, con
                               d_noise_vars, d_llrs, st);
   retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "demodulator launch");
+}
+
+int lh_ulsch_demux_get_sizes(const lh_ulsch_demux_desc* desc, lh_ulsch_demux_sizes* sizes)
+{
+  if (desc == nullptr || sizes == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_ulsch_demux_get_sizes");
+  }
+  const char* why = nullptr;
+  if (demux_plan(*desc, nullptr, nullptr, sizes, &why) != 0) {
+    return fail(LH_ERR_INVALID, "UL-SCH demultiplexer: %s", why);
+  }
+  return LH_OK;
+}
+
+int lh_ulsch_demultiplex(lh_ctx* ctx, const lh_ulsch_demux_desc* descs, unsigned n, const int8_t* d_llrs,
+                         int8_t* d_sch, int8_t* d_harq_ack, int8_t* d_csi_part1, int8_t* d_csi_part2, void* stream)
+{
+  if (ctx == nullptr || (n > 0 && (descs == nullptr || d_llrs == nullptr))) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_ulsch_demultiplex");
+  }
+  std::vector<demux_sym> syms((size_t)n * 14);
+  size_t nof_syms = 0, nof_tiles = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    const char* why = nullptr;
+    unsigned ns = 0;
+    lh_ulsch_demux_sizes sz;
+    if (demux_plan(descs[i], syms.data() + nof_syms, &ns, &sz, &why) != 0) {
+      return fail(LH_ERR_INVALID, "codeword %u: %s", i, why);
+    }
+    if ((sz.nof_sch && d_sch == nullptr) || (sz.nof_harq_ack && d_harq_ack == nullptr) ||
+        (sz.nof_csi_part1 && d_csi_part1 == nullptr) || (sz.nof_csi_part2 && d_csi_part2 == nullptr)) {
+      return fail(LH_ERR_INVALID, "codeword %u: an output it writes is NULL", i);
+    }
+    for (unsigned k = 0; k < ns; ++k) {
+      const demux_sym& s = syms[nof_syms + k];
+      const uint32_t units = s.M * s.nbpr / s.unit;
+      nof_tiles += (units + kDemuxTileUnits - 1) / kDemuxTileUnits;
+    }
+    nof_syms += ns;
+  }
+  const size_t sym_bytes = (nof_syms * sizeof(demux_sym) + 15) & ~(size_t)15;
+  const size_t bytes = sym_bytes + nof_tiles * sizeof(demux_tile);
+  if (bytes > ctx->slot_bytes) {
+    return fail(LH_ERR_CAPACITY, "%u codewords of %zu tiles exceed the context's staging (%zu bytes)", n, nof_tiles,
+                ctx->slot_bytes);
+  }
+  if (nof_tiles == 0) {
+    return LH_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  slot* s = acquire_slot(ctx);
+  if (s == nullptr) {
+    return LH_ERR_HIP;
+  }
+  std::memcpy(s->host, syms.data(), nof_syms * sizeof(demux_sym));
+  demux_tile* tiles = reinterpret_cast<demux_tile*>(s->host + sym_bytes);
+  size_t t = 0;
+  for (size_t k = 0; k < nof_syms; ++k) {
+    const uint32_t units = syms[k].M * syms[k].nbpr / syms[k].unit;
+    for (uint32_t j = 0; j * kDemuxTileUnits < units; ++j) {
+      tiles[t++] = demux_tile{(uint32_t)k, j};
+    }
+  }
+  int rc = upload(ctx, s, bytes, st);
+  if (rc != LH_OK) {
+    return rc;
+  }
+  const uint8_t* dv = reinterpret_cast<const uint8_t*>(s->dev);
+  hipError_t e = launch_demux(reinterpret_cast<const demux_tile*>(dv + sym_bytes), (unsigned)nof_tiles,
+                              reinterpret_cast<const demux_sym*>(dv), ctx->d_demod, d_llrs, d_sch, d_harq_ack,
+                              d_csi_part1, d_csi_part2, st);
+  retire(s, st);
+  return e == hipSuccess ? LH_OK : hip_fail(e, "demultiplexer launch");
 }
 
 } // extern "C"

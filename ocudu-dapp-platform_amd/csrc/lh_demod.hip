@@ -1,5 +1,6 @@
 // Soft demodulation and descrambling of PUSCH codeword blocks: equalized symbols and
-// their noise variances -> int8 LLRs with the scrambling sequence removed.
+// their noise variances -> int8 LLRs with the scrambling sequence removed; then the
+// UL-SCH demultiplexing of whole codewords (k_ulsch_demux, below).
 //
 // Follows the reference's generic (scalar) demappers and the PUSCH descrambling:
 //  * demodulation_mapper_impl.cpp:13-47 (BPSK, pi/2-BPSK), demodulation_mapper_qpsk.cpp
@@ -28,6 +29,7 @@
 #include <cstdint>
 
 #include "lh_internal.h"
+#include "lh_demux_rank.h"
 
 namespace lh {
 
@@ -384,7 +386,124 @@ __global__ void __launch_bounds__(kDemodThreads)
   }
 }
 
+// ---- UL-SCH demultiplexing ---------------------------------------------------------
+//
+// One thread per unit (1, 2 or 4 soft bits of one RE) of one OFDM symbol; the RE's fields
+// and ranks come from demux_classify (lh_demux_rank.h).
+
+__device__ __forceinline__ uint32_t seq_word(uint32_t s1, uint32_t s2)
+{
+  const uint32_t w1 = s1 | (((s1 ^ (s1 >> 3)) & 1u) << 31);
+  const uint32_t w2 = s2 | (((s2 ^ (s2 >> 1) ^ (s2 >> 2) ^ (s2 >> 3)) & 1u) << 31);
+  return w1 ^ w2;
+}
+
+// Sequence bits 32 w .. 32 w + 63 after the symbol's first soft bit.
+__device__ uint64_t demux_seq_window(uint32_t s1, uint32_t s2, const uint32_t* __restrict__ tab, uint32_t w)
+{
+  for (int k = 0; k < kDemodJumps && (w >> k) != 0; ++k) {
+    if ((w >> k) & 1u) {
+      s1 = gf2_apply(tab + k * 31, s1);
+      s2 = gf2_apply(tab + (kDemodJumps + k) * 31, s2);
+    }
+  }
+  const uint32_t lo = seq_word(s1, s2);
+  const uint32_t hi = seq_word(gf2_apply(tab, s1), gf2_apply(tab + kDemodJumps * 31, s2));
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// on_uci_placeholder_1bit / _2bit (ulsch_demultiplex_impl.cpp:86-176) on the unit of u soft
+// bits starting at symbol position r: bit 0 of each modulation symbol kept; bit 1 kept
+// (2-bit payload) or negated where c(n - 1) ^ c(n) = 1 (1-bit payload); bits 2.. negated
+// where c(n) = 1.
+__device__ uint32_t demux_placeholder(uint32_t v, uint32_t r, uint32_t u, uint32_t qm, uint32_t payload,
+                                      uint32_t x1, uint32_t x2, const uint32_t* __restrict__ tab)
+{
+  if (payload == 0 || v == 0) {
+    return v;
+  }
+  const uint32_t base = r == 0 ? 0u : ((r - 1u) & ~31u);
+  const uint64_t win = demux_seq_window(x1, x2, tab, base >> 5);
+  uint32_t out = 0;
+  for (uint32_t i = 0; i < u; ++i) {
+    const uint32_t pos = r + i;
+    const uint32_t k = pos % qm;
+    const uint32_t c = (uint32_t)(win >> (pos - base)) & 1u;
+    uint32_t neg = 0;
+    if (k == 1 && payload == 1) {
+      neg = c ^ ((uint32_t)(win >> (pos - 1u - base)) & 1u);
+    } else if (k >= 2) {
+      neg = c;
+    }
+    const int8_t b = (int8_t)(v >> (8 * i));
+    out |= (uint32_t)(uint8_t)(neg ? (int8_t)-b : b) << (8 * i);
+  }
+  return out;
+}
+
+__device__ __forceinline__ uint32_t load_unit(const int8_t* p, uint32_t u)
+{
+  return u == 4 ? *reinterpret_cast<const uint32_t*>(p)
+                : (u == 2 ? (uint32_t)*reinterpret_cast<const uint16_t*>(p) : (uint32_t)(uint8_t)*p);
+}
+
+__device__ __forceinline__ void store_unit(int8_t* p, uint32_t v, uint32_t u)
+{
+  if (u == 4) {
+    *reinterpret_cast<uint32_t*>(p) = v;
+  } else if (u == 2) {
+    *reinterpret_cast<uint16_t*>(p) = (uint16_t)v;
+  } else {
+    *p = (int8_t)v;
+  }
+}
+
+__global__ void __launch_bounds__(kDemuxThreads)
+    k_ulsch_demux(const demux_tile* __restrict__ tiles, const demux_sym* __restrict__ syms,
+                  const uint32_t* __restrict__ tab, const int8_t* __restrict__ llrs, int8_t* __restrict__ sch,
+                  int8_t* __restrict__ ack, int8_t* __restrict__ c1, int8_t* __restrict__ c2)
+{
+  const demux_tile t = tiles[blockIdx.x];
+  const demux_sym& s = syms[t.sym];
+  const uint32_t u = s.unit, nbpr = s.nbpr;
+  const uint32_t r = (t.tile * kDemuxTileUnits + threadIdx.x) * u;
+  if (r >= s.M * nbpr) {
+    return;
+  }
+  const uint32_t v = load_unit(llrs + s.llr_in + r, u);
+  if (s.plain) {
+    store_unit(sch + s.sch_out + r, v, u);
+    return;
+  }
+  const uint32_t x = r / nbpr, b = r - x * nbpr;
+  const demux_re f = demux_classify(s, x);
+  const uint32_t vz = f.punctured ? 0u : v; // a 1- or 2-bit HARQ-ACK RE reads zero after extraction
+  if (f.ack != kDemuxNone) {
+    store_unit(ack + s.ack_out + (uint64_t)f.ack * nbpr + b,
+               demux_placeholder(v, r, u, s.qm, s.ph_ack, s.x1, s.x2, tab), u);
+  }
+  if (f.c1 != kDemuxNone) {
+    store_unit(c1 + s.c1_out + (uint64_t)f.c1 * nbpr + b, demux_placeholder(v, r, u, s.qm, s.ph_c1, s.x1, s.x2, tab),
+               u);
+  }
+  if (f.c2 != kDemuxNone) {
+    store_unit(c2 + s.c2_out + (uint64_t)f.c2 * nbpr + b,
+               demux_placeholder(vz, r, u, s.qm, s.ph_c2, s.x1, s.x2, tab), u);
+  }
+  if (f.sch != kDemuxNone) {
+    store_unit(sch + s.sch_out + (uint64_t)f.sch * nbpr + b, vz, u);
+  }
+}
+
 } // namespace
+
+hipError_t launch_demux(const demux_tile* tiles, unsigned nof_tiles, const demux_sym* syms, const uint32_t* tab,
+                        const int8_t* llrs, int8_t* sch, int8_t* ack, int8_t* c1, int8_t* c2, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_ulsch_demux, dim3(nof_tiles), dim3(kDemuxThreads), 0, stream, tiles, syms, tab, llrs, sch, ack,
+                     c1, c2);
+  return hipGetLastError();
+}
 
 hipError_t launch_demod(const demod_tile* tiles, unsigned nof_tiles, const demod_job* jobs, const uint32_t* tab,
                         const float* symbols, const float* noise, int8_t* llrs, hipStream_t stream)

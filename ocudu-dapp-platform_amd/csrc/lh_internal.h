@@ -3,6 +3,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "ocudu_hip_ldpc.h"
+
 namespace lh {
 
 constexpr int kMaxZ = 384;
@@ -207,6 +209,44 @@ struct demod_tile {
 // LFSR states at the first sequence bit of each of a block's tiles.
 void demod_tile_states(uint32_t c_init, uint64_t pos, int qm, uint32_t nof_tiles, uint32_t* x1, uint32_t* x2);
 void build_demod_tables(uint32_t* out);
+
+// UL-SCH demultiplexing: one record per OFDM symbol of a codeword (TS38.212 6.2.7 field
+// spacings of that symbol), one tile of kDemuxTileUnits units of `unit` bytes of one symbol
+// per workgroup.  Field sets of a symbol with M REs (RE indices 0..M-1), each "every d-th
+// element of a base set, n of them":
+//   R  reserved (HARQ-ACK <= 2 bits): every dr-th of all REs
+//   A  HARQ-ACK > 2 bits: every da-th of all REs
+//   C1 CSI Part 1: every d1-th of the REs in neither A nor R
+//   C2 CSI Part 2: every d2-th of the REs in neither A nor C1
+//   S  HARQ-ACK <= 2 bits: every ds-th of R (S stays in UL-SCH / C2 with zero soft bits)
+//   UL-SCH: the REs in none of A, C1, C2.
+constexpr int kDemuxThreads = 256;
+constexpr int kDemuxTileUnits = 256;
+struct demux_sym {
+  uint64_t llr_in;  // first soft bit of the symbol in d_llrs
+  uint64_t sch_out; // first UL-SCH soft bit of the symbol in d_sch
+  uint64_t ack_out; // and of its HARQ-ACK, CSI Part 1 and CSI Part 2 soft bits
+  uint64_t c1_out;
+  uint64_t c2_out;
+  uint32_t M;       // REs of the symbol
+  uint32_t x1, x2;  // LFSR states at the symbol's first soft bit (Nc included)
+  uint32_t lcm_ar;  // A and R share the multiples of lcm(da, dr) below lim_ar
+  uint32_t lim_ar;
+  uint16_t nbpr;    // soft bits per RE (bits per symbol x layers)
+  uint8_t qm;
+  uint8_t unit;     // bytes per thread: 1, 2 or 4
+  uint8_t ph_ack, ph_c1, ph_c2; // payload bits (1 or 2) of a field with placeholders, else 0
+  uint8_t plain;    // no field but UL-SCH in the symbol
+  uint16_t dr, nr, da, na, d1, n1, d2, n2, ds, ns;
+};
+struct demux_tile {
+  uint32_t sym;
+  uint32_t tile;
+};
+// Per-symbol records of one codeword; returns 0, or -1 with *why set.  syms may be null
+// (validation and sizes only); it receives at most 14 records.
+int demux_plan(const lh_ulsch_demux_desc& d, demux_sym* syms, unsigned* nof_syms, lh_ulsch_demux_sizes* sizes,
+               const char** why);
 
 inline int crc_index(uint32_t poly)
 {

@@ -193,6 +193,29 @@ class DemodDesc(C.Structure):
                 ("modulation", C.c_uint32), ("c_init", C.c_uint32), ("seq_offset", C.c_uint32)]
 
 
+DEMUX_FIELDS = ("modulation", "nof_layers", "nof_prb", "start_symbol_index", "nof_symbols", "dmrs_symbol_mask",
+                "dmrs_type", "nof_cdm_groups_without_data", "nof_harq_ack_rvd", "nof_harq_ack_bits",
+                "nof_enc_harq_ack_bits", "nof_csi_part1_bits", "nof_enc_csi_part1_bits", "nof_csi_part2_bits",
+                "nof_enc_csi_part2_bits", "c_init")
+
+
+class UlschDemuxDesc(C.Structure):
+    """One codeword of lh_ulsch_demultiplex (lh_ulsch_demux_desc)."""
+    _fields_ = [(f, C.c_uint64) for f in ("llr_offset", "sch_offset", "harq_ack_offset", "csi_part1_offset",
+                                           "csi_part2_offset")] + [(f, C.c_uint32) for f in DEMUX_FIELDS]
+
+
+class UlschDemuxSizes(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("nof_llrs", "nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")]
+
+
+def ulsch_demux_sizes(desc):
+    """Soft-bit counts of one codeword's demultiplexed fields (lh_ulsch_demux_get_sizes)."""
+    sz = UlschDemuxSizes()
+    _lh(lib().lh_ulsch_demux_get_sizes(C.byref(desc), C.byref(sz)), "lh_ulsch_demux_get_sizes")
+    return {f: getattr(sz, f) for f, _ in UlschDemuxSizes._fields_}
+
+
 LH_MOD_PI_2_BPSK, LH_MOD_BPSK, LH_MOD_QPSK, LH_MOD_QAM16, LH_MOD_QAM64, LH_MOD_QAM256 = 0, 1, 2, 4, 6, 8
 
 LH_GEN_MSG_INPUT = 1
@@ -263,7 +286,8 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_nms_adam", "lh_slot_graph_build", "lh_slot_graph_launch", "lh_slot_graph_destroy", "lh_tune_set",
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
             "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch",
-            "lh_pusch_decode_batch_noalloc", "lh_server_run_slot", "lh_server_serve"]
+            "lh_pusch_decode_batch_noalloc", "lh_server_run_slot", "lh_server_serve", "lh_ulsch_demux_get_sizes",
+            "lh_ulsch_demultiplex"]
 
 
 def lib():
@@ -323,6 +347,9 @@ def lib():
                                       C.POINTER(TbResult), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lh_pusch_demodulate.argtypes = [C.c_void_p, C.POINTER(DemodDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]
+        L.lh_ulsch_demux_get_sizes.argtypes = [C.POINTER(UlschDemuxDesc), C.POINTER(UlschDemuxSizes)]
+        L.lh_ulsch_demultiplex.argtypes = [C.c_void_p, C.POINTER(UlschDemuxDesc), C.c_uint, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                                 C.c_void_p, C.c_void_p]
         L.lh_gen_codeblocks.argtypes = [C.c_void_p, C.POINTER(GenDesc), C.c_uint, C.c_void_p, C.c_void_p,
@@ -408,6 +435,18 @@ class Context:
         arr = (DemodDesc * len(descs))(*descs)
         _lh(lib().lh_pusch_demodulate(self.handle, arr, len(descs), d_symbols.ptr, d_noise_vars.ptr, d_llrs.ptr,
                                       stream.handle if stream else None), "lh_pusch_demodulate")
+
+    def ulsch_demultiplex(self, descs, d_llrs, d_sch=None, d_harq_ack=None, d_csi_part1=None, d_csi_part2=None,
+                          stream=None):
+        """UL-SCH demultiplexing of whole descrambled codewords (lh_ulsch_demultiplex)."""
+        arr = (UlschDemuxDesc * len(descs))(*descs)
+
+        def ptr(b):
+            return b.ptr if b is not None else None
+
+        _lh(lib().lh_ulsch_demultiplex(self.handle, arr, len(descs), d_llrs.ptr, ptr(d_sch), ptr(d_harq_ack),
+                                       ptr(d_csi_part1), ptr(d_csi_part2), stream.handle if stream else None),
+            "lh_ulsch_demultiplex")
 
     def gen_codeblocks(self, descs, d_llrs, d_msgs=None, stream=None):
         arr = (GenDesc * len(descs))(*descs)

@@ -1229,3 +1229,266 @@ void orc_pusch_demodulate(int8_t* llrs, const float* iq, const float* nv, unsign
   }
   free(c);
 }
+
+/* ================================================================================= */
+/* f1: UL-SCH demultiplexing (TS38.212 6.2.7), ulsch_demultiplex_impl.cpp            */
+/* ================================================================================= */
+
+/* Per-RE membership flags of one OFDM symbol. */
+enum { DMX_SCH = 1, DMX_UCI = 2, DMX_RVD = 4, DMX_ACK = 8, DMX_C1 = 16, DMX_C2 = 32 };
+
+/* Spreading of one field over `avail` REs with `remainder` REs still to place
+ * (ulsch_demultiplex_impl.cpp:343-349 and the same lines of every step): all REs at step 1
+ * while they do not cover the remainder, else every (avail / remainder)-th. */
+static void dmx_spread(unsigned remainder, unsigned avail, unsigned* d, unsigned* m)
+{
+  *d = 1;
+  *m = avail;
+  if (remainder < avail) {
+    *d = avail / remainder;
+    *m = remainder;
+  }
+}
+
+/* re_set_select (ulsch_demultiplex_impl.cpp:55-78): among the REs holding every flag of
+ * `in` and none of `out`, the ones of rank 0, d, 2d, ... (m of them) get `flag`.  Returns
+ * 0, or -1 when fewer than m exist (the reference asserts). */
+static int dmx_select(uint8_t* re, unsigned M, uint8_t in, uint8_t out, unsigned d, unsigned m, uint8_t flag)
+{
+  unsigned rank = 0, got = 0;
+  for (unsigned x = 0; x < M && got < m; ++x) {
+    if ((re[x] & in) != in || (re[x] & out) != 0) {
+      continue;
+    }
+    if (rank % d == 0) {
+      re[x] |= flag;
+      ++got;
+    }
+    ++rank;
+  }
+  return got == m ? 0 : -1;
+}
+
+static void dmx_clear(uint8_t* re, unsigned M, uint8_t flag, uint8_t clear)
+{
+  for (unsigned x = 0; x < M; ++x) {
+    if (re[x] & flag) {
+      re[x] &= (uint8_t)~clear;
+    }
+  }
+}
+
+/* UCI placeholders of a 1- or 2-bit payload (on_uci_placeholder_1bit / _2bit,
+ * ulsch_demultiplex_impl.cpp:86-176), per modulation symbol of one RE's nbpr soft bits:
+ * bit 0 kept; bit 1 kept (2 bits) or negated where c(n) ^ c(n + 1) = 1 (1 bit, the "y"
+ * placeholder); bits 2.. negated where their own sequence bit is 1 (the "x" placeholders,
+ * the descrambling undone).  c = the sequence bits of these soft bits. */
+static void dmx_placeholder(int8_t* out, const int8_t* in, const uint8_t* c, unsigned nbpr, unsigned qm,
+                            unsigned payload_bits)
+{
+  if (qm == 1 || payload_bits > 2) {
+    memcpy(out, in, nbpr);
+    return;
+  }
+  for (unsigned s = 0; s < nbpr; s += qm) {
+    out[s] = in[s];
+    out[s + 1] = (payload_bits == 1 && (c[s] ^ c[s + 1])) ? (int8_t)-in[s + 1] : in[s + 1];
+    for (unsigned b = 2; b < qm; ++b) {
+      out[s + b] = c[s + b] ? (int8_t)-in[s + b] : in[s + b];
+    }
+  }
+}
+
+/* The allocation of ulsch_demultiplex_impl::configure_current_ofdm_symbol (:313-429), with
+ * CSI Part 2 configured before the first symbol (set_csi_part2 right after demultiplex, the
+ * reference's own resident index builder pusch_demodulator_gpu_impl.cpp:471-625 does the
+ * same), and the extraction of demux_current_ofdm_symbol (:457-573): HARQ-ACK first (REs
+ * of a 1- or 2-bit payload are zeroed after extraction, so the UL-SCH and CSI Part 2 soft
+ * bits of those REs read zero), then CSI Part 1, CSI Part 2 and UL-SCH, each in RE order.
+ * llrs = the descrambled codeword (sum over symbols of REs * nbpr soft bits); c_init =
+ * its scrambling initialisation.  Output pointers may be NULL to only count.  Returns 0,
+ * or -1 for a configuration the reference rejects with an assertion. */
+int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch,
+                          int8_t* harq_ack, int8_t* csi1, int8_t* csi2, orc_demux_sizes* sizes)
+{
+  const int mod = cfg->modulation;
+  if (mod != 0 && mod != 1 && mod != 2 && mod != 4 && mod != 6 && mod != 8) {
+    return -1;
+  }
+  const unsigned qm = mod == 0 ? 1u : (unsigned)mod;
+  const unsigned nbpr = qm * cfg->nof_layers;
+  if (cfg->nof_layers < 1 || cfg->nof_layers > 4 || cfg->nof_prb == 0 || cfg->nof_prb > 275) {
+    return -1;
+  }
+  if (cfg->start_symbol_index + cfg->nof_symbols > 14 || cfg->nof_symbols == 0) {
+    return -1;
+  }
+  const unsigned dmask = cfg->dmrs_symbol_mask & 0x3fffu;
+  const unsigned max_cdm = cfg->dmrs_type == 1 ? 2u : 3u;
+  if ((cfg->dmrs_type != 1 && cfg->dmrs_type != 2) || cfg->nof_cdm_groups_without_data < 1 ||
+      cfg->nof_cdm_groups_without_data > max_cdm || dmask == 0) {
+    return -1;
+  }
+  /* get_ulsch_demultiplex_l1 / _l1_csi / _nof_re_prb_dmrs (:15-53). */
+  unsigned first_dmrs = 0;
+  while (!((dmask >> first_dmrs) & 1u)) {
+    ++first_dmrs;
+  }
+  unsigned l1 = first_dmrs;
+  while (l1 < 14 && ((dmask >> l1) & 1u)) {
+    ++l1;
+  }
+  unsigned l1_csi = 0;
+  while (l1_csi < 14 && ((dmask >> l1_csi) & 1u)) {
+    ++l1_csi;
+  }
+  if (l1 >= 14 || l1_csi >= 14) {
+    return -1;
+  }
+  const unsigned re_dmrs_prb = cfg->nof_cdm_groups_without_data * (cfg->dmrs_type == 1 ? 6u : 4u);
+  const unsigned M_dmrs = (12u - re_dmrs_prb) * cfg->nof_prb;
+  const unsigned ack_bits = cfg->nof_harq_ack_bits, c1_bits = cfg->nof_csi_part1_bits,
+                 c2_bits = cfg->nof_csi_part2_bits;
+  const unsigned G_ack = cfg->nof_enc_harq_ack_bits, G_c1 = cfg->nof_enc_csi_part1_bits,
+                 G_c2 = cfg->nof_enc_csi_part2_bits, G_rvd = cfg->nof_harq_ack_rvd;
+  /* A payload without encoded bits (or the reverse) never completes (:302-308). */
+  if ((ack_bits == 0) != (G_ack == 0) || (c1_bits == 0) != (G_c1 == 0) || (c2_bits == 0) != (G_c2 == 0)) {
+    return -1;
+  }
+  if (G_ack % nbpr || G_c1 % nbpr || G_c2 % nbpr) {
+    return -1;
+  }
+  uint8_t re[275 * 12];
+  int8_t tmp[32];
+  /* The codeword's scrambling sequence, for the placeholders. */
+  size_t total = 0;
+  for (unsigned l = cfg->start_symbol_index; l < cfg->start_symbol_index + cfg->nof_symbols; ++l) {
+    total += (size_t)(((dmask >> l) & 1u) ? M_dmrs : cfg->nof_prb * 12u) * nbpr;
+  }
+  uint8_t* seq = NULL;
+  if (llrs != NULL && (harq_ack != NULL || csi1 != NULL || csi2 != NULL)) {
+    seq = (uint8_t*)malloc(total + 1);
+    orc_scrambling_sequence(seq, c_init, 0, (unsigned)total);
+  }
+  int rc = -1;
+  unsigned m_rvd = 0, m_ack = 0, m_c1 = 0, m_c2 = 0;
+  size_t q = 0, n_sch = 0, n_ack = 0, n_c1 = 0, n_c2 = 0;
+  for (unsigned l = cfg->start_symbol_index; l < cfg->start_symbol_index + cfg->nof_symbols; ++l) {
+    const int dm = (dmask >> l) & 1u;
+    const unsigned M = dm ? M_dmrs : cfg->nof_prb * 12u;
+    if (M == 0) {
+      continue;
+    }
+    for (unsigned x = 0; x < M; ++x) {
+      re[x] = (uint8_t)(DMX_SCH | (dm ? 0 : DMX_UCI));
+    }
+    unsigned M_uci = dm ? 0 : M, d, m;
+    /* Step 1: reserved REs for a HARQ-ACK of up to 2 bits, from the UL-SCH set. */
+    const unsigned rem_rvd = (G_rvd - m_rvd) / nbpr;
+    if (l >= l1 && M_uci > 0 && rem_rvd > 0) {
+      dmx_spread(rem_rvd, M_uci, &d, &m);
+      if (dmx_select(re, M, DMX_SCH, 0, d, m, DMX_RVD)) {
+        goto done;
+      }
+      m_rvd += m * nbpr;
+    }
+    /* Step 2: HARQ-ACK of more than 2 bits, out of UL-SCH and UCI. */
+    const unsigned rem_ack = (G_ack - m_ack) / nbpr;
+    if (l >= l1 && M_uci > 0 && ack_bits > 2 && rem_ack > 0) {
+      dmx_spread(rem_ack, M_uci, &d, &m);
+      if (dmx_select(re, M, DMX_UCI, 0, d, m, DMX_ACK)) {
+        goto done;
+      }
+      dmx_clear(re, M, DMX_ACK, DMX_SCH | DMX_UCI);
+      M_uci -= m;
+      m_ack += m * nbpr;
+    }
+    /* Step 3: CSI Part 1 on UCI REs that are not reserved. */
+    const unsigned rem_c1 = (G_c1 - m_c1) / nbpr;
+    unsigned M_rvd = 0;
+    for (unsigned x = 0; x < M; ++x) {
+      M_rvd += (re[x] & DMX_RVD) != 0;
+    }
+    if (l >= l1_csi && M_uci != M_rvd && rem_c1 > 0) {
+      if (M_uci < M_rvd) {
+        goto done; /* the reference's unsigned difference wraps and its selection asserts */
+      }
+      dmx_spread(rem_c1, M_uci - M_rvd, &d, &m);
+      if (dmx_select(re, M, DMX_UCI, DMX_RVD, d, m, DMX_C1)) {
+        goto done;
+      }
+      dmx_clear(re, M, DMX_C1, DMX_SCH | DMX_UCI);
+      M_uci -= m;
+      m_c1 += m * nbpr;
+    }
+    /* Step 3bis: CSI Part 2 on the UCI REs left (reserved ones included). */
+    const unsigned rem_c2 = (G_c2 - m_c2) / nbpr;
+    if (l >= l1_csi && M_uci > 0 && rem_c2 > 0) {
+      dmx_spread(rem_c2, M_uci, &d, &m);
+      if (dmx_select(re, M, DMX_UCI, 0, d, m, DMX_C2)) {
+        goto done;
+      }
+      dmx_clear(re, M, DMX_C2, DMX_SCH | DMX_UCI);
+      m_c2 += m * nbpr;
+    }
+    /* Step 5: HARQ-ACK of up to 2 bits on reserved REs, which stay in UL-SCH / CSI Part 2. */
+    if (M_rvd > 0 && ack_bits <= 2 && rem_ack > 0) {
+      dmx_spread(rem_ack, M_rvd, &d, &m);
+      if (dmx_select(re, M, DMX_RVD, 0, d, m, DMX_ACK)) {
+        goto done;
+      }
+      m_ack += m * nbpr;
+    }
+    /* Extraction. */
+    const int zero_ack = ack_bits <= 2;
+    for (unsigned x = 0; x < M; ++x) {
+      const int8_t* in = llrs ? llrs + q + (size_t)x * nbpr : NULL;
+      const int punct = zero_ack && (re[x] & DMX_ACK);
+      if (re[x] & DMX_ACK) {
+        if (harq_ack) {
+          dmx_placeholder(harq_ack + n_ack, in, seq + q + (size_t)x * nbpr, nbpr, qm, ack_bits);
+        }
+        n_ack += nbpr;
+      }
+      if (re[x] & DMX_C1) {
+        if (csi1) {
+          dmx_placeholder(csi1 + n_c1, in, seq + q + (size_t)x * nbpr, nbpr, qm, c1_bits);
+        }
+        n_c1 += nbpr;
+      }
+      if (re[x] & DMX_C2) {
+        if (csi2) {
+          memset(tmp, 0, sizeof(tmp));
+          dmx_placeholder(csi2 + n_c2, punct ? tmp : in, seq + q + (size_t)x * nbpr, nbpr, qm, c2_bits);
+        }
+        n_c2 += nbpr;
+      }
+      if (re[x] & DMX_SCH) {
+        if (sch) {
+          if (punct) {
+            memset(sch + n_sch, 0, nbpr);
+          } else {
+            memcpy(sch + n_sch, in, nbpr);
+          }
+        }
+        n_sch += nbpr;
+      }
+    }
+    q += (size_t)M * nbpr;
+  }
+  /* on_end_codeword (:302-308): every UCI field complete. */
+  if (m_ack != G_ack || m_c1 != G_c1 || m_c2 != G_c2) {
+    goto done;
+  }
+  if (sizes) {
+    sizes->nof_llrs = (uint32_t)q;
+    sizes->nof_sch = (uint32_t)n_sch;
+    sizes->nof_harq_ack = (uint32_t)n_ack;
+    sizes->nof_csi_part1 = (uint32_t)n_c1;
+    sizes->nof_csi_part2 = (uint32_t)n_c2;
+  }
+  rc = 0;
+done:
+  free(seq);
+  return rc;
+}

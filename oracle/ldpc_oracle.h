@@ -145,6 +145,38 @@ void orc_demodulate_soft(int8_t* llrs, const float* iq, const float* nv, unsigne
 void orc_pusch_demodulate(int8_t* llrs, const float* iq, const float* nv, unsigned nsym, int modulation,
                           uint32_t c_init, unsigned seq_offset);
 
+/* f1: UL-SCH demultiplexing (ulsch_demultiplex::configuration,
+ * include/ocudu/phy/upper/channel_processors/pusch/ulsch_demultiplex.h:35-66, with CSI
+ * Part 2 given up front). */
+typedef struct {
+  int modulation;                       /* 0 pi/2-BPSK, 1, 2, 4, 6, 8 bits per symbol */
+  unsigned nof_layers;                  /* 1..4 */
+  unsigned nof_prb;
+  unsigned start_symbol_index;
+  unsigned nof_symbols;
+  unsigned dmrs_symbol_mask;            /* bit l: OFDM symbol l carries DM-RS */
+  unsigned dmrs_type;                   /* 1 or 2 */
+  unsigned nof_cdm_groups_without_data; /* 1..2 (type 1), 1..3 (type 2) */
+  unsigned nof_harq_ack_rvd;            /* G_rvd^ACK */
+  unsigned nof_harq_ack_bits;           /* O_ACK */
+  unsigned nof_enc_harq_ack_bits;       /* G^ACK */
+  unsigned nof_csi_part1_bits;
+  unsigned nof_enc_csi_part1_bits;
+  unsigned nof_csi_part2_bits;
+  unsigned nof_enc_csi_part2_bits;
+} orc_demux_config;
+
+typedef struct {
+  uint32_t nof_llrs; /* codeword soft bits in */
+  uint32_t nof_sch;
+  uint32_t nof_harq_ack;
+  uint32_t nof_csi_part1;
+  uint32_t nof_csi_part2;
+} orc_demux_sizes;
+
+int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch,
+                          int8_t* harq_ack, int8_t* csi1, int8_t* csi2, orc_demux_sizes* sizes);
+
 #ifdef __cplusplus
 }
 #endif

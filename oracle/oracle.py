@@ -378,6 +378,8 @@ def ref_demod():
         _fp = C.POINTER(C.c_float)
         R.ref_demodulate_soft.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int]
         R.ref_scrambling_sequence.argtypes = [_u8p, C.c_uint32, C.c_uint, C.c_uint]
+        R.ref_ulsch_demultiplex.argtypes = [C.POINTER(C.c_uint), _i8p, C.c_uint, C.c_uint32, C.c_uint, _i8p, _i8p,
+                                            _i8p, _i8p, C.POINTER(C.c_uint)]
         _ref_demod = R
     return _ref_demod
 
@@ -421,3 +423,66 @@ def pusch_demodulate(iq, nv, modulation, c_init, seq_offset=0):
     L.orc_pusch_demodulate(_p(out, _i8p), _p(iq, C.POINTER(C.c_float)), _p(nv, C.POINTER(C.c_float)), nv.size,
                            modulation, c_init, seq_offset)
     return out
+
+
+# --- f1: UL-SCH demultiplexing ---------------------------------------------------------
+
+DEMUX_FIELDS = ("modulation", "nof_layers", "nof_prb", "start_symbol_index", "nof_symbols", "dmrs_symbol_mask",
+                "dmrs_type", "nof_cdm_groups_without_data", "nof_harq_ack_rvd", "nof_harq_ack_bits",
+                "nof_enc_harq_ack_bits", "nof_csi_part1_bits", "nof_enc_csi_part1_bits", "nof_csi_part2_bits",
+                "nof_enc_csi_part2_bits")
+
+
+class _DemuxCfg(C.Structure):
+    _fields_ = [("modulation", C.c_int)] + [(f, C.c_uint) for f in DEMUX_FIELDS[1:]]
+
+
+class _DemuxSizes(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("nof_llrs", "nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")]
+
+
+def _demux_cfg(cfg):
+    return _DemuxCfg(**{f: int(cfg.get(f, 0)) for f in DEMUX_FIELDS})
+
+
+def demux_sizes(cfg):
+    """Soft-bit counts {nof_llrs, nof_sch, nof_harq_ack, nof_csi_part1, nof_csi_part2} of a
+    demultiplexer configuration, or None when the configuration is invalid."""
+    L = lib()
+    sz = _DemuxSizes()
+    if L.orc_ulsch_demultiplex(C.byref(_demux_cfg(cfg)), None, 0, None, None, None, None, C.byref(sz)) != 0:
+        return None
+    return {f: getattr(sz, f) for f, _ in _DemuxSizes._fields_}
+
+
+def ulsch_demultiplex(cfg, llrs, c_init):
+    """(sch, harq_ack, csi_part1, csi_part2) soft bits of one descrambled codeword."""
+    L = lib()
+    sz = demux_sizes(cfg)
+    if sz is None:
+        raise ValueError("invalid demultiplexer configuration")
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    assert llrs.size == sz["nof_llrs"]
+    outs = [np.zeros(max(sz[k], 1), np.int8) for k in ("nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")]
+    rc = L.orc_ulsch_demultiplex(C.byref(_demux_cfg(cfg)), _p(llrs, _i8p), c_init, *[_p(o, _i8p) for o in outs],
+                                 None)
+    assert rc == 0
+    return tuple(o[:sz[k]] for o, k in zip(outs, ("nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")))
+
+
+def ref_ulsch_demultiplex(cfg, llrs, c_init, block=1000):
+    """The reference's ulsch_demultiplex_impl built from source, fed in blocks of `block`
+    soft bits, CSI Part 2 configured before the first block; None for a configuration
+    whose first symbol has no data REs (the reference's streaming never leaves it)."""
+    R = ref_demod()
+    words = (C.c_uint * 15)(*[int(cfg.get(f, 0)) for f in DEMUX_FIELDS])
+    llrs = np.ascontiguousarray(llrs, dtype=np.int8)
+    n = llrs.size
+    outs = [np.zeros(max(n, 1), np.int8) for _ in range(4)]
+    sizes = (C.c_uint * 4)()
+    rc = R.ref_ulsch_demultiplex(words, _p(llrs, _i8p), n, c_init, block, *[_p(o, _i8p) for o in outs], sizes)
+    if rc == -2:
+        return None  # a first symbol without data REs: the reference's streaming never advances
+    if rc != 0:
+        raise RuntimeError("a decoder buffer did not receive its end")
+    return tuple(o[:sizes[k]] for k, o in enumerate(outs))

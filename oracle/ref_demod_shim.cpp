@@ -1,11 +1,15 @@
-// extern "C" access to the reference's own soft demappers and pseudo-random generator,
-// built from their sources (oracle/Makefile.ref) -- TEST INFRASTRUCTURE ONLY: the checker of
-// the oracle's demodulation and scrambling restatement.  Built without AVX/NEON flags, so
+// extern "C" access to the reference's own soft demappers, pseudo-random generator and
+// UL-SCH demultiplexer, built from their sources (oracle/Makefile.ref) -- TEST
+// INFRASTRUCTURE ONLY: the checker of the oracle's demodulation, scrambling and
+// demultiplexing restatements.  Built without AVX/NEON flags, so
 // the reference's scalar (generic) demapper path runs.
 #include "demodulation_mapper_impl.h"
 #include "pseudo_random_generator_impl.h"
+#include "ulsch_demultiplex_impl.h"
+#include "ocudu/phy/upper/channel_processors/pusch/pusch_decoder_buffer.h"
 #include "ocudu/adt/bit_buffer.h"
 
+#include <algorithm>
 #include <vector>
 
 using namespace ocudu;
@@ -34,6 +38,110 @@ void ref_scrambling_sequence(uint8_t* c, uint32_t c_init, unsigned offset, unsig
   for (unsigned i = 0; i < n; ++i) {
     c[i] = (uint8_t)seq.extract(i, 1);
   }
+}
+
+} // extern "C"
+
+// ---- UL-SCH demultiplexer (ulsch_demultiplex_impl.cpp built from source) -------------
+
+namespace {
+
+// Collects the soft bits handed to one decoder buffer.
+class collecting_buffer : public pusch_decoder_buffer
+{
+public:
+  std::vector<int8_t> bits;
+  bool ended = false;
+  std::vector<log_likelihood_ratio> view;
+
+  span<log_likelihood_ratio> get_next_block_view(unsigned block_size) override
+  {
+    view.resize(block_size);
+    return view;
+  }
+  void on_new_softbits(span<const log_likelihood_ratio> softbits) override
+  {
+    for (const log_likelihood_ratio& v : softbits) {
+      bits.push_back(static_cast<int8_t>(v));
+    }
+  }
+  void on_end_softbits() override { ended = true; }
+};
+
+} // namespace
+
+extern "C" {
+
+// ulsch_demultiplex::demultiplex, then set_csi_part2 before any soft bits (when CSI Part 2
+// is present), then the codeword in blocks of `block` soft bits (the last one shorter) with
+// the sequence of pseudo_random_generator(c_init).  cfg: modulation, layers, PRBs, start
+// symbol, symbols, DM-RS symbol mask, DM-RS type, CDM groups without data, G_rvd, O_ACK,
+// G_ACK, O_CSI1, G_CSI1, O_CSI2, G_CSI2.  Writes the four outputs and their sizes
+// (sizes[0..3]); returns 0, -1 when a buffer did not receive its end, -2 for a first
+// symbol without data REs.
+int ref_ulsch_demultiplex(const unsigned* cfg, const int8_t* llrs, unsigned nof_llrs, uint32_t c_init,
+                          unsigned block, int8_t* sch, int8_t* ack, int8_t* csi1, int8_t* csi2, unsigned* sizes)
+{
+  ulsch_demultiplex::configuration c;
+  c.modulation = static_cast<modulation_scheme>(cfg[0]);
+  c.nof_layers = cfg[1];
+  c.nof_prb = cfg[2];
+  c.start_symbol_index = cfg[3];
+  c.nof_symbols = cfg[4];
+  c.dmrs_symbol_mask = symbol_slot_mask(14);
+  for (unsigned l = 0; l < 14; ++l) {
+    if ((cfg[5] >> l) & 1u) {
+      c.dmrs_symbol_mask.set(l);
+    }
+  }
+  c.dmrs = cfg[6] == 1 ? dmrs_config_type::type1 : dmrs_config_type::type2;
+  c.nof_cdm_groups_without_data = cfg[7];
+  c.nof_harq_ack_rvd = cfg[8];
+  c.nof_harq_ack_bits = cfg[9];
+  c.nof_enc_harq_ack_bits = cfg[10];
+  c.nof_csi_part1_bits = cfg[11];
+  c.nof_enc_csi_part1_bits = cfg[12];
+
+  // A first symbol without data REs (DM-RS with every CDM group empty of data) leaves the
+  // reference's codeword buffer at a zero block size forever (on_new_block,
+  // ulsch_demultiplex_impl.cpp:262-300): not fed to it.
+  const bool first_dmrs = (cfg[5] >> cfg[3]) & 1u;
+  if (first_dmrs && cfg[7] * (cfg[6] == 1 ? 6u : 4u) == 12u) {
+    return -2;
+  }
+  static ulsch_demultiplex_impl demux;
+  collecting_buffer b_sch, b_ack, b_c1, b_c2;
+  pusch_codeword_buffer& cw = demux.demultiplex(b_sch, b_ack, b_c1, c);
+  if (cfg[13] != 0) {
+    demux.set_csi_part2(b_c2, cfg[13], cfg[14]);
+  }
+  pseudo_random_generator_impl prg;
+  prg.init(c_init);
+  for (unsigned off = 0; off < nof_llrs;) {
+    const unsigned n = std::min(block, nof_llrs - off);
+    span<log_likelihood_ratio> dst = cw.get_next_block_view(n);
+    const unsigned m = std::min(n, (unsigned)dst.size());
+    for (unsigned i = 0; i < m; ++i) {
+      dst[i] = log_likelihood_ratio(llrs[off + i]);
+    }
+    dynamic_bit_buffer seq(m);
+    prg.generate(seq);
+    cw.on_new_block(dst.first(m), seq);
+    off += m;
+  }
+  cw.on_end_codeword();
+  const std::vector<int8_t>* outs[4] = {&b_sch.bits, &b_ack.bits, &b_c1.bits, &b_c2.bits};
+  int8_t* dsts[4] = {sch, ack, csi1, csi2};
+  for (int k = 0; k < 4; ++k) {
+    sizes[k] = (unsigned)outs[k]->size();
+    if (dsts[k] != nullptr) {
+      std::copy(outs[k]->begin(), outs[k]->end(), dsts[k]);
+    }
+  }
+  if (!b_sch.ended || (cfg[10] && !b_ack.ended) || (cfg[12] && !b_c1.ended) || (cfg[14] && !b_c2.ended)) {
+    return -1;
+  }
+  return 0;
 }
 
 } // extern "C"

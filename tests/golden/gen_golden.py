@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Writes the decoder golden fixtures from the reference's own generic decoder.
+"""Writes the golden fixtures from the reference's own code built from source: the
+generic decoder's, and the UL-SCH demultiplexer's (ulsch_demux).
 
 The reference CPU code is built from its sources into oracle/_ref/libref_ldpc.so
 (oracle/Makefile.ref); this script feeds it seeded synthetic codeblocks and stores the
@@ -9,6 +10,8 @@ fixtures are data only; the tests compare the oracle and the HIP path against th
 
     python tests/golden/gen_golden.py              # every fixture
     python tests/golden/gen_golden.py high_rate    # only high_rate_l4.npz
+    python tests/golden/gen_golden.py demux        # only ulsch_demux.npz (the reference's
+                                                   # UL-SCH demultiplexer, libref_demod.so)
 """
 import sys
 import zlib
@@ -72,7 +75,42 @@ def high_rate(dec):
     _pack(rows, llr_blob, bits_blob, HERE / "high_rate_l4.npz", "bg Z E Qm rv F crc_poly crc_bits iters rm_crc32 snr_cdb")
 
 
+def ulsch_demux():
+    """UL-SCH demultiplexing by the reference's ulsch_demultiplex_impl built from source
+    (oracle/_ref/libref_demod.so): 48 seeded configurations (tests/demux_util.py), small
+    PRB counts to keep the fixture small, int8 codeword soft bits from a seeded generator;
+    stored: the configuration words (+ c_init), the input and the four outputs."""
+    sys.path.insert(0, str(HERE.parent))
+    from demux_util import config_words, random_config
+
+    rng = np.random.default_rng(20260301)
+    words, llr, outs = [], [], {k: [] for k in ("sch", "harq_ack", "csi_part1", "csi_part2")}
+    while len(words) < 48:
+        cfg = random_config(rng, max_prb=6)
+        sz = O.demux_sizes(cfg)
+        x = rng.integers(-120, 121, sz["nof_llrs"]).astype(np.int8)
+        c_init = int(rng.integers(0, 2 ** 31))
+        got = O.ref_ulsch_demultiplex(cfg, x, c_init, block=int(rng.integers(1, 3000)))
+        if got is None:
+            continue
+        words.append(config_words(cfg, c_init))
+        llr.append(x)
+        for k, v in zip(outs, got):
+            outs[k].append(v)
+    arrays = {"config": np.array(words, np.int64), "llr": np.concatenate(llr),
+              "llr_off": np.cumsum([0] + [a.size for a in llr])}
+    for k, v in outs.items():
+        arrays[k] = np.concatenate(v)
+        arrays[k + "_off"] = np.cumsum([0] + [a.size for a in v])
+    np.savez_compressed(HERE / "ulsch_demux.npz", **arrays)
+
+
 def main():
+    if sys.argv[1:] == ["demux"]:
+        if O.ref_demod() is None:
+            raise SystemExit("build oracle/_ref first (make -f oracle/Makefile.ref)")
+        ulsch_demux()
+        return
     R = O.ref()
     if R is None:
         raise SystemExit("build oracle/_ref first (make -f oracle/Makefile.ref)")
@@ -118,6 +156,7 @@ def main():
           "bg Z E Qm rv F crc_poly crc_bits iters rm_crc32")
     high_rate(dec)
     R.ref_decoder_destroy(dec)
+    ulsch_demux()
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
 

@@ -1,7 +1,7 @@
 """Soft demodulation + descrambling on the device (lh_pusch_demodulate) against the
 oracle (orc_pusch_demodulate, pinned to the reference's own scalar demappers and
 pseudo-random generator in test_oracle_demod.py), bit-exact: every modulation, blocks of
-several 1024-symbol tiles with ragged ends, blocks continuing a sequence (seq_offset),
+several 256-symbol tiles with ragged ends, blocks continuing a sequence (seq_offset),
 special noise variances (0, negative, NaN), near-zero symbols and symbols on the demapper's
 interval boundaries, and one full 4-layer 256-QAM slot."""
 import numpy as np
