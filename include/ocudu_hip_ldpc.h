@@ -373,6 +373,55 @@ typedef struct lh_ulsch_demux_sizes {
   uint32_t nof_csi_part2;
 } lh_ulsch_demux_sizes;
 
+/* ulsch_configuration (include/ocudu/ran/pusch/ulsch_info.h:21-60). */
+typedef struct lh_ulsch_config {
+  uint32_t tbs_bits;           /* 0: no UL-SCH (UCI only) */
+  float target_code_rate;      /* normalised R; up to 0.25 (0 included) selects BG2 for the
+                                  codeblock sizes, as get_ldpc_base_graph does */
+  uint32_t modulation;         /* LH_MOD_* */
+  uint32_t nof_harq_ack_bits;
+  uint32_t nof_csi_part1_bits;
+  uint32_t nof_csi_part2_bits;
+  float alpha_scaling;
+  float beta_offset_harq_ack;
+  float beta_offset_csi_part1;
+  float beta_offset_csi_part2;
+  uint32_t nof_rb;
+  uint32_t start_symbol_index;
+  uint32_t nof_symbols;
+  uint32_t dmrs_type;          /* 1 or 2 */
+  uint32_t dmrs_symbol_mask;   /* bit l: OFDM symbol l carries DM-RS */
+  uint32_t nof_cdm_groups_without_data;
+  uint32_t nof_layers;
+  int32_t contains_dc;
+} lh_ulsch_config;
+
+/* ulsch_information with its sch_information (include/ocudu/ran/sch/sch_segmentation.h:17-31). */
+typedef struct lh_ulsch_info {
+  uint32_t sch_base_graph;     /* 0 without a TB */
+  uint32_t sch_tb_crc_size;
+  uint32_t sch_nof_cb;
+  uint32_t sch_lifting_size;
+  uint32_t sch_nof_bits_per_cb;
+  uint32_t sch_nof_filler_bits_per_cb;
+  uint32_t nof_ul_sch_bits;
+  uint32_t nof_harq_ack_bits;  /* G^ACK */
+  uint32_t nof_harq_ack_rvd;   /* G_rvd^ACK */
+  uint32_t nof_csi_part1_bits; /* G^CSI-1 */
+  uint32_t nof_csi_part2_bits; /* G^CSI-2 */
+  uint32_t nof_harq_ack_re;
+  uint32_t nof_csi_part1_re;
+  uint32_t nof_csi_part2_re;
+  uint32_t nof_dc_overlap_bits;
+  uint32_t pad;
+} lh_ulsch_info;
+
+/* get_ulsch_information (lib/ran/pusch/ulsch_info.cpp:148-303, with get_sch_segmentation_info,
+ * lib/ran/sch/sch_segmentation.cpp:11-42): the UL-SCH / HARQ-ACK / CSI bit accounting of a
+ * grant, in the reference's float arithmetic (host only, no context).  Rejects what the
+ * reference asserts on (CDM groups, DM-RS symbols outside the allocation). */
+int lh_ulsch_information(const lh_ulsch_config* cfg, lh_ulsch_info* info);
+
 /* Sizes of one codeword's demultiplexed fields (host only, no context).  Rejects what the
  * reference rejects with an assertion: a field whose encoded bits are not a multiple of
  * the bits per RE, that does not fit its REs, or whose payload and encoded sizes are not

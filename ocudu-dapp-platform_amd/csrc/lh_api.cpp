@@ -1608,6 +1608,18 @@ int lh_pusch_demodulate(lh_ctx* ctx, const lh_demod_desc* descs, unsigned n, con
   return e == hipSuccess ? LH_OK : hip_fail(e, "demodulator launch");
 }
 
+int lh_ulsch_information(const lh_ulsch_config* cfg, lh_ulsch_info* info)
+{
+  if (cfg == nullptr || info == nullptr) {
+    return fail(LH_ERR_INVALID, "invalid arguments to lh_ulsch_information");
+  }
+  const char* why = nullptr;
+  if (ulsch_information(*cfg, *info, &why) != 0) {
+    return fail(LH_ERR_INVALID, "UL-SCH information: %s", why);
+  }
+  return LH_OK;
+}
+
 int lh_ulsch_demux_get_sizes(const lh_ulsch_demux_desc* desc, lh_ulsch_demux_sizes* sizes)
 {
   if (desc == nullptr || sizes == nullptr) {

@@ -6,6 +6,8 @@
 // uploaded.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "ocudu_hip_ldpc.h"
 #include "lh_internal.h"
 
@@ -213,6 +215,164 @@ int demux_plan(const lh_ulsch_demux_desc& d, demux_sym* syms, unsigned* nof_syms
     sizes->nof_csi_part1 = (uint32_t)o_c1;
     sizes->nof_csi_part2 = (uint32_t)o_c2;
   }
+  return 0;
+}
+
+
+// ---- UL-SCH bit accounting ------------------------------------------------------------
+
+namespace {
+
+// get_uci_crc_size (include/ocudu/ran/uci/uci_info.h:36-47).
+unsigned uci_crc_size(unsigned payload)
+{
+  return payload < 12 ? 0u : (payload < 20 ? 6u : 11u);
+}
+
+// The reference converts std::ceil of a float expression to unsigned.
+unsigned ceil_u(float v)
+{
+  return (unsigned)std::ceil(v);
+}
+
+// calculate_nof_re_harq_ack (lib/ran/pusch/ulsch_info.cpp:12-31) and its UCI-only form (:33-52).
+unsigned harq_ack_res(bool with_sch, unsigned bits, float beta, unsigned nre_uci, unsigned sum_cb, float R, unsigned qm,
+                      float alpha, unsigned nre_l0)
+{
+  if (bits == 0) {
+    return 0;
+  }
+  const float payload = (float)(bits + uci_crc_size(bits));
+  const unsigned left = with_sch ? ceil_u(payload * beta * (float)nre_uci / (float)sum_cb)
+                                 : ceil_u(payload * beta / (R * (float)qm));
+  const unsigned right = ceil_u(alpha * (float)nre_l0);
+  return left < right ? left : right;
+}
+
+// The NR lifting sizes (TS38.212 Table 5.3.2-1), ascending.
+constexpr unsigned kZs[kNofLiftingSizes] = {2,   3,   4,   5,   6,   7,   8,   9,   10,  11,  12,  13,  14,
+                                            15,  16,  18,  20,  22,  24,  26,  28,  30,  32,  36,  40,  44,
+                                            48,  52,  56,  60,  64,  72,  80,  88,  96,  104, 112, 120, 128,
+                                            144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
+
+// get_sch_segmentation_info (lib/ran/sch/sch_segmentation.cpp:11-42): base graph by
+// get_ldpc_base_graph (include/ocudu/ran/sch/ldpc_base_graph.h:20-30), codeblocks, lifting
+// size by compute_lifting_size (sch_segmentation.h:49-95).
+void sch_info(unsigned tbs, float R, lh_ulsch_info& o)
+{
+  const unsigned crc = tbs <= 3824u ? 16u : 24u;
+  const bool bg2 = tbs <= 292u || R <= 0.25F || (tbs <= 3824u && R <= 0.67F);
+  const unsigned B = tbs + crc;
+  const unsigned k_cb = bg2 ? 3840u : 8448u;
+  const unsigned C = B <= k_cb ? 1u : (B + k_cb - 25u) / (k_cb - 24u);
+  const unsigned payload_per_cb = B / C + (C > 1 ? 24u : 0u);
+  unsigned ref = 22;
+  if (bg2) {
+    ref = B > 640u ? 10u : (B > 560u ? 9u : (B > 192u ? 8u : 6u));
+  }
+  const unsigned out_bits = B + (C > 1 ? 24u * C : 0u);
+  unsigned Z = 0;
+  for (unsigned i = 0; i < (unsigned)kNofLiftingSizes && Z == 0; ++i) {
+    if (kZs[i] * ref * C >= out_bits) {
+      Z = kZs[i];
+    }
+  }
+  o.sch_base_graph = bg2 ? 2u : 1u;
+  o.sch_tb_crc_size = crc;
+  o.sch_nof_cb = C;
+  o.sch_lifting_size = Z;
+  o.sch_nof_bits_per_cb = (bg2 ? 10u : 22u) * Z;
+  o.sch_nof_filler_bits_per_cb = o.sch_nof_bits_per_cb - payload_per_cb;
+}
+
+} // namespace
+
+int ulsch_information(const lh_ulsch_config& c, lh_ulsch_info& o, const char** why)
+{
+  o = lh_ulsch_info{};
+  const unsigned mask = c.dmrs_symbol_mask & 0x3fffu;
+  const unsigned mod = c.modulation;
+  if (mod != 0 && mod != 1 && mod != 2 && mod != 4 && mod != 6 && mod != 8) {
+    *why = "modulation is not 0, 1, 2, 4, 6 or 8";
+    return -1;
+  }
+  if ((c.dmrs_type != 1 && c.dmrs_type != 2) || c.nof_cdm_groups_without_data < 1 ||
+      c.nof_cdm_groups_without_data > (c.dmrs_type == 1 ? 2u : 3u)) {
+    *why = "invalid DM-RS type or CDM groups without data";
+    return -1;
+  }
+  if (mask == 0 || c.nof_symbols == 0 || c.start_symbol_index + c.nof_symbols > 14 ||
+      (unsigned)__builtin_ctz(mask) < c.start_symbol_index ||
+      31u - (unsigned)__builtin_clz(mask) >= c.start_symbol_index + c.nof_symbols) {
+    *why = "DM-RS symbols outside the allocation";
+    return -1;
+  }
+  if (c.tbs_bits == 0 && !(c.target_code_rate > 0.0F)) {
+    *why = "a grant without UL-SCH needs a code rate";
+    return -1;
+  }
+  const bool with_sch = c.tbs_bits > 0;
+  if (with_sch) {
+    sch_info(c.tbs_bits, c.target_code_rate, o);
+  }
+  const unsigned nsym_dmrs = (unsigned)__builtin_popcount(mask);
+  const unsigned re_dmrs_rb = nsym_dmrs * c.nof_cdm_groups_without_data * (c.dmrs_type == 1 ? 6u : 4u);
+  const unsigned nre_total = c.nof_rb * (c.nof_symbols * 12u - re_dmrs_rb);
+  const unsigned nre_uci = (c.nof_symbols - nsym_dmrs) * c.nof_rb * 12u;
+  const unsigned qm = mod == 0 ? 1u : mod;
+  unsigned nre_l0 = 0;
+  for (unsigned l = (unsigned)__builtin_ctz(mask); l < c.start_symbol_index + c.nof_symbols; ++l) {
+    nre_l0 += ((mask >> l) & 1u) ? 0u : c.nof_rb * 12u;
+  }
+  const unsigned sum_cb = o.sch_nof_cb * o.sch_nof_bits_per_cb;
+  const float R = c.target_code_rate;
+  const unsigned O_ack = c.nof_harq_ack_bits, O_c1 = c.nof_csi_part1_bits, O_c2 = c.nof_csi_part2_bits;
+  o.nof_harq_ack_re =
+      harq_ack_res(with_sch, O_ack, c.beta_offset_harq_ack, nre_uci, sum_cb, R, qm, c.alpha_scaling, nre_l0);
+  // Reservation sized for two bits when fewer are sent (:225-252).
+  unsigned rvd_re = 0;
+  if (O_ack < 2) {
+    rvd_re = harq_ack_res(with_sch, 2, c.beta_offset_harq_ack, nre_uci, sum_cb, R, qm, c.alpha_scaling, nre_l0);
+  } else if (O_ack == 2) {
+    rvd_re = o.nof_harq_ack_re;
+  }
+  // CSI Part 1 (:56-104) against the reserved REs when HARQ-ACK has at most 2 bits.
+  const unsigned ack_c1 = O_ack <= 2 ? rvd_re : o.nof_harq_ack_re;
+  if (O_c1 != 0) {
+    const float payload = (float)(O_c1 + uci_crc_size(O_c1));
+    unsigned left, right;
+    if (with_sch) {
+      left = ceil_u(payload * c.beta_offset_csi_part1 * (float)nre_uci / (float)sum_cb);
+      right = ceil_u(c.alpha_scaling * (float)nre_uci) - ack_c1;
+    } else if (O_c2 == 0) {
+      left = right = nre_uci - ack_c1;
+    } else {
+      left = ceil_u(payload * c.beta_offset_csi_part1 / (R * (float)qm));
+      right = nre_uci - ack_c1;
+    }
+    o.nof_csi_part1_re = left < right ? left : right;
+  }
+  // CSI Part 2 (:106-146), HARQ-ACK REs not counted when at most 2 bits.
+  const unsigned ack_c2 = O_ack <= 2 ? 0u : o.nof_harq_ack_re;
+  if (O_c2 != 0) {
+    if (with_sch) {
+      const float payload = (float)(O_c2 + uci_crc_size(O_c2));
+      const unsigned left = ceil_u(payload * c.beta_offset_csi_part2 * (float)nre_uci / (float)sum_cb);
+      const unsigned right = ceil_u(c.alpha_scaling * (float)nre_uci) - ack_c2 - o.nof_csi_part1_re;
+      o.nof_csi_part2_re = left < right ? left : right;
+    } else {
+      o.nof_csi_part2_re = nre_uci - ack_c2 - o.nof_csi_part1_re;
+    }
+  }
+  const unsigned ack_sch = O_ack > 2 ? o.nof_harq_ack_re : 0u;
+  const unsigned nre_sch = with_sch ? nre_total - ack_sch - o.nof_csi_part1_re - o.nof_csi_part2_re : 0u;
+  const unsigned bpr = c.nof_layers * qm;
+  o.nof_ul_sch_bits = nre_sch * bpr;
+  o.nof_harq_ack_bits = o.nof_harq_ack_re * bpr;
+  o.nof_harq_ack_rvd = rvd_re * bpr;
+  o.nof_csi_part1_bits = o.nof_csi_part1_re * bpr;
+  o.nof_csi_part2_bits = o.nof_csi_part2_re * bpr;
+  o.nof_dc_overlap_bits = c.contains_dc ? c.nof_symbols * qm : 0u;
   return 0;
 }
 

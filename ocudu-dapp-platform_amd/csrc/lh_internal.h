@@ -247,6 +247,8 @@ struct demux_tile {
 // (validation and sizes only); it receives at most 14 records.
 int demux_plan(const lh_ulsch_demux_desc& d, demux_sym* syms, unsigned* nof_syms, lh_ulsch_demux_sizes* sizes,
                const char** why);
+// get_ulsch_information of a grant; returns 0, or -1 with *why set.
+int ulsch_information(const lh_ulsch_config& c, lh_ulsch_info& o, const char** why);
 
 inline int crc_index(uint32_t poly)
 {

@@ -209,6 +209,33 @@ class UlschDemuxSizes(C.Structure):
     _fields_ = [(f, C.c_uint32) for f in ("nof_llrs", "nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")]
 
 
+class UlschConfig(C.Structure):
+    """lh_ulsch_config (ulsch_configuration)."""
+    _fields_ = [("tbs_bits", C.c_uint32), ("target_code_rate", C.c_float), ("modulation", C.c_uint32),
+                ("nof_harq_ack_bits", C.c_uint32), ("nof_csi_part1_bits", C.c_uint32),
+                ("nof_csi_part2_bits", C.c_uint32), ("alpha_scaling", C.c_float), ("beta_offset_harq_ack", C.c_float),
+                ("beta_offset_csi_part1", C.c_float), ("beta_offset_csi_part2", C.c_float), ("nof_rb", C.c_uint32),
+                ("start_symbol_index", C.c_uint32), ("nof_symbols", C.c_uint32), ("dmrs_type", C.c_uint32),
+                ("dmrs_symbol_mask", C.c_uint32), ("nof_cdm_groups_without_data", C.c_uint32),
+                ("nof_layers", C.c_uint32), ("contains_dc", C.c_int32)]
+
+
+class UlschInfo(C.Structure):
+    """lh_ulsch_info (ulsch_information)."""
+    _fields_ = [(f, C.c_uint32) for f in (
+        "sch_base_graph", "sch_tb_crc_size", "sch_nof_cb", "sch_lifting_size", "sch_nof_bits_per_cb",
+        "sch_nof_filler_bits_per_cb", "nof_ul_sch_bits", "nof_harq_ack_bits", "nof_harq_ack_rvd",
+        "nof_csi_part1_bits", "nof_csi_part2_bits", "nof_harq_ack_re", "nof_csi_part1_re", "nof_csi_part2_re",
+        "nof_dc_overlap_bits", "pad")]
+
+
+def ulsch_information(**cfg):
+    """get_ulsch_information of a grant (lh_ulsch_information), as a dict."""
+    info = UlschInfo()
+    _lh(lib().lh_ulsch_information(C.byref(UlschConfig(**cfg)), C.byref(info)), "lh_ulsch_information")
+    return {f: getattr(info, f) for f, _ in UlschInfo._fields_ if f != "pad"}
+
+
 def ulsch_demux_sizes(desc):
     """Soft-bit counts of one codeword's demultiplexed fields (lh_ulsch_demux_get_sizes)."""
     sz = UlschDemuxSizes()
@@ -287,7 +314,7 @@ EXPORTED = ["lh_ctx_create", "lh_ctx_destroy", "lh_last_error", "lh_ldpc_decode"
             "lh_tune_set_rows", "lh_gen_transport_blocks", "lh_segment_rx", "lh_server_create", "lh_server_destroy", "lh_server_submit",
             "lh_server_step", "lh_server_poll", "lh_server_harq_release", "lh_server_stats", "lh_server_submit_batch",
             "lh_pusch_decode_batch_noalloc", "lh_server_run_slot", "lh_server_serve", "lh_ulsch_demux_get_sizes",
-            "lh_ulsch_demultiplex"]
+            "lh_ulsch_demultiplex", "lh_ulsch_information"]
 
 
 def lib():
@@ -348,6 +375,7 @@ def lib():
         L.lh_pusch_demodulate.argtypes = [C.c_void_p, C.POINTER(DemodDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]
         L.lh_ulsch_demux_get_sizes.argtypes = [C.POINTER(UlschDemuxDesc), C.POINTER(UlschDemuxSizes)]
+        L.lh_ulsch_information.argtypes = [C.POINTER(UlschConfig), C.POINTER(UlschInfo)]
         L.lh_ulsch_demultiplex.argtypes = [C.c_void_p, C.POINTER(UlschDemuxDesc), C.c_uint, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.lh_ldpc_encode_rate_match.argtypes = [C.c_void_p, C.POINTER(EncodeDesc), C.c_uint, C.c_void_p, C.c_void_p,

@@ -1308,8 +1308,15 @@ static void dmx_placeholder(int8_t* out, const int8_t* in, const uint8_t* c, uns
  * llrs = the descrambled codeword (sum over symbols of REs * nbpr soft bits); c_init =
  * its scrambling initialisation.  Output pointers may be NULL to only count.  Returns 0,
  * or -1 for a configuration the reference rejects with an assertion. */
-int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch,
-                          int8_t* harq_ack, int8_t* csi1, int8_t* csi2, orc_demux_sizes* sizes)
+static void dmx_positions(int32_t* pos, size_t n, size_t q)
+{
+  for (size_t b = 0; b < n; ++b) {
+    pos[b] = (int32_t)(q + b);
+  }
+}
+
+static int dmx_run(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch, int8_t* harq_ack,
+                   int8_t* csi1, int8_t* csi2, int32_t* const* pos, orc_demux_sizes* sizes)
 {
   const int mod = cfg->modulation;
   if (mod != 0 && mod != 1 && mod != 2 && mod != 4 && mod != 6 && mod != 8) {
@@ -1444,6 +1451,21 @@ int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint3
     for (unsigned x = 0; x < M; ++x) {
       const int8_t* in = llrs ? llrs + q + (size_t)x * nbpr : NULL;
       const int punct = zero_ack && (re[x] & DMX_ACK);
+      if (pos) {
+        const size_t at = q + (size_t)x * nbpr;
+        if ((re[x] & DMX_ACK) && pos[1]) {
+          dmx_positions(pos[1] + n_ack, nbpr, at);
+        }
+        if ((re[x] & DMX_C1) && pos[2]) {
+          dmx_positions(pos[2] + n_c1, nbpr, at);
+        }
+        if ((re[x] & DMX_C2) && pos[3]) {
+          dmx_positions(pos[3] + n_c2, nbpr, at);
+        }
+        if ((re[x] & DMX_SCH) && pos[0]) {
+          dmx_positions(pos[0] + n_sch, nbpr, at);
+        }
+      }
       if (re[x] & DMX_ACK) {
         if (harq_ack) {
           dmx_placeholder(harq_ack + n_ack, in, seq + q + (size_t)x * nbpr, nbpr, qm, ack_bits);
@@ -1491,4 +1513,184 @@ int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint3
 done:
   free(seq);
   return rc;
+}
+
+int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch,
+                          int8_t* harq_ack, int8_t* csi1, int8_t* csi2, orc_demux_sizes* sizes)
+{
+  return dmx_run(cfg, llrs, c_init, sch, harq_ack, csi1, csi2, NULL, sizes);
+}
+
+/* The codeword position of every soft bit of each field (NULL: not wanted), in the
+ * fields' order: the transmitter's multiplexing is the inverse map (a 1- or 2-bit
+ * HARQ-ACK overwrites the UL-SCH / CSI Part 2 bits of its REs). */
+int orc_ulsch_positions(const orc_demux_config* cfg, int32_t* sch, int32_t* harq_ack, int32_t* csi1, int32_t* csi2)
+{
+  int32_t* const pos[4] = {sch, harq_ack, csi1, csi2};
+  return dmx_run(cfg, NULL, 0, NULL, NULL, NULL, NULL, pos, NULL);
+}
+
+/* ================================================================================= */
+/* f1/f4: UL-SCH bit accounting, lib/ran/pusch/ulsch_info.cpp (TS38.212 6.3.2.4)      */
+/* ================================================================================= */
+
+/* get_uci_crc_size (include/ocudu/ran/uci/uci_info.h:36-47). */
+static unsigned uci_crc_bits(unsigned payload)
+{
+  return payload < 12 ? 0u : (payload < 20 ? 6u : 11u);
+}
+
+/* get_sch_segmentation_info (lib/ran/sch/sch_segmentation.cpp:11-42) with
+ * get_ldpc_base_graph (include/ocudu/ran/sch/ldpc_base_graph.h:20-30); a rate of 0 (which
+ * the reference accepts when its assertions are compiled out) selects BG2 like any rate up
+ * to 0.25. */
+void orc_sch_info(unsigned tbs, float R, orc_sch_info_t* out)
+{
+  static const unsigned Zs[51] = {2,  3,  4,  5,  6,  7,  8,  9,  10,  11,  12,  13,  14,  15,  16,  18,  20,
+                                  22, 24, 26, 28, 30, 32, 36, 40, 44, 48, 52, 56, 60, 64, 72, 80, 88,
+                                  96, 104, 112, 120, 128, 144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384};
+  const unsigned crc = tbs <= 3824u ? 16u : 24u;
+  const int bg = (tbs <= 292u || R <= 0.25F || (tbs <= 3824u && R <= 0.67F)) ? 2 : 1;
+  const unsigned B = tbs + crc;
+  const unsigned Kcb = bg == 1 ? 8448u : 3840u;
+  const unsigned C = B <= Kcb ? 1u : (B + (Kcb - 24u) - 1u) / (Kcb - 24u);
+  unsigned per_cb = B / C + (C > 1 ? 24u : 0u);
+  unsigned kb = 22;
+  if (bg == 2) {
+    kb = B > 640u ? 10u : (B > 560u ? 9u : (B > 192u ? 8u : 6u));
+  }
+  const unsigned Bp = B + (C > 1 ? 24u * C : 0u);
+  unsigned Z = 0;
+  for (int i = 0; i < 51 && Z == 0; ++i) {
+    if (Zs[i] * kb * C >= Bp) {
+      Z = Zs[i];
+    }
+  }
+  out->base_graph = (unsigned)bg;
+  out->tb_crc_size = crc;
+  out->nof_cb = C;
+  out->lifting_size = Z;
+  out->nof_bits_per_cb = (bg == 1 ? 22u : 10u) * Z;
+  out->nof_filler_bits_per_cb = out->nof_bits_per_cb - per_cb;
+}
+
+/* ceil of a float expression, as the reference's implicit std::ceil -> unsigned. */
+static unsigned fceil_u(float v)
+{
+  return (unsigned)ceilf(v);
+}
+
+/* calculate_nof_re_harq_ack (ulsch_info.cpp:12-31). */
+static unsigned re_harq_ack(unsigned bits, float beta, unsigned nre_uci, unsigned sum_cb, float alpha, unsigned nre_l0)
+{
+  if (bits == 0) {
+    return 0;
+  }
+  const unsigned left = fceil_u((float)(bits + uci_crc_bits(bits)) * beta * (float)nre_uci / (float)sum_cb);
+  const unsigned right = fceil_u(alpha * (float)nre_l0);
+  return left < right ? left : right;
+}
+
+/* calculate_nof_re_harq_ack_without_sch (:33-52). */
+static unsigned re_harq_ack_no_sch(unsigned bits, float beta, float R, unsigned qm, float alpha, unsigned nre_l0)
+{
+  if (bits == 0) {
+    return 0;
+  }
+  const unsigned left = fceil_u((float)(bits + uci_crc_bits(bits)) * beta / (R * (float)qm));
+  const unsigned right = fceil_u(alpha * (float)nre_l0);
+  return left < right ? left : right;
+}
+
+/* get_ulsch_information (ulsch_info.cpp:148-303). */
+int orc_ulsch_information(const orc_ulsch_config* cfg, orc_ulsch_info* info)
+{
+  memset(info, 0, sizeof(*info));
+  const unsigned mask = cfg->dmrs_symbol_mask & 0x3fffu;
+  if (cfg->nof_cdm_groups_without_data < 1 || cfg->nof_cdm_groups_without_data > (cfg->dmrs_type == 1 ? 2u : 3u) ||
+      mask == 0 || cfg->start_symbol_index + cfg->nof_symbols > 14) {
+    return -1;
+  }
+  unsigned first = 0, last = 13;
+  while (!((mask >> first) & 1u)) {
+    ++first;
+  }
+  while (!((mask >> last) & 1u)) {
+    --last;
+  }
+  if (first < cfg->start_symbol_index || last >= cfg->start_symbol_index + cfg->nof_symbols) {
+    return -1;
+  }
+  const int has_sch = cfg->tbs_bits > 0;
+  if (has_sch) {
+    orc_sch_info(cfg->tbs_bits, cfg->target_code_rate, &info->sch);
+  }
+  unsigned nsym_dmrs = 0;
+  for (unsigned l = 0; l < 14; ++l) {
+    nsym_dmrs += (mask >> l) & 1u;
+  }
+  const unsigned re_dmrs_rb = nsym_dmrs * cfg->nof_cdm_groups_without_data * (cfg->dmrs_type == 1 ? 6u : 4u);
+  const unsigned nre_total = cfg->nof_rb * (cfg->nof_symbols * 12u - re_dmrs_rb);
+  const unsigned nre_uci = (cfg->nof_symbols - nsym_dmrs) * cfg->nof_rb * 12u;
+  const unsigned qm = cfg->modulation == 0 ? 1u : (unsigned)cfg->modulation;
+  unsigned nre_l0 = 0;
+  for (unsigned l = first; l < cfg->start_symbol_index + cfg->nof_symbols; ++l) {
+    if (!((mask >> l) & 1u)) {
+      nre_l0 += cfg->nof_rb * 12u;
+    }
+  }
+  const unsigned sum_cb = info->sch.nof_cb * info->sch.nof_bits_per_cb;
+  const float R = cfg->target_code_rate;
+  const unsigned O_ack = cfg->nof_harq_ack_bits, O_c1 = cfg->nof_csi_part1_bits, O_c2 = cfg->nof_csi_part2_bits;
+  info->nof_harq_ack_re = has_sch ? re_harq_ack(O_ack, cfg->beta_offset_harq_ack, nre_uci, sum_cb, cfg->alpha_scaling,
+                                                nre_l0)
+                                  : re_harq_ack_no_sch(O_ack, cfg->beta_offset_harq_ack, R, qm, cfg->alpha_scaling,
+                                                       nre_l0);
+  /* Reserved REs as for two HARQ-ACK bits when fewer than two are sent (:225-252). */
+  unsigned rvd_re = 0;
+  if (O_ack < 2) {
+    rvd_re = has_sch ? re_harq_ack(2, cfg->beta_offset_harq_ack, nre_uci, sum_cb, cfg->alpha_scaling, nre_l0)
+                     : re_harq_ack_no_sch(2, cfg->beta_offset_harq_ack, R, qm, cfg->alpha_scaling, nre_l0);
+  } else if (O_ack == 2) {
+    rvd_re = info->nof_harq_ack_re;
+  }
+  const unsigned ack_for_c1 = O_ack <= 2 ? rvd_re : info->nof_harq_ack_re;
+  /* CSI Part 1 (:56-104). */
+  if (O_c1 != 0) {
+    if (has_sch) {
+      const unsigned left = fceil_u((float)(O_c1 + uci_crc_bits(O_c1)) * cfg->beta_offset_csi_part1 * (float)nre_uci /
+                                    (float)sum_cb);
+      const unsigned right = fceil_u(cfg->alpha_scaling * (float)nre_uci) - ack_for_c1;
+      info->nof_csi_part1_re = left < right ? left : right;
+    } else if (O_c2 == 0) {
+      info->nof_csi_part1_re = nre_uci - ack_for_c1;
+    } else {
+      const unsigned left =
+          fceil_u((float)(O_c1 + uci_crc_bits(O_c1)) * cfg->beta_offset_csi_part1 / (R * (float)qm));
+      const unsigned right = nre_uci - ack_for_c1;
+      info->nof_csi_part1_re = left < right ? left : right;
+    }
+  }
+  /* CSI Part 2 (:106-146), HARQ-ACK REs ignored when at most two bits. */
+  const unsigned ack_for_c2 = O_ack <= 2 ? 0u : info->nof_harq_ack_re;
+  if (O_c2 != 0) {
+    if (has_sch) {
+      const unsigned left = fceil_u((float)(O_c2 + uci_crc_bits(O_c2)) * cfg->beta_offset_csi_part2 * (float)nre_uci /
+                                    (float)sum_cb);
+      const unsigned right = fceil_u(cfg->alpha_scaling * (float)nre_uci) - ack_for_c2 - info->nof_csi_part1_re;
+      info->nof_csi_part2_re = left < right ? left : right;
+    } else {
+      info->nof_csi_part2_re = nre_uci - ack_for_c2 - info->nof_csi_part1_re;
+    }
+  }
+  const unsigned ack_re_sch = O_ack > 2 ? info->nof_harq_ack_re : 0u;
+  const unsigned nre_sch = has_sch ? nre_total - ack_re_sch - info->nof_csi_part1_re - info->nof_csi_part2_re : 0u;
+  const unsigned bpr = cfg->nof_layers * qm;
+  info->nof_ul_sch_bits = nre_sch * bpr;
+  info->nof_harq_ack_bits = info->nof_harq_ack_re * bpr;
+  info->nof_harq_ack_rvd = rvd_re * bpr;
+  info->nof_csi_part1_bits = info->nof_csi_part1_re * bpr;
+  info->nof_csi_part2_bits = info->nof_csi_part2_re * bpr;
+  info->nof_dc_overlap_bits = cfg->contains_dc ? cfg->nof_symbols * qm : 0u;
+  return 0;
 }

@@ -176,6 +176,56 @@ typedef struct {
 
 int orc_ulsch_demultiplex(const orc_demux_config* cfg, const int8_t* llrs, uint32_t c_init, int8_t* sch,
                           int8_t* harq_ack, int8_t* csi1, int8_t* csi2, orc_demux_sizes* sizes);
+int orc_ulsch_positions(const orc_demux_config* cfg, int32_t* sch, int32_t* harq_ack, int32_t* csi1,
+                        int32_t* csi2);
+
+/* UL-SCH bit accounting: ulsch_configuration / ulsch_information / sch_information
+ * (include/ocudu/ran/pusch/ulsch_info.h, include/ocudu/ran/sch/sch_segmentation.h). */
+typedef struct {
+  unsigned base_graph;
+  unsigned tb_crc_size;
+  unsigned nof_cb;
+  unsigned lifting_size;
+  unsigned nof_bits_per_cb;
+  unsigned nof_filler_bits_per_cb;
+} orc_sch_info_t;
+
+typedef struct {
+  unsigned tbs_bits;
+  float target_code_rate; /* normalised R */
+  int modulation;         /* bits per symbol, 0 pi/2-BPSK */
+  unsigned nof_harq_ack_bits;
+  unsigned nof_csi_part1_bits;
+  unsigned nof_csi_part2_bits;
+  float alpha_scaling;
+  float beta_offset_harq_ack;
+  float beta_offset_csi_part1;
+  float beta_offset_csi_part2;
+  unsigned nof_rb;
+  unsigned start_symbol_index;
+  unsigned nof_symbols;
+  unsigned dmrs_type;
+  unsigned dmrs_symbol_mask;
+  unsigned nof_cdm_groups_without_data;
+  unsigned nof_layers;
+  int contains_dc;
+} orc_ulsch_config;
+
+typedef struct {
+  orc_sch_info_t sch;
+  unsigned nof_ul_sch_bits;
+  unsigned nof_harq_ack_bits;
+  unsigned nof_harq_ack_rvd;
+  unsigned nof_csi_part1_bits;
+  unsigned nof_csi_part2_bits;
+  unsigned nof_harq_ack_re;
+  unsigned nof_csi_part1_re;
+  unsigned nof_csi_part2_re;
+  unsigned nof_dc_overlap_bits;
+} orc_ulsch_info;
+
+void orc_sch_info(unsigned tbs, float R, orc_sch_info_t* out);
+int orc_ulsch_information(const orc_ulsch_config* cfg, orc_ulsch_info* info);
 
 #ifdef __cplusplus
 }

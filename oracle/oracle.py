@@ -378,6 +378,7 @@ def ref_demod():
         _fp = C.POINTER(C.c_float)
         R.ref_demodulate_soft.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int]
         R.ref_scrambling_sequence.argtypes = [_u8p, C.c_uint32, C.c_uint, C.c_uint]
+        R.ref_ulsch_information.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_float), C.POINTER(C.c_uint)]
         R.ref_ulsch_demultiplex.argtypes = [C.POINTER(C.c_uint), _i8p, C.c_uint, C.c_uint32, C.c_uint, _i8p, _i8p,
                                             _i8p, _i8p, C.POINTER(C.c_uint)]
         _ref_demod = R
@@ -470,6 +471,19 @@ def ulsch_demultiplex(cfg, llrs, c_init):
     return tuple(o[:sz[k]] for o, k in zip(outs, ("nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")))
 
 
+def ulsch_positions(cfg):
+    """(sch, harq_ack, csi_part1, csi_part2): the codeword position of each field's soft
+    bits (int32 arrays)."""
+    sz = demux_sizes(cfg)
+    if sz is None:
+        raise ValueError("invalid demultiplexer configuration")
+    keys = ("nof_sch", "nof_harq_ack", "nof_csi_part1", "nof_csi_part2")
+    outs = [np.zeros(max(sz[k], 1), np.int32) for k in keys]
+    i32p = C.POINTER(C.c_int32)
+    assert lib().orc_ulsch_positions(C.byref(_demux_cfg(cfg)), *[_p(o, i32p) for o in outs]) == 0
+    return tuple(o[:sz[k]] for o, k in zip(outs, keys))
+
+
 def ref_ulsch_demultiplex(cfg, llrs, c_init, block=1000):
     """The reference's ulsch_demultiplex_impl built from source, fed in blocks of `block`
     soft bits, CSI Part 2 configured before the first block; None for a configuration
@@ -486,3 +500,73 @@ def ref_ulsch_demultiplex(cfg, llrs, c_init, block=1000):
     if rc != 0:
         raise RuntimeError("a decoder buffer did not receive its end")
     return tuple(o[:sizes[k]] for k, o in enumerate(outs))
+
+
+# --- UL-SCH bit accounting (get_ulsch_information) --------------------------------------
+
+ULSCH_INT_FIELDS = ("tbs_bits", "modulation", "nof_harq_ack_bits", "nof_csi_part1_bits", "nof_csi_part2_bits",
+                    "nof_rb", "start_symbol_index", "nof_symbols", "dmrs_type", "dmrs_symbol_mask",
+                    "nof_cdm_groups_without_data", "nof_layers", "contains_dc")
+ULSCH_FLOAT_FIELDS = ("target_code_rate", "alpha_scaling", "beta_offset_harq_ack", "beta_offset_csi_part1",
+                      "beta_offset_csi_part2")
+ULSCH_INFO_FIELDS = ("nof_ul_sch_bits", "nof_harq_ack_bits", "nof_harq_ack_rvd", "nof_csi_part1_bits",
+                     "nof_csi_part2_bits", "nof_harq_ack_re", "nof_csi_part1_re", "nof_csi_part2_re",
+                     "nof_dc_overlap_bits")
+SCH_INFO_FIELDS = ("base_graph", "nof_cb", "nof_bits_per_cb", "lifting_size", "nof_filler_bits_per_cb",
+                   "tb_crc_size")
+
+
+class _SchInfo(C.Structure):
+    _fields_ = [(f, C.c_uint) for f in ("base_graph", "tb_crc_size", "nof_cb", "lifting_size", "nof_bits_per_cb",
+                                         "nof_filler_bits_per_cb")]
+
+
+class _UlschCfg(C.Structure):
+    _fields_ = [("tbs_bits", C.c_uint), ("target_code_rate", C.c_float), ("modulation", C.c_int),
+                ("nof_harq_ack_bits", C.c_uint), ("nof_csi_part1_bits", C.c_uint), ("nof_csi_part2_bits", C.c_uint),
+                ("alpha_scaling", C.c_float), ("beta_offset_harq_ack", C.c_float),
+                ("beta_offset_csi_part1", C.c_float), ("beta_offset_csi_part2", C.c_float), ("nof_rb", C.c_uint),
+                ("start_symbol_index", C.c_uint), ("nof_symbols", C.c_uint), ("dmrs_type", C.c_uint),
+                ("dmrs_symbol_mask", C.c_uint), ("nof_cdm_groups_without_data", C.c_uint), ("nof_layers", C.c_uint),
+                ("contains_dc", C.c_int)]
+
+
+class _UlschInfo(C.Structure):
+    _fields_ = [("sch", _SchInfo)] + [(f, C.c_uint) for f in ULSCH_INFO_FIELDS]
+
+
+def ulsch_information(cfg):
+    """get_ulsch_information of a configuration dict (ULSCH_*_FIELDS); None if invalid."""
+    c = _UlschCfg(**{f: cfg.get(f, 0) for f in ULSCH_INT_FIELDS + ULSCH_FLOAT_FIELDS})
+    info = _UlschInfo()
+    if lib().orc_ulsch_information(C.byref(c), C.byref(info)) != 0:
+        return None
+    out = {f: getattr(info, f) for f in ULSCH_INFO_FIELDS}
+    if cfg.get("tbs_bits", 0) > 0:
+        out.update({"sch_" + f: getattr(info.sch, f) for f in SCH_INFO_FIELDS})
+    return out
+
+
+def ref_ulsch_information(cfg):
+    """The reference's get_ulsch_information built from source (no assertions, as its
+    Release build)."""
+    w = (C.c_uint * 13)(*[int(cfg.get(f, 0)) for f in ULSCH_INT_FIELDS])
+    fl = (C.c_float * 5)(*[float(cfg.get(f, 0.0)) for f in ULSCH_FLOAT_FIELDS])
+    out = (C.c_uint * 15)()
+    ref_demod().ref_ulsch_information(w, fl, out)
+    res = dict(zip(ULSCH_INFO_FIELDS, out[:9]))
+    if cfg.get("tbs_bits", 0) > 0:
+        res.update({"sch_" + f: v for f, v in zip(SCH_INFO_FIELDS, out[9:15])})
+    return res
+
+
+def compute_n_ref(tbs_lbrm_bytes, nof_codeblocks):
+    """ldpc::compute_N_ref (include/ocudu/phy/upper/channel_coding/ldpc/ldpc.h:119-122)."""
+    return min(tbs_lbrm_bytes * 8 * 3 // (2 * nof_codeblocks), 384 * 66)  # MAX_CODEBLOCK_SIZE
+
+
+def nof_codeblocks(tbs_bits, bg):
+    """compute_nof_codeblocks (include/ocudu/ran/sch/sch_segmentation.h:36-47)."""
+    B = tbs_bits + (16 if tbs_bits <= 3824 else 24)
+    K = 8448 if bg == 1 else 3840
+    return 1 if B <= K else -(-B // (K - 24))

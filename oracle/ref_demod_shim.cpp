@@ -145,3 +145,63 @@ int ref_ulsch_demultiplex(const unsigned* cfg, const int8_t* llrs, unsigned nof_
 }
 
 } // extern "C"
+
+// ---- UL-SCH bit accounting (lib/ran/pusch/ulsch_info.cpp built from source) ------------
+
+#include "ocudu/ran/pusch/ulsch_info.h"
+
+extern "C" {
+
+// get_ulsch_information.  w: tbs bits, modulation, O_ACK, O_CSI1, O_CSI2, PRBs, start
+// symbol, symbols, DM-RS type, DM-RS mask, CDM groups without data, layers, contains_dc;
+// f: target code rate (normalised), alpha, beta ACK, beta CSI1, beta CSI2.  out: UL-SCH
+// bits, HARQ-ACK bits, reserved bits, CSI1 bits, CSI2 bits, HARQ-ACK REs, CSI1 REs,
+// CSI2 REs, DC overlap bits, then (with a TB) the SCH base graph, codeblocks, bits per
+// codeblock, lifting size, filler bits per codeblock, TB CRC bits.
+void ref_ulsch_information(const unsigned* w, const float* f, unsigned* out)
+{
+  ulsch_configuration c;
+  c.tbs = units::bits(w[0]);
+  c.mcs_descr.modulation = static_cast<modulation_scheme>(w[1]);
+  c.mcs_descr.target_code_rate = f[0] * 1024.0F;
+  c.nof_harq_ack_bits = units::bits(w[2]);
+  c.nof_csi_part1_bits = units::bits(w[3]);
+  c.nof_csi_part2_bits = units::bits(w[4]);
+  c.alpha_scaling = f[1];
+  c.beta_offset_harq_ack = f[2];
+  c.beta_offset_csi_part1 = f[3];
+  c.beta_offset_csi_part2 = f[4];
+  c.nof_rb = w[5];
+  c.start_symbol_index = w[6];
+  c.nof_symbols = w[7];
+  c.dmrs_type = w[8] == 1 ? dmrs_config_type::type1 : dmrs_config_type::type2;
+  c.dmrs_symbol_mask = bounded_bitset<MAX_NSYMB_PER_SLOT>(MAX_NSYMB_PER_SLOT);
+  for (unsigned l = 0; l < MAX_NSYMB_PER_SLOT; ++l) {
+    if ((w[9] >> l) & 1u) {
+      c.dmrs_symbol_mask.set(l);
+    }
+  }
+  c.nof_cdm_groups_without_data = w[10];
+  c.nof_layers = w[11];
+  c.contains_dc = w[12] != 0;
+  const ulsch_information i = get_ulsch_information(c);
+  out[0] = i.nof_ul_sch_bits.value();
+  out[1] = i.nof_harq_ack_bits.value();
+  out[2] = i.nof_harq_ack_rvd.value();
+  out[3] = i.nof_csi_part1_bits.value();
+  out[4] = i.nof_csi_part2_bits.value();
+  out[5] = i.nof_harq_ack_re;
+  out[6] = i.nof_csi_part1_re;
+  out[7] = i.nof_csi_part2_re;
+  out[8] = i.nof_dc_overlap_bits.value();
+  if (i.sch.has_value()) {
+    out[9] = i.sch->base_graph == ldpc_base_graph_type::BG1 ? 1 : 2;
+    out[10] = i.sch->nof_cb;
+    out[11] = i.sch->nof_bits_per_cb.value();
+    out[12] = i.sch->lifting_size;
+    out[13] = i.sch->nof_filler_bits_per_cb.value();
+    out[14] = i.sch->tb_crc_size.value();
+  }
+}
+
+} // extern "C"

@@ -15,8 +15,8 @@ TB_CRC_OK, TB_CB_FAILED, TB_CRC_FAILED = 1, 0, -1
 
 
 class HarqModel:
-    def __init__(self, bg, tbs_bits, qm, layers, nsym):
-        self.bg, self.tbs, self.qm = bg, tbs_bits, qm
+    def __init__(self, bg, tbs_bits, qm, layers, nsym, Nref=0):
+        self.bg, self.tbs, self.qm, self.Nref = bg, tbs_bits, qm, Nref
         self.cbs, self.seg = O.segment_rx(bg, tbs_bits // 8, qm, layers, nsym)
         self.C = len(self.cbs)
         self.Z = self.cbs[0]["Z"]
@@ -37,8 +37,8 @@ class HarqModel:
         res = np.zeros(self.C, np.int32)
         for c, m in enumerate(self.cbs):
             e = np.asarray(llrs[m["cw_offset"]:m["cw_offset"] + m["rm_length"]], np.int8)
-            self.soft[c] = O.rate_dematch(e, self.bg, self.Z, rv=rv, Qm=self.qm, F=self.F, new_data=new_data,
-                                          harq=None if new_data else self.soft[c])
+            self.soft[c] = O.rate_dematch(e, self.bg, self.Z, rv=rv, Qm=self.qm, Nref=self.Nref, F=self.F,
+                                          new_data=new_data, harq=None if new_data else self.soft[c])
             if new_data:
                 self.passed[c] = 0
             if self.passed[c]:

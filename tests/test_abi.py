@@ -44,6 +44,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(H.TrainDesc) == 2 * 8 + 4 * 4
     assert ctypes.sizeof(H.UlschDemuxDesc) == 5 * 8 + 16 * 4
     assert ctypes.sizeof(H.UlschDemuxSizes) == 5 * 4
+    assert ctypes.sizeof(H.UlschConfig) == 18 * 4
+    assert ctypes.sizeof(H.UlschInfo) == 16 * 4
 
 
 def test_invalid_arguments_fail_without_gpu():

@@ -204,3 +204,76 @@ def test_device_arithmetic_replayed_on_host():
     out = subprocess.run([str(RANK_BIN)], input="\n".join(lines) + "\n", capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip() == f"ok {len(lines)}"
+
+
+def _random_ulsch_config(rng):
+    while True:
+        mod = int(rng.choice([0, 1, 2, 4, 6, 8]))
+        start = int(rng.integers(0, 4))
+        nsym = int(rng.integers(4, 15 - start))
+        from demux_util import _dmrs_mask
+        cfg = dict(tbs_bits=int(rng.choice([0, 24, 200, 300, 1000, 3824, 3840, 8000, 40000, 150000])) // 8 * 8,
+                   target_code_rate=float(rng.choice([0.0, 0.12, 0.3, 0.5, 0.66, 0.7, 0.9, rng.uniform(0.05, 0.95)])),
+                   modulation=mod, nof_harq_ack_bits=int(rng.choice([0, 1, 2, 3, 11, 12, 19, 20, 40])),
+                   nof_csi_part1_bits=int(rng.choice([0, 0, 5, 15, 30])), nof_csi_part2_bits=0,
+                   alpha_scaling=float(rng.choice([0.5, 0.65, 0.8, 1.0])),
+                   beta_offset_harq_ack=float(rng.choice([2.0, 5.0, 8.0, 20.0, 126.0])),
+                   beta_offset_csi_part1=float(rng.choice([1.125, 2.0, 6.25])),
+                   beta_offset_csi_part2=float(rng.choice([1.125, 2.0, 6.25])), nof_rb=int(rng.integers(1, 276)),
+                   start_symbol_index=start, nof_symbols=nsym, dmrs_type=int(rng.integers(1, 3)),
+                   dmrs_symbol_mask=_dmrs_mask(rng, start, nsym), nof_layers=1 if mod == 0 else int(rng.integers(1, 5)),
+                   contains_dc=int(rng.integers(0, 2)))
+        cfg["nof_cdm_groups_without_data"] = int(rng.integers(1, 3 if cfg["dmrs_type"] == 1 else 4))
+        if cfg["nof_csi_part1_bits"] and rng.random() < 0.5:
+            cfg["nof_csi_part2_bits"] = int(rng.choice([3, 20, 60]))
+        if cfg["tbs_bits"] == 0 and cfg["target_code_rate"] == 0.0:
+            continue  # a UCI-only grant divides by the code rate
+        return cfg
+
+
+@pytest.mark.skipif(O.ref_demod() is None, reason="reference build absent (oracle/_ref)")
+def test_ulsch_information_matches_reference_build():
+    """orc_ulsch_information against the reference's get_ulsch_information
+    (lib/ran/pusch/ulsch_info.cpp, lib/ran/sch/sch_segmentation.cpp built from source),
+    every field, over grants with and without a TB, every UCI size class (CRC 0/6/11
+    bits), reservation for 0..2 HARQ-ACK bits, CSI Part 2, code rates including 0."""
+    rng = np.random.default_rng(44)
+    for case in range(3000):
+        cfg = _random_ulsch_config(rng)
+        want = O.ref_ulsch_information(cfg)
+        got = O.ulsch_information(cfg)
+        assert got == want, f"case {case} {cfg}"
+
+
+def test_ulsch_information_hand_worked():
+    """TBS 1000 bits at R 0.5: BG2 (1000 <= 3824, R <= 0.67), one codeblock, K_b 10
+    (1016 > 640), Z = 104 (smallest with 10 Z >= 1016), 1040 bits per codeblock.  16-QAM,
+    1 layer, 10 PRBs, symbols 0..13, DM-RS on 2 and 11 (type 1, 2 CDM groups: no data
+    there): 12 x 120 = 1440 UCI REs, 11 x 120 = 1320 after the first DM-RS.  HARQ-ACK 3
+    bits, beta 8: ceil(3 * 8 * 1440 / 1040) = 34 REs (alpha 1 bound 1320); no reservation.
+    CSI Part 1 5 bits, beta 2: ceil(5 * 2 * 1440 / 1040) = 14.  UL-SCH: 1440 - 34 - 14 REs."""
+    cfg = dict(tbs_bits=1000, target_code_rate=0.5, modulation=4, nof_harq_ack_bits=3, nof_csi_part1_bits=5,
+               alpha_scaling=1.0, beta_offset_harq_ack=8.0, beta_offset_csi_part1=2.0, beta_offset_csi_part2=1.0,
+               nof_rb=10, start_symbol_index=0, nof_symbols=14, dmrs_type=1, dmrs_symbol_mask=(1 << 2) | (1 << 11),
+               nof_cdm_groups_without_data=2, nof_layers=1)
+    i = O.ulsch_information(cfg)
+    assert (i["sch_base_graph"], i["sch_nof_cb"], i["sch_lifting_size"], i["sch_nof_bits_per_cb"]) == (2, 1, 104, 1040)
+    assert (i["nof_harq_ack_re"], i["nof_harq_ack_rvd"], i["nof_csi_part1_re"]) == (34, 0, 14)
+    assert i["nof_ul_sch_bits"] == (1440 - 34 - 14) * 4
+    # One HARQ-ACK bit: the reservation is sized for two bits (ceil(2 * 8 * 1440 / 1040) = 23),
+    # the HARQ-ACK itself for one (ceil(8 * 1440 / 1040) = 12), and neither leaves UL-SCH.
+    i1 = O.ulsch_information(dict(cfg, nof_harq_ack_bits=1))
+    assert (i1["nof_harq_ack_re"], i1["nof_harq_ack_rvd"]) == (12, 23 * 4)
+    assert i1["nof_csi_part1_re"] == 14 and i1["nof_ul_sch_bits"] == (1440 - 14) * 4
+
+
+def test_abi_ulsch_information_matches_oracle():
+    """lh_ulsch_information (host only) gives the oracle's accounting, field by field."""
+    import ocudu_hip as H
+    rng = np.random.default_rng(45)
+    for case in range(3000):
+        cfg = _random_ulsch_config(rng)
+        want = O.ulsch_information(cfg)
+        got = H.ulsch_information(**cfg)
+        for k, v in want.items():
+            assert got[k] == v, f"case {case} {cfg}: {k} {got[k]} vs oracle {v}"
