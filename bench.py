@@ -12,7 +12,8 @@ decoder (tables/nms_bg1.json, lh_pusch_decode_batch_learned) is measured on the 
 inputs there, and both again at the operating point: the lowest Es/N0 on a 0.05 dB grid
 where the plain decoder's BLER is <= 10%.  Also reported: one-slot latency, the transmit
 side (LDPC encoder + rate matcher) on the same messages, the step before the path (soft
-demodulation + descrambling of one full 4-layer 256-QAM slot), the uneven batch, and the
+demodulation + descrambling of one full 4-layer 256-QAM slot, then its UL-SCH
+demultiplexing with UCI), the uneven batch, and the
 slot-deadline serving loop.  The roofline object carries the HBM figure (the declared
 bound) and the VALU fraction (the binding resource); the CPU baseline is the reference's
 own AVX-512 decoder on every usable host core, with one-thread AVX-512 and generic figures
@@ -248,6 +249,8 @@ def main():
     # block per OFDM symbol as the PUSCH demodulator calls it).
     demod = demod_slot(ctx, stream, ev0, ev1, cpu=not a.no_cpu_baseline)
     print(f"# demodulation + descrambling: {json.dumps(demod)}", file=sys.stderr, flush=True)
+    demux = demux_slot(ctx, stream, ev0, ev1, cpu=not a.no_cpu_baseline)
+    print(f"# UL-SCH demultiplexing: {json.dumps(demux)}", file=sys.stderr, flush=True)
 
     uneven = uneven_batch(ctx, a) if a.uneven_ues else None
     serving = serving_runs(ctx, a) if a.serve_slots else None
@@ -282,6 +285,7 @@ def main():
         "operating_point": op,
         "encoder": encoder,
         "demodulation": demod,
+        "demultiplexing": demux,
         "uneven_batch": uneven,
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -420,6 +424,62 @@ def demod_slot(ctx, stream, ev0, ev1, cpu=True):
     return {"symbols": nsym, "llrs": nsym * qm, "blocks": nblk, "ms": round(ms, 4),
             "llr_gbps": round(nsym * qm / (ms * 1e-3) / 1e9, 1), "hbm_gb_s": round(gbytes / (ms * 1e-3), 1),
             "cpu_reference": ref}
+
+
+def demux_slot(ctx, stream, ev0, ev1, cpu=True):
+    """lh_ulsch_demultiplex over the same full slot's codeword (273 PRB, 14 symbols, DM-RS
+    on 2 and 11 with both CDM groups empty of data, 4 layers, 256-QAM: 1,257,984 soft
+    bits) carrying a 1-bit HARQ-ACK over its reservation and a 20-bit CSI Part 1, sized by
+    lh_ulsch_information for a 120,000-byte TB at R 0.9 with beta offsets 20 and 6.25.
+    HBM bytes per launch: the codeword read plus every field written.  With cpu: the
+    reference's own ulsch_demultiplex_impl (oracle/_ref, built from its sources) over the
+    same codeword on one host thread, timed, and its four fields compared with the
+    device's."""
+    import ocudu_hip as H
+
+    rng = np.random.default_rng(12)
+    grant = dict(modulation=8, nof_layers=4, nof_rb=273, start_symbol_index=0, nof_symbols=14, dmrs_type=1,
+                 dmrs_symbol_mask=(1 << 2) | (1 << 11), nof_cdm_groups_without_data=2)
+    info = H.ulsch_information(tbs_bits=120000 * 8, target_code_rate=0.9, nof_harq_ack_bits=1,
+                               nof_csi_part1_bits=20, alpha_scaling=1.0, beta_offset_harq_ack=20.0,
+                               beta_offset_csi_part1=6.25, beta_offset_csi_part2=6.25, **grant)
+    d = H.UlschDemuxDesc(modulation=8, nof_layers=4, nof_prb=273, start_symbol_index=0, nof_symbols=14,
+                         dmrs_symbol_mask=grant["dmrs_symbol_mask"], dmrs_type=1, nof_cdm_groups_without_data=2,
+                         nof_harq_ack_rvd=info["nof_harq_ack_rvd"], nof_harq_ack_bits=1,
+                         nof_enc_harq_ack_bits=info["nof_harq_ack_bits"], nof_csi_part1_bits=20,
+                         nof_enc_csi_part1_bits=info["nof_csi_part1_bits"], c_init=0x1234 * 2**15 + 1)
+    sz = H.ulsch_demux_sizes(d)
+    llr = rng.integers(-120, 121, sz["nof_llrs"]).astype(np.int8)
+    d_in = H.DeviceBuffer.from_array(llr)
+    outs = [H.DeviceBuffer(max(sz[k], 1)) for k in ("nof_sch", "nof_harq_ack", "nof_csi_part1")]
+    ctx.ulsch_demultiplex([d], d_in, *outs, None, stream=stream)
+    ms = float("inf")
+    for _ in range(20):
+        ev0.record(stream)
+        ctx.ulsch_demultiplex([d], d_in, *outs, None, stream=stream)
+        ev1.record(stream)
+        ev1.sync()
+        ms = min(ms, ev0.elapsed_ms(ev1))
+    ref = None
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        if O.ref_demod() is not None:
+            cfg = {f: getattr(d, f) for f in O.DEMUX_FIELDS}
+            t0 = time.perf_counter()
+            want = O.ref_ulsch_demultiplex(cfg, llr, d.c_init, block=273 * 12 * 32)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            got = [o.download(np.int8, sz[k]) for o, k in zip(outs, ("nof_sch", "nof_harq_ack", "nof_csi_part1"))]
+            ref = {"kind": "reference", "cores": 1, "ms": round(cpu_ms, 2),
+                   "sample": "the same codeword, reference ulsch_demultiplex_impl fed one OFDM symbol per block",
+                   "bit_exact": bool(all(np.array_equal(g, w) for g, w in zip(got, want[:3])))}
+    for b in [d_in] + outs:
+        b.free()
+    nbytes = sz["nof_llrs"] + sz["nof_sch"] + sz["nof_harq_ack"] + sz["nof_csi_part1"]
+    return {"llrs": sz["nof_llrs"], "sch": sz["nof_sch"], "harq_ack": sz["nof_harq_ack"],
+            "csi_part1": sz["nof_csi_part1"], "ms": round(ms, 4),
+            "hbm_gb_s": round(nbytes / 1e9 / (ms * 1e-3), 1), "cpu_reference": ref}
 
 
 def device_attr(attr):
