@@ -506,10 +506,55 @@ __device__ uint32_t syndrome_fail_pair(const int8_t* soft, int goff, const uint3
 // their previous value.  all_ones: ones instead (no-CRC failure path).
 // PAIR: soft bits in the row-pair layout of lifting size Z; stride * 8 is a multiple of
 // Z (the row of a thread's first bit is the same in every pass).
+// Bit 7 of each byte of w: set where the byte, as int8, is <= 0 (sign bit, or zero: no
+// bit of the low seven set; the per-byte sums 0x7f + x <= 0xfe never carry).
+__device__ __forceinline__ uint32_t le0_mask(uint32_t w)
+{
+  const uint32_t nz = ((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w;
+  return (w | ~nz) & 0x80808080u;
+}
+
 template <bool PAIR>
 __device__ void write_hard_bits(const int8_t* soft, uint8_t* out, int nbits, bool all_ones, int first, int stride,
                                 int Z)
 {
+  if (!all_ones && (nbits & 7) == 0 && (!PAIR || (Z & 15) == 0)) {
+    // Whole output bytes from wide LDS reads (soft is 16-byte aligned).
+    if (!PAIR) {
+      // Bits 8b .. 8b + 7 are the eight bytes at 8b: one 8-byte read per output byte.
+      const uint2* s8 = reinterpret_cast<const uint2*>(soft);
+      for (int b = first; b < (nbits >> 3); b += stride) {
+        const uint2 w = s8[b];
+        const uint32_t m0 = le0_mask(w.x), m1 = le0_mask(w.y);
+        // Bit k (k = 0 .. 7, soft bit 8b + k) goes to bit 7 - k of the byte.
+        const uint32_t v = ((m0 >> 7) & 1u) | ((m0 >> 14) & 2u) | ((m0 >> 21) & 4u) | ((m0 >> 28) & 8u) |
+                           ((m1 >> 3) & 16u) | ((m1 >> 10) & 32u) | ((m1 >> 17) & 64u) | ((m1 >> 24) & 128u);
+        out[b] = (uint8_t)(__builtin_bitreverse32(v) >> 24);
+      }
+    } else {
+      // Row pairs: rows j .. j + 7 of half 0 and of half 1 (j a multiple of 8 below Z/2) of
+      // a column are the even and the odd bytes of 16 bytes at column * Z + 2j: one
+      // 16-byte read gives output bytes (column Z + j) / 8 and (column Z + Z/2 + j) / 8.
+      const int h = Z >> 1, hb = h >> 3;
+      const int tasks = (nbits / Z) * hb;
+      for (int t = first; t < tasks; t += stride) {
+        const int c = t / hb, j = (t - c * hb) * 8;
+        const uint4 w = *reinterpret_cast<const uint4*>(soft + c * Z + 2 * j);
+        const uint32_t m[4] = {le0_mask(w.x), le0_mask(w.y), le0_mask(w.z), le0_mask(w.w)};
+        uint32_t v0 = 0, v1 = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          // Dword q: rows 2q (byte 0), 2q + 1 (byte 2) of half 0, the same rows of half 1
+          // in bytes 1 and 3.
+          v0 |= (((m[q] >> 7) & 1u) | ((m[q] >> 22) & 2u)) << (2 * q);
+          v1 |= (((m[q] >> 15) & 1u) | ((m[q] >> 30) & 2u)) << (2 * q);
+        }
+        out[(c * Z + j) >> 3] = (uint8_t)(__builtin_bitreverse32(v0) >> 24);
+        out[(c * Z + h + j) >> 3] = (uint8_t)(__builtin_bitreverse32(v1) >> 24);
+      }
+    }
+    return;
+  }
   const int nbytes = (nbits + 7) >> 3;
   const int j0 = (8 * first) % Z;
   for (int b = first; b < nbytes; b += stride) {
@@ -647,27 +692,33 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
         }
       }
     }
-    // The zero tail: soft bits below cap and the HARQ buffer.
+    // The zero tail: soft bits below cap, then the HARQ buffer in 16-byte stores from
+    // its first 16-byte boundary (bytes before and after it one by one).
+    const int zend = min(n_in, cap);
     const int jt0 = PAIR ? (n_main + 4 * (int)r) % Z : 0;
-    for (int p0 = n_main + 4 * (int)r; p0 < n_in; p0 += 4 * (int)NT) {
-      if (p0 < cap) {
+    for (int p0 = n_main + 4 * (int)r; p0 < zend; p0 += 4 * (int)NT) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (p0 + k < min(n_in, cap)) {
-            int j = jt0 + k;
-            j = j >= Z ? j - Z : j;
-            soft2[soft_pos<PAIR>(p0 + k, j, Z)] = 0;
-          }
+      for (int k = 0; k < 4; ++k) {
+        if (p0 + k < zend) {
+          int j = jt0 + k;
+          j = j >= Z ? j - Z : j;
+          soft2[soft_pos<PAIR>(p0 + k, j, Z)] = 0;
         }
       }
-      if (wr) {
-        if (al4 && p0 + 4 <= n_in) {
-          *reinterpret_cast<uint32_t*>(harq + p0) = 0u;
-        } else {
-          for (int k = 0; k < 4 && p0 + k < n_in; ++k) {
-            harq[p0 + k] = 0;
-          }
-        }
+    }
+    if (wr && n_main < n_in) {
+      const int lead = (int)((16u - (uint32_t)((reinterpret_cast<uintptr_t>(harq) + (uint32_t)n_main) & 15u)) & 15u);
+      const int a16 = min(n_main + lead, n_in);
+      const int nb = (n_in - a16) >> 4;
+      for (int p = n_main + (int)r; p < a16; p += (int)NT) {
+        harq[p] = 0;
+      }
+      uint4* h4 = reinterpret_cast<uint4*>(harq + a16);
+      for (int b = (int)r; b < nb; b += (int)NT) {
+        h4[b] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      for (int p = a16 + 16 * nb + (int)r; p < n_in; p += (int)NT) {
+        harq[p] = 0;
       }
     }
     p_from = n_in; // nothing left for the general loop
