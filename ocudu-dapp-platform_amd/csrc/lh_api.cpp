@@ -537,23 +537,29 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
     it.rows2 = (uint8_t)m;
     int lmax = 4, maxE = 0;
     bool fused = false;
+    // Scatter load: new data where every position takes at most one LLR (E <= P).
+    bool scatter = Z % 8 == 0;
     size_t k = i;
     while (k < jobs.size() && (int)(k - i) < G && jobs[k].Z == Z && jobs[k].gt_off == jobs[i].gt_off &&
            jobs[k].lbound == jobs[i].lbound) {
       lmax = std::max<int>(lmax, jobs[k].lbound);
-      if (jobs[k].flags & kFlagFused) {
+      const uint8_t fl = jobs[k].flags;
+      if (fl & kFlagFused) {
         fused = true;
         maxE = std::max<int>(maxE, (int)jobs[k].rm.E);
       }
+      scatter = scatter && (fl & kFlagFused) && (fl & kFlagNewData) && !(fl & kFlagPassed) &&
+                jobs[k].rm.E <= jobs[k].rm.P;
       ++k;
     }
     it.ngroups = (uint16_t)(k - i);
     it.lmax = (uint16_t)lmax;
+    it.scatter = scatter ? 1 : 0;
     size_t need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, 0);
     if (need > kLdsBudget) {
       return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", Z, lmax);
     }
-    const int stage = (fused && te.stage != 0) ? ((maxE + 15) & ~15) : 0;
+    const int stage = (fused && !scatter && te.stage != 0) ? ((maxE + 15) & ~15) : 0;
     if (stage > 0 && decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage) <= kLdsBudget) {
       it.stage = (uint32_t)stage;
       need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage);

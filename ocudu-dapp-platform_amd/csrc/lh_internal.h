@@ -86,7 +86,7 @@ struct alignas(16) dec_item {
   uint8_t bg;
   uint8_t rows2; // 1: two check rows per thread (even Z)
   uint16_t lmax;  // layer bound of the item (max over its jobs)
-  uint16_t pad1;
+  uint16_t scatter; // 1: every job is a fused new-data load with E <= P and Z % 8 == 0 (scatter load)
   uint32_t stage; // LDS bytes per group for the codeword LLRs (0: read them from HBM)
 };
 static_assert(sizeof(dec_item) == 16, "dec_item layout");

@@ -761,6 +761,193 @@ __device__ __forceinline__ void fused_positions(const rm_params& rp, const int8_
   }
 }
 
+// --- Scatter load of new data (every position receives at most one LLR) ------------
+// For new data with E <= P the dematched buffer is a permutation of the codeword LLRs
+// into zeros and +127 fillers: LLR e[i] = cw[(i mod Kq) Qm + i div Kq] lands on rank
+// q = start + i (mod P) and position p = q (q < info) or q + F.  The codeword is read in
+// aligned 16-byte blocks and each byte is written to its position in natural order, in
+// the LDS of the soft bits; the HARQ buffer is then written from there in 8-byte units,
+// and the soft bits are clamped to +-64 (and put in the row-pair order) in place.
+
+// Four int8 values clamped to [-64, 64] in packed 16-bit lanes.
+__device__ __forceinline__ uint32_t clamp64x4(uint32_t w)
+{
+  const i16x2 lo = as_i16x2(w << 8) >> (short)8; // bytes 0, 2
+  const i16x2 hi = as_i16x2(w) >> (short)8;      // bytes 1, 3
+  const i16x2 a = clamp2(lo, -64, 64), b = clamp2(hi, -64, 64);
+  return __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06020400u);
+}
+
+// Natural position of rank q (q < P): past the filler run when q >= info.
+__device__ __forceinline__ uint32_t rank_pos(uint32_t q, uint32_t info, uint32_t F)
+{
+  return q < info ? q : q + F;
+}
+
+// Codeword bytes c0 .. c0 + 15 of one 16-byte block (c0 = 16 b - lead) to their natural
+// positions.  LQ = log2(Qm) when Qm divides c0 and the ranks do not wrap (start + E <= P):
+// byte m is then LLR i = (m mod Qm) Kq + c0 / Qm + m / Qm.  LQ < 0: any Qm, lead and
+// wrap, with an exact division per byte.
+template <int LQ>
+__device__ __forceinline__ void scatter_block(const uint4 w, int c0, const rm_params& rp, uint32_t nat, bool whole)
+{
+  const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+  const uint32_t F = rp.nsys - rp.info;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int c = c0 + m;
+    if (whole || (unsigned)c < rp.E) {
+      uint32_t q;
+      if constexpr (LQ >= 0) {
+        constexpr int QM = 1 << LQ;
+        q = (uint32_t)(c0 >> LQ) + ((uint32_t)(m & (QM - 1)) * rp.Kq + (uint32_t)(m >> LQ) + rp.start);
+      } else {
+        const int qm = (int)rp.Qm;
+        int k = (int)((float)c * (1.0f / (float)qm));
+        int jq = c - k * qm;
+        if (jq < 0) {
+          --k;
+          jq += qm;
+        } else if (jq >= qm) {
+          ++k;
+          jq -= qm;
+        }
+        q = (uint32_t)jq * rp.Kq + (uint32_t)k + rp.start;
+        q = min(q, q - rp.P); // wrap (q < 2P)
+      }
+      *lds_at(nat + rank_pos(q, rp.info, F)) = (int8_t)(ww[m >> 2] >> (8 * (m & 3)));
+    }
+  }
+}
+
+// The scatter load of one codeblock (group), positions [0, n_in) of the HARQ buffer and
+// [0, cap) of the soft bits (cap <= n_in, cap a multiple of Z, Z a multiple of 8).  All
+// threads of the workgroup take part (barriers); `work` groups load.  PASS_COLS columns
+// per row-pair pass: each thread holds two words per round of the pass.
+template <bool PAIR>
+__device__ __forceinline__ void scatter_load(const rm_params& rp, const int8_t* __restrict__ cw,
+                                             int8_t* __restrict__ harq, bool wr, bool work, int8_t* soft,
+                                             unsigned r, unsigned NT, int Z, int cap, int n_in, int& last)
+{
+  int8_t* const nat = soft + 2 * Z; // 16-byte aligned (Z % 8 == 0)
+  const uint32_t nat_a = lds_addr(nat);
+  // 1. Zeros, and +127 over the filler positions [info, nsys).
+  if (work) {
+    uint2* n8 = reinterpret_cast<uint2*>(nat);
+    const int units = cap >> 3;
+    for (int d = (int)r; d < units; d += (int)NT) {
+      const uint32_t p0 = 8u * (uint32_t)d;
+      uint32_t v[2] = {0u, 0u};
+      if (p0 + 8 > rp.info && p0 < rp.nsys) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t p = p0 + (uint32_t)k;
+          v[k >> 2] |= (p >= rp.info && p < rp.nsys) ? (0x7fu << (8 * (k & 3))) : 0u;
+        }
+      }
+      n8[d] = make_uint2(v[0], v[1]);
+    }
+  }
+  __syncthreads();
+  // 2. LLRs to their positions.
+  if (work) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(cw);
+    const uint4* blk = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
+    const int lead = (int)(a & 15);
+    const int nblk = (lead + (int)rp.E + 15) >> 4;
+    const int qm = (int)rp.Qm;
+    const bool pow2 = (qm & (qm - 1)) == 0 && (lead & (qm - 1)) == 0 && rp.start + rp.E <= rp.P;
+    for (int b = (int)r; b < nblk; b += (int)NT) {
+      const uint4 w = blk[b];
+      const int c0 = 16 * b - lead;
+      const bool whole = c0 >= 0 && c0 + 16 <= (int)rp.E;
+      if (pow2) {
+        switch (qm) {
+        case 1: scatter_block<0>(w, c0, rp, nat_a, whole); break;
+        case 2: scatter_block<1>(w, c0, rp, nat_a, whole); break;
+        case 4: scatter_block<2>(w, c0, rp, nat_a, whole); break;
+        default: scatter_block<3>(w, c0, rp, nat_a, whole); break;
+        }
+      } else {
+        scatter_block<-1>(w, c0, rp, nat_a, whole);
+      }
+    }
+  }
+  __syncthreads();
+  // 3. HARQ buffer in 8-byte units (zeros from cap on), the last nonzero position, and in
+  // the one-row layout the clamp in place (each unit is read and written by one thread).
+  if (work) {
+    uint2* n8 = reinterpret_cast<uint2*>(nat);
+    const int units = (n_in + 7) >> 3;
+    const int cap_units = cap >> 3;
+    const uintptr_t ha = reinterpret_cast<uintptr_t>(harq);
+    for (int d = (int)r; d < units; d += (int)NT) {
+      uint2 v = make_uint2(0u, 0u);
+      if (d < cap_units) {
+        v = n8[d];
+        if (v.y != 0u) {
+          last = max(last, 8 * d + 4 + ((31 - __builtin_clz(v.y)) >> 3));
+        } else if (v.x != 0u) {
+          last = max(last, 8 * d + ((31 - __builtin_clz(v.x)) >> 3));
+        }
+        if (!PAIR) {
+          n8[d] = make_uint2(clamp64x4(v.x), clamp64x4(v.y));
+        }
+      }
+      if (wr) {
+        const int p = 8 * d;
+        if (p + 8 <= n_in && ((ha + (uint32_t)p) & 7) == 0) {
+          *reinterpret_cast<uint2*>(harq + p) = v;
+        } else if (p + 8 <= n_in && ((ha + (uint32_t)p) & 3) == 0) {
+          *reinterpret_cast<uint32_t*>(harq + p) = v.x;
+          *reinterpret_cast<uint32_t*>(harq + p + 4) = v.y;
+        } else {
+          for (int k = 0; k < 8 && p + k < n_in; ++k) {
+            harq[p + k] = (int8_t)((k < 4 ? v.x : v.y) >> (8 * (k & 3)));
+          }
+        }
+      }
+    }
+  }
+  if constexpr (PAIR) {
+    // 4. Row-pair order, in place: unit t of column c holds rows 4t .. 4t + 3 of both
+    // halves, bytes [A0 B0 A1 B1 A2 B2 A3 B3] at c Z + 8t from A = rows 4t.., B = rows
+    // h + 4t..  The NT = Z/2 threads cover four columns per round (thread r: column
+    // r / (Z/8) of the four, unit r mod (Z/8)).  A pass reads its columns, waits, then
+    // writes them (a pass covers 4 kRounds whole columns, so passes share no column).
+    constexpr int kRounds = 6;
+    const int h = Z >> 1;
+    const int upc = Z >> 3; // units per column
+    const int ncol = cap / Z;
+    const int cr = (int)r / upc;
+    const int o = cr * Z + 4 * ((int)r - cr * upc); // byte of A in column 0 of the round
+    const uint32_t* n4 = reinterpret_cast<const uint32_t*>(nat);
+    for (int c0 = 0; c0 < ncol; c0 += 4 * kRounds) {
+      uint32_t A[kRounds], B[kRounds];
+#pragma unroll
+      for (int k = 0; k < kRounds; ++k) {
+        A[k] = B[k] = 0u;
+        if (work && c0 + 4 * k + cr < ncol) {
+          const int a = (c0 + 4 * k) * Z + o;
+          A[k] = n4[a >> 2];
+          B[k] = n4[(a + h) >> 2];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kRounds; ++k) {
+        if (work && c0 + 4 * k + cr < ncol) {
+          const int a = (c0 + 4 * k) * Z + o;
+          const uint32_t lo = clamp64x4(__builtin_amdgcn_perm(B[k], A[k], 0x05010400u));
+          const uint32_t hi = clamp64x4(__builtin_amdgcn_perm(B[k], A[k], 0x07030602u));
+          // Output bytes c Z + 8t .. (from A at c Z + 4t): offset a + 4t = 2a - c Z.
+          *reinterpret_cast<uint2*>(nat + 2 * a - (c0 + 4 * k + cr) * Z) = make_uint2(lo, hi);
+        }
+      }
+    }
+  }
+}
+
 // Per-group reductions of the persistent decoder: LDS atomics, with a wave-level
 // shuffle reduction first when the whole wave belongs to one group.
 __device__ __forceinline__ bool wave_in_one_group(int g)
@@ -897,7 +1084,11 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   const int full = (n_in / Z) * Z;
   const int cap = (K + lmax - 2) * Z;
   int last = -1;
-  if (jb.flags & kFlagFused) {
+  if (item.scatter != 0 && fb_iters == 0) {
+    // New data, at most one LLR per position: scatter load (uniform per item).
+    scatter_load<PAIR>(jb.rm, cw_base + jb.rm.cw_off, rm_base + jb.rm.rm_off, (jb.flags & kFlagHarqWrite) != 0, work,
+                       soft, r, (unsigned)NT, Z, cap, n_in, last);
+  } else if (jb.flags & kFlagFused) {
     const rm_params rp = jb.rm;
     const int8_t* in = cw_base + rp.cw_off;
     int8_t* harq = rm_base + rp.rm_off;
