@@ -42,7 +42,7 @@ hipError_t launch_demod(const demod_tile* tiles, unsigned nof_tiles, const demod
 hipError_t launch_demux(const demux_tile* tiles, unsigned nof_tiles, const demux_sym* syms, const uint32_t* tab,
                         const int8_t* llrs, int8_t* sch, int8_t* ack, int8_t* c1, int8_t* c2, hipStream_t stream);
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
-                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s);
+                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, bool any_small, hipStream_t s);
 size_t nms_train_lds_bytes(int bg, int Z, int L);
 hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt, size_t lds, int T,
                             const float* alpha, const float* beta, const float* wt, float tau, const int8_t* in,
@@ -1545,8 +1545,12 @@ int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned
   if (rc != LH_OK) {
     return rc;
   }
+  bool any_small = false;
+  for (unsigned i = 0; i < n; ++i) {
+    any_small = any_small || descs[i].lifting_size < 32;
+  }
   hipError_t e = launch_encode_rate_match(reinterpret_cast<const gen_job*>(s->dev), n, const_cast<uint8_t*>(d_msgs),
-                                          d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, st);
+                                          d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, any_small, st);
   retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "encoder launch");
 }

@@ -98,17 +98,254 @@ __device__ __forceinline__ int edge_shift(const uint32_t* gt, int rs0, int rs1, 
 
 constexpr int kGenMaxBytes = 68 * kMaxZ; // (K_b + M) * Z of BG1 at Z = 384
 
+// --- Packed-word encoder (lifting sizes Z >= 32) -------------------------------------
+// A lifted column is a Z-bit vector held in kColWords words (bit j of the column = bit
+// j mod 32 of word j / 32, bits past Z zero), one zero word behind it.  The cyclic shift
+// of an edge is then a 32-bit window of the vector: one funnel shift, plus the wrapped
+// start of the vector OR-ed in near its end.
+constexpr int kColWords = kMaxZ / 32 + 1;              // words per column, with the zero word
+constexpr int kMsgWords = 22 * kMaxZ / 32 + 1;         // packed message of BG1 at Z = 384, + 1
+constexpr int kPackedWords = kMsgWords + (22 + 4 + 5) * kColWords + 316; // message, columns, lambda, edges
+
+// Bits a .. a + 31 (mod Z) of the column vector v (0 <= a < Z, Z >= 32).
+__device__ __forceinline__ uint32_t cyc32(const uint32_t* v, int Z, int a)
+{
+  const int q = a >> 5;
+  uint32_t x = __builtin_amdgcn_alignbit(v[q + 1], v[q], (uint32_t)(a & 31));
+  if (a + 32 > Z) {
+    x |= v[0] << (Z - a);
+  }
+  return x;
+}
+
+// Valid bits of word w of a Z-bit vector.
+__device__ __forceinline__ uint32_t word_mask(int w, int Z)
+{
+  const int n = Z - 32 * w;
+  return n >= 32 ? 0xffffffffu : ((1u << n) - 1u);
+}
+
+// (x mod Z) for -Z <= x < 2Z.
+__device__ __forceinline__ int wrap_z(int x, int Z)
+{
+  x = x < 0 ? x + Z : x;
+  return x >= Z ? x - Z : x;
+}
+
+// Bytes 0/1 of 32 bits, four per word store: bit k of x to byte k of the four at `out`.
+__device__ __forceinline__ void spread_bits(uint32_t x, uint8_t* out, int n)
+{
+  if ((reinterpret_cast<uintptr_t>(out) & 3) == 0 && n == 32) {
+    uint32_t* o4 = reinterpret_cast<uint32_t*>(out);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      // Bit k of the nibble to bit 8k: the products of distinct bits never meet.
+      o4[q] = (((x >> (4 * q)) & 15u) * 0x00204081u) & 0x01010101u;
+    }
+  } else {
+    for (int k = 0; k < n; ++k) {
+      out[k] = (uint8_t)((x >> k) & 1u);
+    }
+  }
+}
+
+// Column of a lifted edge word (col * Z in the low half): col * Z / Z.
+__device__ __forceinline__ int edge_col(uint32_t g, float rz)
+{
+  return (int)((float)(g & 0xffffu) * rz + 0.5f);
+}
+
+// LDPC encoding of the message of jb (msg_input) into the codeword bytes cw (one bit per
+// byte, (K + M) Z), TS38.212 5.3.2 with the double-diagonal core of
+// ldpc_encoder_generic.cpp: lambda_i = systematic sums of the four core rows, column K
+// from their sum, the other core columns one row at a time, then every extension row
+// independently.  Each step is word-parallel over the 32-row words of a column.  pk:
+// kPackedWords words of LDS.  Z >= 32.
+template <int BG>
+__device__ void encode_packed(const gen_job& jb, const uint8_t* __restrict__ msgs, const uint32_t* __restrict__ gtab,
+                              uint8_t* cw, uint32_t* pk)
+{
+  constexpr int K = BG == 1 ? 22 : 10, M = BG == 1 ? 46 : 42;
+  const uint16_t* rs = BG == 1 ? lh_bg1_row_start : lh_bg2_row_start;
+  const int t = (int)threadIdx.x, NT = (int)blockDim.x;
+  const int Z = jb.Z;
+  const int W = (Z + 31) >> 5;
+  const int kz = K * Z;
+  const int nmw = (kz + 31) >> 5;
+  const float rz = 1.0f / (float)Z;
+  uint32_t* pm = pk;                         // packed message, LSB first, + one zero word
+  uint32_t* col = pk + kMsgWords;            // columns 0 .. K + 3
+  uint32_t* lam = col + (K + 4) * kColWords; // lambda_0..3, then their sum
+  uint32_t* gt = lam + 5 * kColWords;        // the graph's lifted edge table, staged in LDS
+  {
+    const uint32_t* g = gtab + jb.gt_off;
+    for (int e = t; e < rs[M]; e += NT) {
+      gt[e] = g[e];
+    }
+  }
+
+  // 1. Message bytes (MSB first) to LSB-first words, and to the codeword bytes.
+  {
+    const uint8_t* src = msgs + jb.msg_off;
+    const int nbytes = (kz + 7) >> 3;
+    const bool al4 = (jb.msg_off & 3u) == 0;
+    for (int w = t; w <= nmw; w += NT) {
+      uint32_t x = 0;
+      if (w < nmw) {
+        if (al4 && 4 * w + 4 <= nbytes) {
+          x = reinterpret_cast<const uint32_t*>(src)[w];
+        } else {
+          for (int k = 0; k < 4 && 4 * w + k < nbytes; ++k) {
+            x |= (uint32_t)src[4 * w + k] << (8 * k);
+          }
+        }
+        // Byte k of x holds bits 8k .. 8k + 7 MSB first: reverse the bits of each byte.
+        x = __builtin_bitreverse32(__builtin_bswap32(x));
+        if (32 * w + 32 > kz) {
+          x &= (1u << (kz - 32 * w)) - 1u;
+        }
+        spread_bits(x, cw + 32 * w, min(32, kz - 32 * w));
+      }
+      pm[w] = x;
+    }
+    for (int j = kz + t; j < (K + M) * Z; j += NT) {
+      cw[j] = 0;
+    }
+  }
+  __syncthreads();
+  // 2. Info columns from the message string; zero words behind every column.
+  for (int task = t; task < (K + 4) * kColWords; task += NT) {
+    const int c = task / kColWords, w = task - c * kColWords;
+    uint32_t x = 0;
+    if (c < K && w < W) {
+      const int a = c * Z + 32 * w;
+      x = __builtin_amdgcn_alignbit(pm[(a >> 5) + 1], pm[a >> 5], (uint32_t)(a & 31)) & word_mask(w, Z);
+    }
+    col[task] = x;
+  }
+  for (int task = t; task < 5 * kColWords; task += NT) {
+    lam[task] = 0;
+  }
+  __syncthreads();
+  // 3. lambda_i[r] = XOR over the systematic edges of core row i of column[(r + s) mod Z].
+  for (int task = t; task < 4 * W; task += NT) {
+    const int i = task / W, w = task - i * W;
+    uint32_t x = 0;
+    for (int e = rs[i]; e < rs[i + 1]; ++e) {
+      const uint32_t g = gt[e];
+      const int c = edge_col(g, rz);
+      if (c < K) {
+        x ^= cyc32(col + c * kColWords, Z, wrap_z(32 * w + (int)(g >> 16), Z));
+      }
+    }
+    lam[i * kColWords + w] = x & word_mask(w, Z);
+  }
+  __syncthreads();
+  // Shifts of the core columns in the core rows (-1: no edge), uniform.
+  int sh[4][4];
+  for (int i = 0; i < 4; ++i) {
+    for (int c = 0; c < 4; ++c) {
+      sh[i][c] = edge_shift(gt, rs[i], rs[i + 1], (uint32_t)((K + c) * Z));
+    }
+  }
+  // The sum of the four core rows leaves one circulant on column K: the shift that occurs
+  // an odd number of times among the column-K entries.
+  int tshift = -1;
+  for (int i = 0; i < 4; ++i) {
+    if (sh[i][0] < 0) {
+      continue;
+    }
+    int dup = 0;
+    for (int i2 = 0; i2 < 4; ++i2) {
+      dup += (i2 != i && sh[i2][0] == sh[i][0]) ? 1 : 0;
+    }
+    if ((dup & 1) == 0) {
+      tshift = sh[i][0];
+    }
+  }
+  uint32_t* lsum = lam + 4 * kColWords;
+  for (int w = t; w < W; w += NT) {
+    lsum[w] = lam[w] ^ lam[kColWords + w] ^ lam[2 * kColWords + w] ^ lam[3 * kColWords + w];
+  }
+  __syncthreads();
+  // 4. Column K: p[q] = sum[(q - tshift) mod Z].
+  for (int w = t; w < W; w += NT) {
+    col[K * kColWords + w] = cyc32(lsum, Z, wrap_z(32 * w - tshift, Z)) & word_mask(w, Z);
+  }
+  __syncthreads();
+  // The other core columns: a core row with exactly one unknown column gives it,
+  // column[j] = lambda_i[(j - s_u) mod Z] ^ XOR over the known core columns c of
+  // column_c[(j - s_u + s_c) mod Z].
+  bool known[4] = {true, false, false, false};
+  for (int pass = 0; pass < 4; ++pass) {
+    for (int i = 0; i < 4; ++i) {
+      int unk = -1, nunk = 0;
+      for (int c = 0; c < 4; ++c) {
+        if (sh[i][c] >= 0 && !known[c]) {
+          unk = c;
+          ++nunk;
+        }
+      }
+      if (nunk != 1) {
+        continue;
+      }
+      const int su = sh[i][unk];
+      for (int w = t; w < W; w += NT) {
+        uint32_t x = cyc32(lam + i * kColWords, Z, wrap_z(32 * w - su, Z));
+        for (int c = 0; c < 4; ++c) {
+          if (sh[i][c] >= 0 && known[c]) {
+            x ^= cyc32(col + (K + c) * kColWords, Z, wrap_z(wrap_z(32 * w - su, Z) + sh[i][c], Z));
+          }
+        }
+        col[(K + unk) * kColWords + w] = x & word_mask(w, Z);
+      }
+      known[unk] = true;
+      __syncthreads();
+    }
+  }
+  // 5. Core columns to the codeword bytes; extension rows i >= 4 (identity part on column
+  //    K + i with shift s_own): column_{K+i}[j] = XOR over the other edges of
+  //    column_c[(j - s_own + s) mod Z].
+  for (int task = t; task < (M - 4 + 4) * W; task += NT) {
+    const int row = task / W, w = task - row * W;
+    const int n = min(32, Z - 32 * w);
+    if (row < 4) {
+      spread_bits(col[(K + row) * kColWords + w], cw + (K + row) * Z + 32 * w, n);
+      continue;
+    }
+    const int i = row;
+    int own = 0;
+    for (int e = rs[i]; e < rs[i + 1]; ++e) {
+      const uint32_t g = gt[e];
+      if (edge_col(g, rz) == K + i) {
+        own = (int)(g >> 16);
+      }
+    }
+    const int base = wrap_z(32 * w - own, Z);
+    uint32_t x = 0;
+    for (int e = rs[i]; e < rs[i + 1]; ++e) {
+      const uint32_t g = gt[e];
+      const int c = edge_col(g, rz);
+      if (c < K + 4) {
+        x ^= cyc32(col + c * kColWords, Z, wrap_z(base + (int)(g >> 16), Z));
+      }
+    }
+    spread_bits(x & word_mask(w, Z), cw + (K + i) * Z + 32 * w, n);
+  }
+  __syncthreads();
+}
+
 } // namespace
 
 // ENCODE: the transmit side only (ldpc_encoder + ldpc_rate_matcher): the message comes
 // from msgs, and the rate-matched bits are written packed (MSB first) to out_bits +
 // rm.cw_off, with the codeword (N_short * Z bits, one per byte, as
 // ldpc_encoder_buffer::write_codeblock) at cw_bits + cwb_off when asked.
-template <int BG, bool ENCODE>
+template <int BG, bool ENCODE, bool PACKED = false>
 __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint8_t* __restrict__ msgs,
                               const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab, uint8_t* cw,
                               uint8_t* lam, uint32_t* red, uint8_t* __restrict__ out_bits = nullptr,
-                              uint8_t* __restrict__ cw_bits = nullptr)
+                              uint8_t* __restrict__ cw_bits = nullptr, uint32_t* pk = nullptr)
 {
   constexpr int K = BG == 1 ? 22 : 10, M = BG == 1 ? 46 : 42;
   const uint16_t* rs = BG == 1 ? lh_bg1_row_start : lh_bg2_row_start;
@@ -118,9 +355,16 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   const int kd = K * Z - jb.F - order;
   const uint32_t* gt = gtab + jb.gt_off;
 
+  // The transmit side encodes word-parallel for lifting sizes from 32 on (PACKED: the
+  // message is an input and Z >= 32).
+  constexpr bool packed = ENCODE && PACKED;
+  if constexpr (packed) {
+    encode_packed<BG>(jb, msgs, gtab, cw, pk);
+  }
+
   // 1. Message: data bits, CRC, zero fillers; the parity part starts zeroed.  With
   //    msg_input the whole message (data, CRC, fillers) comes from msgs + msg_off.
-  if (jb.msg_input) {
+  if (jb.msg_input && !packed) {
     // One 4-byte load per thread step (the message is read once, not once per bit) where
     // the message is 4-byte aligned and the word lies inside it; bytes otherwise.
     const uint8_t* src = msgs + jb.msg_off;
@@ -169,7 +413,7 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   }
 
   // 2. Core parity (rows 0..3, double diagonal): lambda_i = systematic sum of row i.
-  for (int r = t; r < Z; r += NT) {
+  for (int r = t; !packed && r < Z; r += NT) {
     for (int i = 0; i < 4; ++i) {
       uint8_t acc = 0;
       for (int e = rs[i]; e < rs[i + 1]; ++e) {
@@ -206,14 +450,15 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
     }
   }
   __syncthreads();
-  for (int q = t; q < Z; q += NT) {
+  for (int q = t; !packed && q < Z; q += NT) {
     int r = q - tshift;
     r = r < 0 ? r + Z : r;
     cw[K * Z + q] = lam[r] ^ lam[kMaxZ + r] ^ lam[2 * kMaxZ + r] ^ lam[3 * kMaxZ + r];
   }
   __syncthreads();
   // Remaining core columns: repeatedly a core row with exactly one unknown column.
-  bool known[4] = {true, false, false, false};
+  bool known[4] = {packed, packed, packed, packed};
+  known[0] = true;
   for (int pass = 0; pass < 4; ++pass) {
     for (int i = 0; i < 4; ++i) {
       int unk = -1, nunk = 0;
@@ -245,7 +490,7 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
   }
 
   // 3. Extension parity (rows 4..M-1, identity diagonal): independent rows.
-  for (int idx = t; idx < (M - 4) * Z; idx += NT) {
+  for (int idx = t; !packed && idx < (M - 4) * Z; idx += NT) {
     const int i = 4 + idx / Z, r = idx - (i - 4) * Z;
     uint8_t acc = 0;
     int own = 0;
@@ -280,26 +525,35 @@ __device__ void gen_codeblock(const gen_job& jb, int8_t* __restrict__ llrs, uint
     }
     uint8_t* ro = out_bits + rp.cw_off;
     const uint32_t qm = rp.Qm;
-    // Rank of e_{q Kq} for each q (reduced mod P once), then bit by bit: rank = base[q] + s,
-    // wrapped by subtraction (E / P + 1 times at most).
-    uint32_t base[8];
-#pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) {
-      base[q] = q < qm ? (uint32_t)(((uint64_t)rp.start + (uint64_t)q * rp.Kq) % rp.P) : 0u;
-    }
+    const uint32_t F = rp.nsys - rp.info;
+    // Bit e_i, i = q Kq + s: circular-buffer rank start + i (mod P), fillers skipped.
+    const auto rm_bit = [&](uint32_t q, uint32_t s) -> uint32_t {
+      uint32_t rank = rp.start + q * rp.Kq + s;
+      if (rank >= rp.P) {
+        rank = rank - rp.P < rp.P ? rank - rp.P : rank % rp.P;
+      }
+      return d[rank < rp.info ? rank : rank + F];
+    };
+    // Qm a power of two: output byte b holds symbols 8b / Qm .. of whole bits (s, q) known
+    // per bit position.
+    const bool p2 = (qm & (qm - 1u)) == 0;
+    const uint32_t lq = (uint32_t)__builtin_ctz(qm);
     for (uint32_t b = (uint32_t)t; 8 * b < rp.E; b += (uint32_t)NT) {
       uint32_t v = 0;
-      uint32_t s = (8 * b) / qm, q = 8 * b - s * qm;
-      for (uint32_t k = 0; k < 8 && 8 * b + k < rp.E; ++k) {
-        uint32_t rank = base[q] + s;
-        while (rank >= rp.P) {
-          rank -= rp.P;
+      if (p2 && 8 * b + 8 <= rp.E) {
+        const uint32_t s0 = (8 * b) >> lq;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+          v |= rm_bit(k & (qm - 1u), s0 + (k >> lq)) << (7 - k);
         }
-        const uint32_t p = rank < rp.info ? rank : rank + (rp.nsys - rp.info);
-        v |= (uint32_t)d[p] << (7 - k);
-        if (++q == qm) {
-          q = 0;
-          ++s;
+      } else {
+        uint32_t s = (8 * b) / qm, q = 8 * b - s * qm;
+        for (uint32_t k = 0; k < 8 && 8 * b + k < rp.E; ++k) {
+          v |= rm_bit(q, s) << (7 - k);
+          if (++q == qm) {
+            q = 0;
+            ++s;
+          }
         }
       }
       ro[b] = (uint8_t)v;
@@ -380,20 +634,27 @@ __global__ void __launch_bounds__(384) k_gen_codeblocks(const gen_job* __restric
   }
 }
 
-// LDPC encoder + rate matcher, one workgroup per codeblock (message from msgs).
+// LDPC encoder + rate matcher, one workgroup per codeblock (message from msgs).  PACKED:
+// the codeblocks with Z >= 32 (word-parallel encoder), otherwise those with Z < 32; each
+// instance leaves the others' codeblocks alone.
+template <bool PACKED>
 __global__ void __launch_bounds__(384) k_encode_rate_match(const gen_job* __restrict__ jobs, uint8_t* __restrict__ msgs,
                                                            uint8_t* __restrict__ out_bits, uint8_t* __restrict__ cw_bits,
                                                            const uint32_t* __restrict__ crc_x,
                                                            const uint32_t* __restrict__ gtab)
 {
-  __shared__ uint8_t cw[kGenMaxBytes];
-  __shared__ uint8_t lam[4 * kMaxZ];
+  __shared__ __align__(16) uint8_t cw[kGenMaxBytes];
+  __shared__ uint8_t lam[PACKED ? 1 : 4 * kMaxZ];
   __shared__ uint32_t red[16];
+  __shared__ uint32_t pk[PACKED ? kPackedWords : 1];
   const gen_job jb = jobs[blockIdx.x];
+  if ((jb.Z >= 32) != PACKED) {
+    return;
+  }
   if (jb.bg == 1) {
-    gen_codeblock<1, true>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits);
+    gen_codeblock<1, true, PACKED>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits, pk);
   } else {
-    gen_codeblock<2, true>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits);
+    gen_codeblock<2, true, PACKED>(jb, nullptr, msgs, crc_x, gtab, cw, lam, red, out_bits, cw_bits, pk);
   }
 }
 
@@ -500,10 +761,15 @@ hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs
 }
 
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
-                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, hipStream_t s)
+                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, bool any_small,
+                                    hipStream_t s)
 {
-  hipLaunchKernelGGL(k_encode_rate_match, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits, crc_x,
-                     gtab);
+  hipLaunchKernelGGL(k_encode_rate_match<true>, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits,
+                     crc_x, gtab);
+  if (any_small) {
+    hipLaunchKernelGGL(k_encode_rate_match<false>, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits,
+                       crc_x, gtab);
+  }
   return hipGetLastError();
 }
 

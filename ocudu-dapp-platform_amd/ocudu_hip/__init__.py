@@ -451,10 +451,12 @@ class Context:
         _lh(lib().lh_tb_finalize(self.handle, arr, len(tbs), d_cb_bits.ptr, d_cb_results.ptr, d_tb.ptr, d_status.ptr,
                                  stream.handle if stream else None), "lh_tb_finalize")
 
-    def encode_rate_match(self, descs, d_msgs, d_out, d_codewords=None, stream=None):
-        """LDPC encoding + rate matching (lh_ldpc_encode_rate_match)."""
-        arr = (EncodeDesc * len(descs))(*descs)
-        _lh(lib().lh_ldpc_encode_rate_match(self.handle, arr, len(descs), d_msgs.ptr, d_out.ptr,
+    def encode_rate_match(self, descs, d_msgs, d_out, d_codewords=None, stream=None, arr=None):
+        """LDPC encoding + rate matching (lh_ldpc_encode_rate_match); `arr`: the descriptors
+        already as a ctypes array (no per-call conversion)."""
+        if arr is None:
+            arr = (EncodeDesc * len(descs))(*descs)
+        _lh(lib().lh_ldpc_encode_rate_match(self.handle, arr, len(arr), d_msgs.ptr, d_out.ptr,
                                             d_codewords.ptr if d_codewords else None,
                                             stream.handle if stream else None), "lh_ldpc_encode_rate_match")
 

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = []
 for bg in (1, 2):
-    for Z in (2, 7, 16, 52, 144, 208, 384):
+    for Z in (2, 7, 16, 30, 32, 36, 52, 60, 112, 144, 208, 320, 384):
         N = O.N_short(bg) * Z
         K = O.K_b(bg) * Z
         for rv, qm, rate, F, nref in ((0, 2, 0.5, 0, 0), (1, 4, 0.33, 0, 0), (2, 6, 0.8, Z // 2, 0),
