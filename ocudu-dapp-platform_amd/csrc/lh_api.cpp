@@ -23,12 +23,12 @@
 #include "lh_internal.h"
 
 namespace lh {
-size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage);
+size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage, bool ext8);
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         bool pairs, hipStream_t s);
-int decode_occupancy(int nt, size_t lds, bool learned, bool pairs);
+                         bool pairs, bool ext8, hipStream_t s);
+int decode_occupancy(int nt, size_t lds, bool learned, bool pairs, bool ext8);
 hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
                               uint8_t* tb, int32_t* status, const uint32_t* lut, const uint32_t* pow2, hipStream_t s,
@@ -416,7 +416,7 @@ bool pair_rows(const lh_ctx* ctx, const dec_job& j, size_t n)
   }
   const int stage = ((j.flags & kFlagFused) && te.stage != 0) ? (((int)j.rm.E + 15) & ~15) : 0;
   const double by_waves = 1024.0 / Z;
-  const double by_lds = (double)kLdsBudget / (double)decode_item_lds_bytes(bg, Z, j.lbound, 1, stage);
+  const double by_lds = (double)kLdsBudget / (double)decode_item_lds_bytes(bg, Z, j.lbound, 1, stage, false);
   return by_lds >= 1.45 * by_waves && (double)n >= by_waves * ctx->nof_cus;
 }
 
@@ -433,12 +433,13 @@ struct plan_block {
   unsigned grid[2] = {0, 0};        // persistent workgroups per row mode
   int nt[2] = {64, 64};
   size_t lds[2] = {0, 0};
+  bool ext8 = false; // one-row launch with int8 extension-layer messages
 };
 
 // decode_occupancy, remembered per launch shape (the runtime query costs microseconds).
-int occupancy(lh_ctx* ctx, int nt, size_t lds, bool learned, bool pairs)
+int occupancy(lh_ctx* ctx, int nt, size_t lds, bool learned, bool pairs, bool ext8 = false)
 {
-  const int key = (learned ? 1 : 0) | (pairs ? 2 : 0);
+  const int key = (learned ? 1 : 0) | (pairs ? 2 : 0) | (ext8 ? 4 : 0);
   for (const auto& c : ctx->occ_cache) {
     if (c.key == key && c.nt == nt && c.lds == lds) {
       return c.occ;
@@ -449,7 +450,7 @@ int occupancy(lh_ctx* ctx, int nt, size_t lds, bool learned, bool pairs)
   c.nt = nt;
   c.lds = lds;
   c.key = key;
-  c.occ = decode_occupancy(nt, lds, learned, pairs);
+  c.occ = decode_occupancy(nt, lds, learned, pairs, ext8);
   return c.occ;
 }
 
@@ -528,7 +529,7 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
     const int lanes = m ? Z / 2 : Z;
     int G = te.groups > 0 ? std::min<int>(te.groups, groups_per_item(lanes, nt[m])) : groups_per_item(lanes, nt[m]);
     // Fewer codeblocks per item when their soft bits would not fit the LDS.
-    while (G > 1 && decode_item_lds_bytes(bg, Z, jobs[i].lbound, G, 0) > kLdsBudget) {
+    while (G > 1 && decode_item_lds_bytes(bg, Z, jobs[i].lbound, G, 0, false) > kLdsBudget) {
       --G;
     }
     dec_item it{};
@@ -555,14 +556,14 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
     it.ngroups = (uint16_t)(k - i);
     it.lmax = (uint16_t)lmax;
     it.scatter = scatter ? 1 : 0;
-    size_t need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, 0);
+    size_t need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, 0, false);
     if (need > kLdsBudget) {
       return fail(LH_ERR_CAPACITY, "LDS budget exceeded (Z=%d, L=%d)", Z, lmax);
     }
     const int stage = (fused && !scatter && te.stage != 0) ? ((maxE + 15) & ~15) : 0;
-    if (stage > 0 && decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage) <= kLdsBudget) {
+    if (stage > 0 && decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage, false) <= kLdsBudget) {
       it.stage = (uint32_t)stage;
-      need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage);
+      need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage, false);
     }
     lds[m] = std::max(lds[m], need);
     items.push_back(it);
@@ -597,6 +598,27 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
     const int per_cu = std::max(1, std::min(occupancy(ctx, nt[m], lds[m], false, m == 1),
                                              occupancy(ctx, nt[m], lds[m], true, m == 1)));
     pb.grid[m] = (unsigned)std::min<size_t>(pb.n_items[m], (size_t)ctx->nof_cus * per_cu);
+  }
+  // The one-row launch keeps its extension-layer messages as int8 (fewer instructions per
+  // layer, more LDS per codeblock) when all its items are resident at once with that LDS:
+  // the latency regime, where no item waits for a workgroup.
+  pb.ext8 = false;
+  if (pb.n_items[0] > 0) {
+    size_t lds8 = 0;
+    for (size_t k = n_pair; k < items.size(); ++k) {
+      const dec_item& it = items[k];
+      lds8 = std::max(lds8, decode_item_lds_bytes(it.bg, (int)jobs[it.first_job].Z, it.lmax, it.ngroups,
+                                                  (int)it.stage, true));
+    }
+    if (lds8 <= kLdsBudget) {
+      const int per_cu8 = std::min(occupancy(ctx, nt[0], lds8, false, false, true),
+                                   occupancy(ctx, nt[0], lds8, true, false, true));
+      if (per_cu8 >= 1 && pb.n_items[0] <= (size_t)ctx->nof_cus * (size_t)per_cu8) {
+        pb.ext8 = true;
+        lds[0] = lds8;
+        pb.grid[0] = (unsigned)pb.n_items[0];
+      }
+    }
   }
   for (int pass = 0; pass < 2; ++pass) {
     for (int m = 1; m >= 0; --m) {
@@ -653,7 +675,8 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
         continue;
       }
       e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, pb.grid[m], pb.nt[m], pb.lds[m], rm, cw, bits,
-                        results, ctx->d_crc_xz, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1, stream);
+                        results, ctx->d_crc_xz, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1,
+                        m == 0 && pb.ext8, stream);
     }
   }
   return e;

@@ -216,6 +216,66 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
   }
 }
 
+// Extension-layer update for check row r (one row per thread) with the messages as int8
+// in LDS: the message of edge e at msg[e * Z] (zero: the reference's first visit).  The
+// arithmetic is layer_update's; only where the messages live differs (no compressed word
+// to take apart and put back together).
+template <int DEG, bool LEARNED>
+__device__ __forceinline__ void layer_update_ext8(int8_t* __restrict__ soft, int goff, const uint32_t* __restrict__ gt,
+                                                  int deg, unsigned r, unsigned Z, int8_t* __restrict__ msg,
+                                                  uint32_t ab)
+{
+  const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
+  const uint32_t mbase = lds_addr(msg);
+  uint32_t addr[DEG];
+  int sv[DEG], cv[DEG];
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (e < deg) {
+      const uint32_t g = gt[e];
+      unsigned j = r + (g >> 16);
+      j = min(j, j - Z);
+      addr[e] = (g & 0xffffu) + j + gbase;
+      sv[e] = *lds_at(addr[e]);
+      cv[e] = *lds_at(mbase + (uint32_t)e * Z);
+    }
+  }
+  uint32_t m1 = (120u * 120u) << 5, m2 = ((120u * 120u) << 5) | 31u;
+  int sp = 0;
+  int v[DEG];
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (e < deg) {
+      const int x = v2c_of(sv[e], cv[e]);
+      v[e] = x;
+      const uint32_t key = (sq24(x) << 5) | (uint32_t)e;
+      m2 = med3u_asm(m1, m2, key);
+      m1 = min(m1, key);
+      sp ^= x;
+    }
+  }
+  const int amin = (int)(m1 & 31u);
+  int s1, s2;
+  if constexpr (LEARNED) {
+    s1 = learned_mag(isqrt_sq(m1 >> 5), ab);
+    s2 = learned_mag(isqrt_sq(m2 >> 5), ab);
+  } else {
+    s1 = scale08(isqrt_sq(m1 >> 5));
+    s2 = scale08(isqrt_sq(m2 >> 5));
+  }
+  asm volatile("" : "+v"(s1), "+v"(s2));
+#pragma unroll
+  for (int e = 0; e < DEG; ++e) {
+    if (e < deg) {
+      const int mag = (e == amin) ? s2 : s1;
+      const int neg = (sp ^ v[e]) >> 31;
+      const int c = (mag ^ neg) - neg;
+      *lds_at(mbase + (uint32_t)e * Z) = (int8_t)c;
+      *lds_at(addr[e]) = (int8_t)med3i(c + v[e], -121, 121); // promotion sum
+    }
+  }
+}
+
 // --- Two check rows per thread (even Z) ---------------------------------------------
 // Thread p < h = Z/2 owns check rows p and p + h.  Row p + h reads column
 // (p + h + shift) mod Z, which is row p's column +-h.  In this row-pair layout the rows
@@ -1016,7 +1076,7 @@ constexpr int kQueueBytes = 16;
 // the threads beyond G*n only take part in barriers.  Both modes use the same LDS
 // amounts (only the order of the soft bits in a column differs).
 // Inlined into each kernel instance: as a call its register state would spill.
-template <int BG, bool LEARNED, bool PAIR>
+template <int BG, bool LEARNED, bool PAIR, bool EXT8>
 __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs,
                                             int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base,
                                             uint8_t* __restrict__ bits_base, int32_t* __restrict__ results,
@@ -1056,15 +1116,19 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
 
   // LDS: per-group soft bits | extension-layer state | codeword stage, then control.
   const int soft_stride = ((K + lmax) * Z + 15) & ~15;
-  const int ext_words = (lmax - 4) * Z; // one word per check row and layer
+  // Extension-layer messages: one compressed word per check row and layer, or with EXT8
+  // (one row per thread) an int8 message per edge and row (layer l, edge e of its row
+  // list: byte (rs(l) - rs(4) + e) Z + r).
+  const int ext_stride = EXT8 ? (((T::rs_word(lmax) - T::rs(4)) * Z + 15) & ~15) : 4 * (lmax - 4) * Z;
   const int stage_stride = (int)item.stage;
   int8_t* const soft0 = reinterpret_cast<int8_t*>(lds); // uniform base of the soft bits
   const int goff = gg * soft_stride;                      // this group's soft bits
   int8_t* soft = soft0 + goff;
   uint8_t* const ldsd = lds;
-  uint32_t* ext = reinterpret_cast<uint32_t*>(ldsd + G * soft_stride) + gg * ext_words;
-  int8_t* stage = reinterpret_cast<int8_t*>(ldsd + G * soft_stride + 4 * G * ext_words) + gg * stage_stride;
-  grp_ctl* ctl = reinterpret_cast<grp_ctl*>(ldsd + G * soft_stride + 4 * G * ext_words + G * stage_stride);
+  uint32_t* ext = reinterpret_cast<uint32_t*>(ldsd + G * soft_stride + gg * ext_stride);
+  int8_t* ext8 = reinterpret_cast<int8_t*>(ext);
+  int8_t* stage = reinterpret_cast<int8_t*>(ldsd + G * soft_stride + G * ext_stride) + gg * stage_stride;
+  grp_ctl* ctl = reinterpret_cast<grp_ctl*>(ldsd + G * soft_stride + G * ext_stride + G * stage_stride);
 
   if (t < G) {
     ctl->last[t] = -1;
@@ -1229,13 +1293,22 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
     }
   }
   if (active && !done) {
-    for (int l = 4; l < L; ++l) {
-      // One state word per check row: thread r owns word r (words 2r, 2r + 1 with PAIR).
-      if (PAIR) {
+    if (PAIR) {
+      // One state word per check row: thread r owns words 2r, 2r + 1 of each layer.
+      for (int l = 4; l < L; ++l) {
         ext[(l - 4) * Z + 2 * (int)r] = 0;
         ext[(l - 4) * Z + 2 * (int)r + 1] = 0;
-      } else {
+      }
+    } else if (!EXT8) {
+      for (int l = 4; l < L; ++l) {
         ext[(l - 4) * Z + (int)r] = 0;
+      }
+    } else {
+      // The int8 messages of layers 4 .. L - 1, zeroed as words (the region is 16-byte
+      // aligned and padded to 16 bytes).
+      const int nw = ((T::rs_word(L) - T::rs(4)) * Z + 3) >> 2;
+      for (int i = (int)r; i < nw; i += NT) {
+        ext[i] = 0;
       }
     }
   }
@@ -1313,7 +1386,8 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
       if (run && l < L) {
         const int e0 = e_cur, dg = e_nxt - e_cur;
         const uint32_t ab = LEARNED ? abt[l] : 0u;
-        uint32_t* st = &ext[(l - 4) * Z + (PAIR ? 2 * (int)r : (int)r)];
+        uint32_t* st = &ext[(l - 4) * Z + (PAIR ? 2 * (int)r : (int)r)]; // compressed words
+        int8_t* msg = ext8 + (e0 - T::rs(4)) * Z + (int)r;                 // EXT8
         // Degree classes (uniform branch): the unrolled update does the work of its
         // class's largest degree, so small rows take a small instance.
         if constexpr (PAIR) {
@@ -1330,6 +1404,22 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
               layer_update_pair<4, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
             } else {
               layer_update_pair<6, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, st, ab);
+            }
+          }
+        } else if constexpr (EXT8) {
+          if (BG == 1) {
+            if (dg <= 5) {
+              layer_update_ext8<5, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+            } else if (dg <= 7) {
+              layer_update_ext8<7, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+            } else {
+              layer_update_ext8<10, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+            }
+          } else {
+            if (dg <= 4) {
+              layer_update_ext8<4, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+            } else {
+              layer_update_ext8<6, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
             }
           }
         } else if (BG == 1) {
@@ -1454,7 +1544,7 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
 // the workgroup through a word at the end of the dynamic LDS (qslot).
 // One persistent decoder per row mode: the row-pair instance holds more state per thread
 // and runs at three waves per SIMD (168 VGPRs), the one-row instance at four.
-template <bool LEARNED, bool PAIR>
+template <bool LEARNED, bool PAIR, bool EXT8>
 __global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(PAIR ? 3 : 4)))
 k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ items, const dec_job* __restrict__ jobs,
               int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
@@ -1469,10 +1559,10 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
   for (uint32_t id = w; id < n_items;) {
     const dec_item item = items[hdr->first_item + id];
     if (item.bg == 1) {
-      decode_item<1, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab,
+      decode_item<1, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab,
                                     fb_iters, lds);
     } else {
-      decode_item<2, LEARNED, PAIR>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
+      decode_item<2, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
                                     LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
     }
     if (nq == 0) {
@@ -1688,40 +1778,61 @@ __global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __res
 }
 
 // ---- host-side launchers ------------------------------------------------------------
-size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage)
+size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage, bool ext8)
 {
   const int K = bg == 1 ? 22 : 10;
-  return kQueueBytes + (size_t)G * ((((K + lmax) * Z + 15) & ~15) + 4 * (lmax - 4) * Z + stage) + kCtlBytes;
+  const int ext_edges = bg == 1 ? bgt<1>::rs(lmax) - bgt<1>::rs(4) : bgt<2>::rs(lmax) - bgt<2>::rs(4);
+  const int ext = ext8 ? ((ext_edges * Z + 15) & ~15) : 4 * (lmax - 4) * Z;
+  return kQueueBytes + (size_t)G * ((((K + lmax) * Z + 15) & ~15) + ext + stage) + kCtlBytes;
+}
+
+// The kernel instance of a launch: plain or learned, row pairs or one row, and in the
+// one-row mode the int8 extension-layer messages (EXT8) or the compressed words.
+const void* decode_kernel(bool learned, bool pairs, bool ext8)
+{
+  if (pairs) {
+    return learned ? (const void*)k_ldpc_decode<true, true, false> : (const void*)k_ldpc_decode<false, true, false>;
+  }
+  if (ext8) {
+    return learned ? (const void*)k_ldpc_decode<true, false, true> : (const void*)k_ldpc_decode<false, false, true>;
+  }
+  return learned ? (const void*)k_ldpc_decode<true, false, false> : (const void*)k_ldpc_decode<false, false, false>;
 }
 
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         bool pairs, hipStream_t s)
+                         bool pairs, bool ext8, hipStream_t s)
 {
   const bool learned = scale_tab != nullptr;
-  const void* fn = pairs ? (learned ? (const void*)k_ldpc_decode<true, true> : (const void*)k_ldpc_decode<false, true>)
-                         : (learned ? (const void*)k_ldpc_decode<true, false> : (const void*)k_ldpc_decode<false, false>);
+  ext8 = ext8 && !pairs;
+  const void* fn = decode_kernel(learned, pairs, ext8);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) {
     return e;
   }
   // lds (from decode_item_lds_bytes, a multiple of 4) ends with the claimed-item word.
   const unsigned qslot = (unsigned)lds - kQueueBytes;
-#define LH_LAUNCH(L, P)                                                                                           \
-  hipLaunchKernelGGL((k_ldpc_decode<L, P>), dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, \
+#define LH_LAUNCH(L, P, X)                                                                                        \
+  hipLaunchKernelGGL((k_ldpc_decode<L, P, X>), dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, \
                      results, crc_x, gtab, scale_tab, fb_iters, qslot)
   if (pairs) {
     if (learned) {
-      LH_LAUNCH(true, true);
+      LH_LAUNCH(true, true, false);
     } else {
-      LH_LAUNCH(false, true);
+      LH_LAUNCH(false, true, false);
+    }
+  } else if (ext8) {
+    if (learned) {
+      LH_LAUNCH(true, false, true);
+    } else {
+      LH_LAUNCH(false, false, true);
     }
   } else {
     if (learned) {
-      LH_LAUNCH(true, false);
+      LH_LAUNCH(true, false, false);
     } else {
-      LH_LAUNCH(false, false);
+      LH_LAUNCH(false, false, false);
     }
   }
 #undef LH_LAUNCH
@@ -1743,17 +1854,15 @@ hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s)
   return hipGetLastError();
 }
 
-int decode_occupancy(int nt, size_t lds, bool learned, bool pairs)
+int decode_occupancy(int nt, size_t lds, bool learned, bool pairs, bool ext8)
 {
   int n = 0;
-  const void* fn = pairs ? (learned ? (const void*)k_ldpc_decode<true, true> : (const void*)k_ldpc_decode<false, true>)
-                         : (learned ? (const void*)k_ldpc_decode<true, false> : (const void*)k_ldpc_decode<false, false>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, lds) != hipSuccess) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, decode_kernel(learned, pairs, ext8 && !pairs), nt, lds) !=
+      hipSuccess) {
     return 1;
   }
   return n;
 }
-
 
 hipError_t launch_rate_dematch(const rm_params* d_jobs, unsigned n, unsigned maxN, const int8_t* cw, int8_t* rm,
                                hipStream_t s)
