@@ -1348,10 +1348,6 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
   const int Litem = ctl->lmax_item;
 
   for (int it = 0;; ++it) {
-    // Uniform exit: every group finished (read after the barrier that ordered the updates).
-    if (ctl->ndone >= G) {
-      break;
-    }
     const bool run = !done;
     const int par = it & 1;
     // The edge table is re-read every iteration: values derived from it stay in the
@@ -1366,6 +1362,12 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
       core_layer<BG, 0, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c0, abt);
     }
     __syncthreads();
+    // Uniform exit: every group finished.  The count is read after this barrier, which
+    // orders the previous check's updates; a finished launch reaches it with layer 0
+    // skipped by every group.  The check therefore needs no closing barrier of its own.
+    if (ctl->ndone >= G) {
+      break;
+    }
     if (run) {
       core_layer<BG, 1, LEARNED, PAIR>(soft0, goff, gti, r, (unsigned)Z, c1, abt);
     }
@@ -1517,12 +1519,12 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
         }
       }
     }
-    // Clear the other parity's accumulators for the next iteration.
+    // Clear the other parity's accumulators for the next iteration (its check follows four
+    // layer barriers later; this iteration's are read before the next layer-0 barrier).
     if (t < G) {
       ctl->crc[par ^ 1][t] = 0;
       ctl->zf[par ^ 1][t] = 0;
     }
-    __syncthreads();
   }
 
   if (active && !skip) {
