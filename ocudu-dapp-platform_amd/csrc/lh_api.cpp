@@ -779,8 +779,9 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
       put(2, k, lh_bg2_col[k], lh_bg2_v[ils][k]);
     }
   }
-  // CRC24A byte table and x^(2^k) mod G for the transport-block CRC.
-  std::vector<uint32_t> tbc(256 + 32);
+  // CRC24A byte table, x^(2^k) mod G, and x^(8 d 16^i) mod G (digit d < 16 of a byte count,
+  // i < kTbShiftDigits) for the transport-block CRC.
+  std::vector<uint32_t> tbc(256 + 32 + 16 * kTbShiftDigits);
   for (uint32_t b = 0; b < 256; ++b) {
     uint32_t v = b << 16;
     for (int k = 0; k < 8; ++k) {
@@ -789,21 +790,32 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
     tbc[b] = v & 0xffffffu;
   }
   {
-    uint32_t v = 2; // x^1
-    for (int k = 0; k < 32; ++k) {
-      tbc[256 + k] = v;
-      // square: v = v * v mod G
+    auto mulmod = [](uint32_t a, uint32_t b) {
       uint32_t res = 0;
       for (int i = 23; i >= 0; --i) {
         res <<= 1;
         if (res & 0x1000000u) {
           res ^= LH_CRC24A;
         }
-        if ((v >> i) & 1u) {
-          res ^= v;
+        if ((b >> i) & 1u) {
+          res ^= a;
         }
       }
-      v = res;
+      return res;
+    };
+    uint32_t v = 2; // x^1
+    for (int k = 0; k < 32; ++k) {
+      tbc[256 + k] = v;
+      v = mulmod(v, v);
+    }
+    uint32_t base = tbc[256 + 3]; // x^8: one byte
+    for (int i = 0; i < kTbShiftDigits; ++i) {
+      uint32_t* w = &tbc[256 + 32 + 16 * i];
+      w[0] = 1;
+      for (int d = 1; d < 16; ++d) {
+        w[d] = mulmod(w[d - 1], base);
+      }
+      base = mulmod(w[15], base); // x^(8 16^(i+1))
     }
   }
   if (e == hipSuccess) {
@@ -1241,7 +1253,7 @@ int validate_tbs(const lh_tb_desc* tbs, unsigned n)
 {
   for (unsigned i = 0; i < n; ++i) {
     const lh_tb_desc& d = tbs[i];
-    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || d.nof_cbs == 0) {
+    if (d.tbs_bits == 0 || d.tbs_bits % 8 != 0 || d.nof_cbs == 0 || d.tbs_bits / 8 >= (1u << (4 * kTbShiftDigits))) {
       return fail(LH_ERR_INVALID, "tb %u: invalid size", i);
     }
     if (d.nof_cbs > 1 && (d.cb_data_bits == 0 || (uint64_t)d.cb_data_bits * d.nof_cbs < (uint64_t)d.tbs_bits + 24 ||

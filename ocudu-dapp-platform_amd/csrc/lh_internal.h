@@ -16,6 +16,9 @@ constexpr int kCrcXLen = 22 * 384 + 24;
 // bit (at least order - F > -kCrcPad) reads zero.
 constexpr int kCrcPad = 20 * 384;
 constexpr int kCrcXZLen = kCrcPad + kCrcXLen;
+// The TB CRC moves a chunk's remainder past the m bytes that follow it with one table
+// factor per base-16 digit of m: m < 16^kTbShiftDigits (16 MiB, above any TB).
+constexpr int kTbShiftDigits = 6;
 constexpr uint32_t kMaxRmLength = 22 * 384 * 35; // MAX_CODEBLOCK_RM_SIZE
 constexpr int kNofLiftingSizes = 51;
 // Lifted edge tables of all graphs: BG1 graphs first (316 edges each), then BG2 (197).

@@ -481,7 +481,7 @@ __device__ __forceinline__ void core_layer(int8_t* soft, int goff, const uint32_
   }
 }
 
-// Wave64 + workgroup reductions.  `red` holds 16 words.
+// Wave64 + workgroup reductions (workgroups of up to 16 waves).  `red` holds 32 words.
 __device__ __forceinline__ void block_reduce_xor_or(uint32_t& x, uint32_t& z, uint32_t* red)
 {
 #pragma unroll
@@ -492,14 +492,14 @@ __device__ __forceinline__ void block_reduce_xor_or(uint32_t& x, uint32_t& z, ui
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[w] = x;
-    red[8 + w] = z;
+    red[16 + w] = z;
   }
   __syncthreads();
   x = 0;
   z = 0;
   for (int i = 0; i < nw; ++i) {
     x ^= red[i];
-    z |= red[8 + i];
+    z |= red[16 + i];
   }
 }
 
@@ -1599,7 +1599,7 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
                                              const uint8_t* __restrict__ bits, const uint32_t* __restrict__ xtab,
                                              int order, uint32_t* __restrict__ out)
 {
-  __shared__ uint32_t red[16];
+  __shared__ uint32_t red[32];
   const uint8_t* b = bits + offs[blockIdx.x];
   const int len = (int)nbits[blockIdx.x];
   uint32_t x = 0, z = 0;
@@ -1615,6 +1615,32 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
 }
 
 // GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
+// Byte of the concatenated TB data whose first bit is bit j of codeblock c (MSB-first
+// bytes, codeblocks `stride` bytes apart, D data bits each); the bits past D come from the
+// next codeblock.  Positions past the TB's last codeblock read its CRC bits, which the
+// caller discards.
+__device__ __forceinline__ uint32_t tb_byte(const uint8_t* __restrict__ first, uint32_t stride, uint32_t D,
+                                            uint32_t c, uint32_t j)
+{
+  const uint8_t* src = first + (size_t)c * stride;
+  if (j + 8u <= D) {
+    // The eight bits lie in one codeblock: bits j..j+7 of its MSB-first bytes (the byte
+    // after is inside the codeblock's message, whose CRC follows the data).
+    const uint32_t q = j >> 3, sh = j & 7u;
+    return ((((uint32_t)src[q] << 8) | src[q + 1]) >> (8u - sh)) & 0xffu;
+  }
+  uint32_t v = 0;
+  for (int k = 0; k < 8; ++k) {
+    const uint8_t* sk = first + (size_t)c * stride;
+    v |= (uint32_t)((sk[j >> 3] >> (7 - (j & 7))) & 1u) << (7 - k);
+    if (++j == D) {
+      j = 0;
+      ++c;
+    }
+  }
+  return v;
+}
+
 // TB reassembly + TB CRC24A of transport block d, one workgroup (pusch_decoder_impl.cpp:612-781).
 // WIDE: the TB bytes go out as aligned 4-byte words (the TB offset is a multiple of 4),
 // which matters when they are written across the bus into host memory.
@@ -1626,7 +1652,9 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
                                                 int& all_ok)
 {
   const int t = (int)threadIdx.x, nt = (int)blockDim.x;
-  lut[t] = lut24a[t];
+  for (int i = t; i < 256 + 16 * kTbShiftDigits; i += nt) {
+    lut[i] = i < 256 ? lut24a[i] : pow2_24a[32 + i - 256];
+  }
   if (t == 0) {
     all_ok = 1;
   }
@@ -1672,92 +1700,118 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
     }
     return;
   }
-  // Concatenation: TB bit i comes from codeblock i / D, bit i mod D.  Each thread builds a
-  // contiguous chunk of TB bytes and folds them into its CRC24A remainder in the same pass
-  // (table driven); the remainders are moved to their place by x^(8 * bytes after the
-  // chunk) and XOR-reduced.
+  // Concatenation: TB bit i comes from codeblock i / D, bit i mod D.  Pass 1 assembles the
+  // TB with consecutive threads on consecutive words, so each wave store is one contiguous
+  // run (which matters when the output lies across the bus in host memory).  Pass 2 folds
+  // each thread's contiguous chunk of TB bytes into its CRC24A remainder (table driven;
+  // its codeblock reads hit the cache lines pass 1 brought in); the remainders are moved
+  // to their place by x^(8 * bytes after the chunk) and XOR-reduced.
   const uint32_t D = d.cb_data_bits;
   const uint32_t poly = 0x1864cfbu;
-  // Chunks of whole words when WIDE: every chunk but the last ends on a word boundary.
-  const uint32_t chunk = WIDE ? ((nbytes + 4u * (uint32_t)nt - 1) / (4u * (uint32_t)nt)) * 4u
-                              : (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
-  const uint32_t a0 = min(nbytes, (uint32_t)t * chunk), a1 = min(nbytes, a0 + chunk);
-  uint32_t rem = 0;
-  uint32_t word = 0;
-  // Codeblock c and bit j of TB bit 8 * b (one division per thread, then stepped).
-  uint32_t c = (8u * a0) / D, j = 8u * a0 - c * D;
-  for (uint32_t b = a0; b < a1; ++b) {
-    const uint8_t* src = first + (size_t)c * d.cb_bits_stride;
-    uint32_t v;
-    if (j + 8u <= D) {
-      // The eight bits lie in one codeblock: bits j..j+7 of its MSB-first bytes (the
-      // byte after is inside the codeblock's message, whose CRC follows the data).
-      const uint32_t q = j >> 3, sh = j & 7u;
-      v = ((((uint32_t)src[q] << 8) | src[q + 1]) >> (8u - sh)) & 0xffu;
+  const uint32_t stride = d.cb_bits_stride;
+  {
+    // Word w covers TB bits 32w..32w+31; a step of nt words moves (c, j) by (dc, dj).
+    const uint32_t nwords = WIDE ? (nbytes + 3u) / 4u : 0u;
+    const uint32_t step = 32u * (uint32_t)nt, dc = step / D, dj = step - dc * D;
+    uint32_t c = (32u * (uint32_t)t) / D, j = 32u * (uint32_t)t - c * D;
+    if (WIDE) {
+      uint32_t* out4 = reinterpret_cast<uint32_t*>(out);
+      for (uint32_t w = (uint32_t)t; w < nwords; w += (uint32_t)nt) {
+        uint32_t cc = c, jj = j, word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          word |= tb_byte(first, stride, D, cc, jj) << (8 * k);
+          jj += 8u;
+          if (jj >= D) {
+            jj -= D;
+            ++cc;
+          }
+        }
+        if (4u * w + 4u <= nbytes) {
+          out4[w] = word;
+        } else {
+          for (uint32_t b = 4u * w; b < nbytes; ++b) {
+            out[b] = (uint8_t)(word >> (8 * (b & 3u)));
+          }
+        }
+        c += dc;
+        j += dj;
+        if (j >= D) {
+          j -= D;
+          ++c;
+        }
+      }
     } else {
-      v = 0;
-      uint32_t cc = c, jj = j;
-      for (int k = 0; k < 8; ++k) {
-        const uint8_t* sk = first + (size_t)cc * d.cb_bits_stride;
-        v |= (uint32_t)((sk[jj >> 3] >> (7 - (jj & 7))) & 1u) << (7 - k);
-        if (++jj == D) {
-          jj = 0;
-          ++cc;
+      const uint32_t bstep = 8u * (uint32_t)nt, bdc = bstep / D, bdj = bstep - bdc * D;
+      c = (8u * (uint32_t)t) / D;
+      j = 8u * (uint32_t)t - c * D;
+      for (uint32_t b = (uint32_t)t; b < nbytes; b += (uint32_t)nt) {
+        out[b] = (uint8_t)tb_byte(first, stride, D, c, j);
+        c += bdc;
+        j += bdj;
+        if (j >= D) {
+          j -= D;
+          ++c;
         }
       }
     }
-    if (WIDE) {
-      word |= v << (8 * (b & 3u));
-      if ((b & 3u) == 3u) {
-        *reinterpret_cast<uint32_t*>(out + b - 3) = word;
-        word = 0;
-      }
-    } else {
-      out[b] = (uint8_t)v;
-    }
-    rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ v) & 0xffu];
-    j += 8u;
-    if (j >= D) {
-      j -= D;
-      ++c;
-    }
   }
-  if (WIDE && (a1 & 3u) != 0 && a0 < a1) {
-    // The TB's last, partial word.
-    for (uint32_t b = a1 & ~3u; b < a1; ++b) {
-      out[b] = (uint8_t)(word >> (8 * (b & 3u)));
+  const uint32_t chunk = (nbytes + (uint32_t)nt - 1) / (uint32_t)nt;
+  const uint32_t a0 = min(nbytes, (uint32_t)t * chunk), a1 = min(nbytes, a0 + chunk);
+  uint32_t rem = 0;
+  // Codeblock c and bit j of TB bit 8 * b (one division per thread, then stepped); four
+  // bytes' loads are issued before their four table steps.
+  uint32_t c = (8u * a0) / D, j = 8u * a0 - c * D;
+  for (uint32_t b = a0; b < a1; b += 4u) {
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = b + (uint32_t)k < a1 ? tb_byte(first, stride, D, c, j) : 0u;
+      j += 8u;
+      if (j >= D) {
+        j -= D;
+        ++c;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (b + (uint32_t)k < a1) {
+        rem = ((rem << 8) & 0xffffffu) ^ lut[((rem >> 16) ^ v[k]) & 0xffu];
+      }
     }
   }
   uint32_t x = 0, z = 0;
   if (a0 < a1) {
-    x = gf_mulmod(rem, gf_xpow(8u * (nbytes - a1), pow2_24a, poly, 24), poly, 24);
+    x = gf_shift_bytes(rem, nbytes - a1, lut + 256, poly);
   }
   block_reduce_xor_or(x, z, red);
   if (t == 0) {
-    // The 24-bit TB checksum follows the last codeblock's data bits (:764-769).
+    // The 24-bit TB checksum follows the last codeblock's data bits (:764-769): bits
+    // start..start+23 of its MSB-first bytes, read as one 32-bit window (four independent
+    // loads; the window's last bits lie inside the codeblock's message, whose CRC follows).
     const uint32_t last_c = d.nof_cbs - 1;
     const uint32_t start = d.tbs_bits - last_c * D;
-    const uint8_t* src = first + (size_t)last_c * d.cb_bits_stride;
-    uint32_t ck = 0;
-    for (uint32_t k = 0; k < 24; ++k) {
-      const uint32_t j = start + k;
-      ck = (ck << 1) | ((src[j >> 3] >> (7 - (j & 7))) & 1u);
-    }
+    const uint8_t* src = first + (size_t)last_c * d.cb_bits_stride + (start >> 3);
+    const uint32_t win = ((uint32_t)src[0] << 24) | ((uint32_t)src[1] << 16) | ((uint32_t)src[2] << 8) | src[3];
+    const uint32_t ck = (win >> (8u - (start & 7u))) & 0xffffffu;
     status[blockIdx.x] = (x == ck) ? 1 : -1;
   }
 }
 
+// Threads of a TB workgroup: a large TB's concatenation and CRC chunks spread over 16 waves.
+constexpr int kTbThreads = 1024;
+
 // One workgroup per transport block.  SIGNAL: the outputs are host-visible and the launch
 // reports its completion through `sig` (no copy back, no event on the latency path).
 template <bool SIGNAL>
-__global__ void __launch_bounds__(256) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
+__global__ void __launch_bounds__(kTbThreads) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
                                                      const uint8_t* __restrict__ cb_bits,
                                                      const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
                                                      int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
                                                      const uint32_t* __restrict__ pow2_24a, tb_signal sig)
 {
-  __shared__ uint32_t red[16];
-  __shared__ uint32_t lut[256];
+  __shared__ uint32_t red[32];
+  __shared__ uint32_t lut[256 + 16 * kTbShiftDigits];
   __shared__ int all_ok;
   const lh_tb_desc_dev d = tbs[blockIdx.x];
   tb_finalize_one<SIGNAL>(d, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
@@ -1879,10 +1933,10 @@ hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uin
                               const tb_signal* sig)
 {
   if (sig != nullptr) {
-    hipLaunchKernelGGL(k_tb_finalize<true>, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
+    hipLaunchKernelGGL(k_tb_finalize<true>, dim3(n), dim3(kTbThreads), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
                        *sig);
   } else {
-    hipLaunchKernelGGL(k_tb_finalize<false>, dim3(n), dim3(256), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
+    hipLaunchKernelGGL(k_tb_finalize<false>, dim3(n), dim3(kTbThreads), 0, s, d_tbs, cb_bits, cb_res, tb, status, lut, pow2,
                        tb_signal{});
   }
   return hipGetLastError();

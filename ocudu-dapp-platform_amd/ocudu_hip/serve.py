@@ -347,7 +347,12 @@ def _main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--prb-max", type=int, default=273, help="PRBs per UE uniform in [4, prb-max]")
     ap.add_argument("--in-flight", type=int, default=3)
+    ap.add_argument("--carrier-prbs", type=int, default=0,
+                    help="> 0: the slot's UEs share one carrier of that many PRBs (instead of --prb-max each)")
+    ap.add_argument("--layers", default="1,4", help="layers per UE, uniform in this range")
+    ap.add_argument("--native", action="store_true", help="the slot loop in the library (lh_server_serve)")
     args = ap.parse_args(argv)
+    layers = tuple(int(x) for x in args.layers.split(","))
     ctx = Context(0, max_codeblocks=8192)
 
     def log(m):
@@ -355,7 +360,8 @@ def _main(argv=None):
 
     thr = mcs_thresholds(ctx, log=log)
     t0 = time.time()
-    trace = Trace(ctx, args.slots, args.ues, thr, seed=args.seed, prb_max=args.prb_max)
+    trace = Trace(ctx, args.slots, args.ues, thr, seed=args.seed, prb_max=args.prb_max, carrier_prbs=args.carrier_prbs,
+                  layers=layers)
     log(f"trace generated in {time.time() - t0:.1f} s")
     log(f"{len(trace.tbs)} TBs, {sum(t['seg'].nof_cbs for t in trace.tbs)} codeblocks")
     if args.learned:
@@ -365,7 +371,7 @@ def _main(argv=None):
         for bg in (1, 2):
             _, aq, bq, _ = load_table(tables / f"nms_bg{bg}.json")
             ctx.set_scale_table(bg, aq, bq)
-    res = serve(ctx, trace, learned=args.learned, slots_in_flight=args.in_flight, log=log)
+    res = serve(ctx, trace, learned=args.learned, slots_in_flight=args.in_flight, log=log, native=args.native)
     print(json.dumps(summarize(trace, *res)), flush=True)
     return 0
 
