@@ -5,8 +5,10 @@ Workload (one step): a homogeneous batch of 16 slots x 152 codeblocks, BG1 Z=384
 256-QAM, E = 9088 (code rate ~0.93), new data rv0, int8 LLRs resident in HBM (made by the
 device generator: encoder, rate matcher, mapper, AWGN, max-log demapper), max 8 LDPC
 iterations with CRC24B early stop: rate dematching into the HARQ buffers followed by
-layered min-sum decoding (lh_pusch_decode_batch).  AWGN Es/N0 is swept over 20..26 dB in
-1 dB steps; the headline value is the plain decoder's info Gbit/s (correct codeblocks x
+layered min-sum decoding (lh_pusch_decode_batch).  Consecutive steps alternate over two
+streams, each with its own HARQ buffers and outputs, as consecutive slot batches of a
+receiver would; the one-stream step (each launch alone on the GPU) is reported beside it
+and is the roofline's launch.  AWGN Es/N0 is swept over 20..26 dB in 1 dB steps; the headline value is the plain decoder's info Gbit/s (correct codeblocks x
 8424 info bits / step time) at the lowest swept Es/N0 whose BLER is <= 10%.  The learned
 decoder (tables/nms_bg1.json, lh_pusch_decode_batch_learned) is measured on the same
 inputs there, and both again at the operating point: the lowest Es/N0 on a 0.05 dB grid
@@ -51,6 +53,8 @@ def parse():
     p.add_argument("--serve-slots", type=int, default=2000, help="slots of the serving-loop runs (0 = skip)")
     p.add_argument("--uneven-ues", type=int, default=64, help="UEs of the uneven batch (0 = skip)")
     p.add_argument("--max-iter", type=int, default=8)
+    p.add_argument("--streams", type=int, default=2,
+                   help="consecutive steps alternate over this many streams (own outputs each)")
     p.add_argument("--latency-slots", type=int, default=2000)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU work of the baseline sample")
@@ -114,26 +118,53 @@ def main():
         ctx.set_scale_table(tbg, aq, bq)
         learned_ok = tbg == bg
 
-    def run(steps, warmup, learned=False):
-        """Event-timed steps on resident inputs: (ms/step, BLER, info Gbit/s, mean iterations)."""
-        def call():
+    # Consecutive steps alternate over `a.streams` streams, each with its own HARQ buffers,
+    # bits and results (as consecutive slot batches of a receiver): a step's launch starts
+    # on the compute units that the previous step's last codeblocks leave idle.
+    streams = [stream] + [H.Stream() for _ in range(a.streams - 1)]
+    outs = [(d_rm, d_bits, d_res)]
+    for _ in range(a.streams - 1):
+        o = (H.DeviceBuffer(n_cb * N), H.DeviceBuffer(n_cb * msg_b), H.DeviceBuffer(4 * n_cb))
+        o[1].zero()
+        outs.append(o)
+    ev_end = [H.Event() for _ in streams]
+
+    def run(steps, warmup, learned=False, ns=None):
+        """Event-timed steps on resident inputs, step k on stream k mod ns (default: all):
+        (ms/step, BLER, info Gbit/s, mean iterations).  The time runs from an event before
+        the first launch to one after every stream's last launch."""
+        ns = ns or a.streams
+
+        def call(k):
+            rm, bits, res = outs[k % ns]
+            s = streams[k % ns]
             if learned:
-                ctx.pusch_decode_batch_learned(None, d_cw, d_rm, d_bits, d_res, nof_ldpc_iterations=a.max_iter,
-                                               nof_fallback_iterations=0, stream=stream, arr=cbs)
+                ctx.pusch_decode_batch_learned(None, d_cw, rm, bits, res, nof_ldpc_iterations=a.max_iter,
+                                               nof_fallback_iterations=0, stream=s, arr=cbs)
             else:
-                ctx.pusch_decode_batch(None, d_cw, d_rm, d_bits, d_res, nof_ldpc_iterations=a.max_iter,
-                                       stream=stream, arr=cbs)
-        for _ in range(warmup):
-            call()
-        stream.sync()
-        ev0.record(stream)
-        for _ in range(steps):
-            call()
-        ev1.record(stream)
+                ctx.pusch_decode_batch(None, d_cw, rm, bits, res, nof_ldpc_iterations=a.max_iter, stream=s, arr=cbs)
+        for k in range(warmup):
+            call(k)
+        H.hip().hipDeviceSynchronize()
+        ev0.record(streams[0])
+        for s in streams[1:ns]:
+            s.wait(ev0)
+        for k in range(steps):
+            call(k)
+        for i in range(1, ns):
+            ev_end[i].record(streams[i])
+            streams[0].wait(ev_end[i])
+        ev1.record(streams[0])
         ev1.sync()
+        H.hip().hipDeviceSynchronize()
         ms = ev0.elapsed_ms(ev1) / max(steps, 1)
         res = d_res.download(np.int32, n_cb)
         bits = d_bits.download(np.uint8, n_cb * msg_b).reshape(n_cb, msg_b)
+        for i in range(1, min(ns, steps + warmup)):
+            # Every stream decoded the same inputs: its outputs must be the same.
+            if not (np.array_equal(outs[i][2].download(np.int32, n_cb), res) and
+                    np.array_equal(outs[i][1].download(np.uint8, n_cb * msg_b).reshape(n_cb, msg_b), bits)):
+                raise RuntimeError(f"stream {i}: results differ from stream 0")
         msg = d_msg.download(np.uint8, n_cb * msg_b).reshape(n_cb, msg_b)
         good = (res > 0) & (bits == msg).all(axis=1)
         ok = int(good.sum())
@@ -184,14 +215,19 @@ def main():
         print(f"# operating point {snr} dB: {json.dumps(op)}", file=sys.stderr, flush=True)
         generate(snr_h)
 
+    # One stream at the headline Es/N0: each step's launch alone on the GPU.
+    single = entry(snr_h, run(a.steps, a.warmup, ns=1))
+    print(f"# one stream at {snr_h} dB: {json.dumps(single)}", file=sys.stderr, flush=True)
+
     # Roofline of the dominant kernel (the fused dematch + decode launch of one step):
     # algorithmic bytes per codeblock = E LLRs read + N HARQ soft bits written + packed
-    # message bits written + 4-byte result; duration = the HIP-event time of the step.
-    dec_ms = head["ms_per_step"]
+    # message bits written + 4-byte result; duration = the HIP-event time of one launch
+    # alone (the one-stream step).
+    dec_ms = single["ms_per_step"]
     dec_bytes = n_cb * (E + N + msg_b + 4)
     achieved = dec_bytes / (dec_ms * 1e-3) / 1e9
     traffic = measured_traffic()
-    run(1, 0)  # the HARQ buffers hold the headline inputs for the CPU baseline
+    run(1, 0, ns=1)  # the HARQ buffers hold the headline inputs for the CPU baseline
     # The binding resource: VALU.  Edge updates per launch = iterations executed x Z x the
     # edges of the layers decoded (four layers, 76 edges, at this rate), at 10 lane
     # operations each, against the int32 VALU peak (one wave64 instruction per compute
@@ -275,7 +311,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"homogeneous slot batch: {a.slots} slots x {a.cbs_per_slot} CBs, BG1 Z=384, "
                                f"256QAM, E={E} (R~0.93), rv0 new data, max {a.max_iter} iters, CRC24B early stop",
-                   "global_batch": n_cb, "esn0_db": snr_h, "decoder": "plain (bit-exact generic min-sum)"},
+                   "global_batch": n_cb, "esn0_db": snr_h, "decoder": "plain (bit-exact generic min-sum)",
+                   "streams": a.streams},
+        "one_stream": single,
         "bler": head["bler"],
         "mean_iters": head["mean_iters"],
         "info_gbps_at_10pct_bler": op["plain"]["info_gbps"] if op else None,

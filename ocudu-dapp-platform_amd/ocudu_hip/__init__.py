@@ -43,6 +43,7 @@ def hip():
         h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
         h.hipStreamDestroy.argtypes = [C.c_void_p]
         h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        h.hipStreamWaitEvent.argtypes = [C.c_void_p, C.c_void_p, C.c_uint]
         h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
         h.hipEventDestroy.argtypes = [C.c_void_p]
         h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
@@ -124,6 +125,10 @@ class Stream:
 
     def sync(self):
         _chk(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def wait(self, event):
+        """Work queued on this stream after the call waits for `event`."""
+        _chk(hip().hipStreamWaitEvent(self.handle, event.handle, 0), "hipStreamWaitEvent")
 
 
 class Event:
