@@ -114,6 +114,21 @@ __device__ __forceinline__ uint32_t lds_addr(const int8_t* p)
 {
   return (uint32_t)(uintptr_t)(const lds_i8*)p;
 }
+// A global-memory word pointer (address space 1): loads through it are global loads,
+// which a wait for LDS results does not cover (a generic pointer's flat load counts on
+// both the LDS and the memory counter, so every LDS wait would also wait for it).
+typedef __attribute__((address_space(1))) const uint32_t gbl_u32;
+__device__ __forceinline__ gbl_u32* gbl_at(const uint32_t* p)
+{
+  return reinterpret_cast<gbl_u32*>((uintptr_t)p);
+}
+// Eight loaded values kept in registers up to this point: the loads before it are all
+// issued before the first wait for their results.
+template <typename T>
+__device__ __forceinline__ void hold8(T (&w)[8])
+{
+  asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]), "+v"(w[4]), "+v"(w[5]), "+v"(w[6]), "+v"(w[7]));
+}
 
 // One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
 // two-min over composite keys; pass 2: c2v, promoted soft-bit update).  DEG is the
@@ -1460,17 +1475,35 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
         const int rr = (int)r + half * NT;
         // Opaque base: the weight addresses are formed at each check instead of being
         // hoisted out of the iteration loop into registers.
-        const uint32_t* xt = xbase + (xi0 - half * NT);
-        asm volatile("" : "+v"(xt));
+        const uint32_t* xt0 = xbase + (xi0 - half * NT);
+        asm volatile("" : "+v"(xt0));
+        gbl_u32* xt = gbl_at(xt0);
         const int8_t* sc = soft0 + goff + (PAIR ? 2 * (int)r + half : rr);
         uint32_t acc = 0, zz = 0;
         if (want_crc) {
+          // The weights in batches of eight loads, all in flight before the batch's one
+          // wait (scheduled one by one, each load would wait for the one before).
 #pragma unroll
-          for (int n = 0; n < K; ++n) {
-            const int sv = sc[n * Z];
-            const uint32_t w = xt[-n * Z];
-            acc ^= sv <= 0 ? w : 0u; // weight of a hard one
-            zz |= (uint32_t)(sv == 0);
+          for (int n0 = 0; n0 < K; n0 += 8) {
+            uint32_t wv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              wv[k] = n0 + k < K ? xt[-(n0 + k) * Z] : 0u;
+            }
+            int sv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              sv[k] = n0 + k < K ? sc[(n0 + k) * Z] : 1;
+            }
+            hold8(wv);
+            hold8(sv);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (n0 + k < K) {
+                acc ^= sv[k] <= 0 ? wv[k] : 0u; // weight of a hard one
+                zz |= (uint32_t)(sv[k] == 0);
+              }
+            }
           }
         } else {
 #pragma unroll
