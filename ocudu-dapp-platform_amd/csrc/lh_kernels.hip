@@ -383,14 +383,30 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
   const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
   uint32_t a1[DEG];
   i16x2 sv[DEG];
+  // Four edges' eight reads in flight before one wait.
 #pragma unroll
-  for (int e = 0; e < DEG; ++e) {
-    if (!RT || e < deg) {
-      a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + gbase;
-      i16x2 x;
-      x.x = *lds_at(a1[e]);
-      x.y = *lds_at(a1[e] ^ 1u);
-      sv[e] = x;
+  for (int e0 = 0; e0 < DEG; e0 += 4) {
+    int raw[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k;
+      raw[2 * k] = raw[2 * k + 1] = 0;
+      if (e < DEG && (!RT || e < deg)) {
+        a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + gbase;
+        raw[2 * k] = *lds_at(a1[e]);
+        raw[2 * k + 1] = *lds_at(a1[e] ^ 1u);
+      }
+    }
+    hold8(raw);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k;
+      if (e < DEG && (!RT || e < deg)) {
+        i16x2 x;
+        x.x = (short)raw[2 * k];
+        x.y = (short)raw[2 * k + 1];
+        sv[e] = x;
+      }
     }
   }
   // Previous messages of the compressed form: magnitudes, argmin edges, sign bits.
