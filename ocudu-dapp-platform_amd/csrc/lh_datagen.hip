@@ -208,9 +208,6 @@ __device__ void encode_packed(const gen_job& jb, const uint8_t* __restrict__ msg
       }
       pm[w] = x;
     }
-    for (int j = kz + t; j < (K + M) * Z; j += NT) {
-      cw[j] = 0;
-    }
   }
   __syncthreads();
   // 2. Info columns from the message string; zero words behind every column.
@@ -305,8 +302,17 @@ __device__ void encode_packed(const gen_job& jb, const uint8_t* __restrict__ msg
   }
   // 5. Core columns to the codeword bytes; extension rows i >= 4 (identity part on column
   //    K + i with shift s_own): column_{K+i}[j] = XOR over the other edges of
-  //    column_c[(j - s_own + s) mod Z].
-  for (int task = t; task < (M - 4 + 4) * W; task += NT) {
+  //    column_c[(j - s_own + s) mod Z].  Without the codeword output and when the rate
+  //    matcher's ranks [start, start + E) do not wrap, only the rows up to the column of
+  //    the last position it reads (past the fillers) are needed: the other columns are
+  //    neither computed nor written.
+  int rows = M;
+  if (!jb.want_cw && jb.rm.start + jb.rm.E <= jb.rm.P) {
+    const uint32_t last = jb.rm.start + jb.rm.E - 1u;
+    const uint32_t pos = last < jb.rm.info ? last : last + (jb.rm.nsys - jb.rm.info);
+    rows = min(M, max(4, (int)(pos / (uint32_t)Z) + 2 - K + 1));
+  }
+  for (int task = t; task < rows * W; task += NT) {
     const int row = task / W, w = task - row * W;
     const int n = min(32, Z - 32 * w);
     if (row < 4) {
