@@ -42,7 +42,8 @@ hipError_t launch_demod(const demod_tile* tiles, unsigned nof_tiles, const demod
 hipError_t launch_demux(const demux_tile* tiles, unsigned nof_tiles, const demux_sym* syms, const uint32_t* tab,
                         const int8_t* llrs, int8_t* sch, int8_t* ack, int8_t* c1, int8_t* c2, hipStream_t stream);
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
-                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, bool any_small, hipStream_t s);
+                                    uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, bool any_small,
+                                    size_t packed_cw_bytes, hipStream_t s);
 size_t nms_train_lds_bytes(int bg, int Z, int L);
 hipError_t launch_nms_train(int bg, const train_job* d_jobs, unsigned n, int nt, size_t lds, int T,
                             const float* alpha, const float* beta, const float* wt, float tau, const int8_t* in,
@@ -1580,12 +1581,21 @@ int lh_ldpc_encode_rate_match(lh_ctx* ctx, const lh_encode_desc* descs, unsigned
   if (rc != LH_OK) {
     return rc;
   }
+  // Lifting sizes below 32 take the bit-per-byte encoder; the others the word-parallel one,
+  // whose LDS holds the codeword columns its codeblocks need.
   bool any_small = false;
+  size_t cw_bytes = 0;
   for (unsigned i = 0; i < n; ++i) {
-    any_small = any_small || descs[i].lifting_size < 32;
+    const gen_job& j = jobs[i];
+    if (j.Z < 32) {
+      any_small = true;
+      continue;
+    }
+    const int K = j.bg == 1 ? 22 : 10, M = j.bg == 1 ? 46 : 42;
+    cw_bytes = std::max(cw_bytes, (size_t)(K + encode_parity_rows(j.rm, K, M, j.Z, j.want_cw != 0)) * j.Z);
   }
   hipError_t e = launch_encode_rate_match(reinterpret_cast<const gen_job*>(s->dev), n, const_cast<uint8_t*>(d_msgs),
-                                          d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, any_small, st);
+                                          d_out, d_codewords, ctx->d_crc_x, ctx->d_gtab, any_small, cw_bytes, st);
   retire(s, st);
   return e == hipSuccess ? LH_OK : hip_fail(e, "encoder launch");
 }

@@ -306,12 +306,7 @@ __device__ void encode_packed(const gen_job& jb, const uint8_t* __restrict__ msg
   //    matcher's ranks [start, start + E) do not wrap, only the rows up to the column of
   //    the last position it reads (past the fillers) are needed: the other columns are
   //    neither computed nor written.
-  int rows = M;
-  if (!jb.want_cw && jb.rm.start + jb.rm.E <= jb.rm.P) {
-    const uint32_t last = jb.rm.start + jb.rm.E - 1u;
-    const uint32_t pos = last < jb.rm.info ? last : last + (jb.rm.nsys - jb.rm.info);
-    rows = min(M, max(4, (int)(pos / (uint32_t)Z) + 2 - K + 1));
-  }
+  const int rows = encode_parity_rows(jb.rm, K, M, Z, jb.want_cw != 0);
   for (int task = t; task < rows * W; task += NT) {
     const int row = task / W, w = task - row * W;
     const int n = min(32, Z - 32 * w);
@@ -643,16 +638,20 @@ __global__ void __launch_bounds__(384) k_gen_codeblocks(const gen_job* __restric
 // LDPC encoder + rate matcher, one workgroup per codeblock (message from msgs).  PACKED:
 // the codeblocks with Z >= 32 (word-parallel encoder), otherwise those with Z < 32; each
 // instance leaves the others' codeblocks alone.
+// PACKED (lifting sizes from 32 on): the packed words and then the codeword bytes in
+// dynamic LDS, sized by the host for the columns the launch's codeblocks need.
 template <bool PACKED>
 __global__ void __launch_bounds__(384) k_encode_rate_match(const gen_job* __restrict__ jobs, uint8_t* __restrict__ msgs,
                                                            uint8_t* __restrict__ out_bits, uint8_t* __restrict__ cw_bits,
                                                            const uint32_t* __restrict__ crc_x,
                                                            const uint32_t* __restrict__ gtab)
 {
-  __shared__ __align__(16) uint8_t cw[kGenMaxBytes];
+  extern __shared__ __align__(16) uint8_t enc_lds[];
+  __shared__ __align__(16) uint8_t cw_static[PACKED ? 16 : kGenMaxBytes];
   __shared__ uint8_t lam[PACKED ? 1 : 4 * kMaxZ];
   __shared__ uint32_t red[16];
-  __shared__ uint32_t pk[PACKED ? kPackedWords : 1];
+  uint32_t* pk = PACKED ? reinterpret_cast<uint32_t*>(enc_lds) : nullptr;
+  uint8_t* cw = PACKED ? enc_lds + 4 * kPackedWords : cw_static;
   const gen_job jb = jobs[blockIdx.x];
   if ((jb.Z >= 32) != PACKED) {
     return;
@@ -766,12 +765,17 @@ hipError_t launch_gen_codeblocks(const gen_job* d_jobs, unsigned n, int8_t* llrs
   return hipGetLastError();
 }
 
+// Threads of a word-parallel encoder workgroup: its phases have at most a few hundred
+// tasks, and small workgroups with small LDS keep more codeblocks resident per CU.
+constexpr int kEncodeThreads = 192;
+
 hipError_t launch_encode_rate_match(const gen_job* d_jobs, unsigned n, uint8_t* msgs, uint8_t* out_bits,
                                     uint8_t* cw_bits, const uint32_t* crc_x, const uint32_t* gtab, bool any_small,
-                                    hipStream_t s)
+                                    size_t packed_cw_bytes, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_encode_rate_match<true>, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits,
-                     crc_x, gtab);
+  const size_t lds = 4 * (size_t)kPackedWords + ((packed_cw_bytes + 15) & ~(size_t)15);
+  hipLaunchKernelGGL(k_encode_rate_match<true>, dim3(n), dim3(kEncodeThreads), lds, s, d_jobs, msgs, out_bits,
+                     cw_bits, crc_x, gtab);
   if (any_small) {
     hipLaunchKernelGGL(k_encode_rate_match<false>, dim3(n), dim3(kMaxThreads), 0, s, d_jobs, msgs, out_bits, cw_bits,
                        crc_x, gtab);

@@ -60,6 +60,28 @@ struct alignas(16) rm_params {
 };
 static_assert(sizeof(rm_params) == 64, "rm_params layout");
 
+#if defined(__HIP__)
+#define LH_HOST_DEVICE __host__ __device__
+#else
+#define LH_HOST_DEVICE
+#endif
+
+// Parity rows (of M) the word-parallel encoder computes for one codeblock (K_b systematic
+// columns, lifting size Z): all of them with the codeword output or when the rate
+// matcher's ranks [start, start + E) wrap; otherwise the rows up to the column of the
+// last position it reads (past the fillers), at least the four core rows.  The host
+// sizes the launch's LDS from it.
+LH_HOST_DEVICE inline int encode_parity_rows(const rm_params& rm, int K, int M, int Z, bool want_cw)
+{
+  if (want_cw || rm.start + rm.E > rm.P) {
+    return M;
+  }
+  const uint32_t last = rm.start + rm.E - 1u;
+  const uint32_t pos = last < rm.info ? last : last + (rm.nsys - rm.info);
+  const int rows = (int)(pos / (uint32_t)Z) + 2 - K + 1;
+  return rows < 4 ? 4 : (rows > M ? M : rows);
+}
+
 // One codeblock decode, as seen by the decoder kernel (112 bytes).
 struct alignas(16) dec_job {
   uint64_t in_off;   // non-fused: decoder input (rate-dematched soft bits) byte offset
