@@ -756,8 +756,8 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   // Lifted edge tables: (col * Z) | ((V mod Z) << 16) per edge, rows in order
   // (ldpc_graph_impl.cpp:11-39, ldpc_luts_impl.cpp:4505-4523).
   // Behind them the row-pair tables of the decoder's two-rows-per-thread mode (even Z):
-  // two words per edge, col * Z | S << 16 and S - T, with h = Z / 2, f = (shift >= h),
-  // S = 2 (shift mod h) + f and T = Z - 1 + 2f.
+  // two words per edge, col * Z | S << 16 and the 16-bit pair S | (S - T) << 16, with
+  // h = Z / 2, f = (shift >= h), S = 2 (shift mod h) + f and T = Z - 1 + 2f.
   std::vector<uint32_t> gt(3 * (size_t)kGraphTableLen);
   for (int pos = 0; pos < kNofLiftingSizes; ++pos) {
     const unsigned Z = kLiftingSizes[pos];
@@ -770,7 +770,7 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
         const unsigned h = Z / 2, f = sh >= h ? 1u : 0u;
         const unsigned S = 2 * (sh - f * h) + f, T = Z - 1 + 2 * f;
         gt[kGraphTableLen + 2 * o] = (uint32_t)(col * Z) | (S << 16);
-        gt[kGraphTableLen + 2 * o + 1] = S - T;
+        gt[kGraphTableLen + 2 * o + 1] = (S & 0xffffu) | (((S - T) & 0xffffu) << 16);
       }
     };
     for (int k = 0; k < 316; ++k) {

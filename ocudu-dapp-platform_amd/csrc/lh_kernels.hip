@@ -301,15 +301,6 @@ __device__ __forceinline__ void layer_update_ext8(int8_t* __restrict__ soft, int
 // do 1.4x the instructions of one row each: the mode pays where a compute unit's
 // resident codeblocks are limited by waves rather than by LDS.
 
-// LDS byte of row (r + shift) mod Z of a column for thread r (and a ^ 1 for row r + h).
-// With u = r + (shift mod h), f = (shift >= h), S = 2 (shift mod h) + f and
-// T = Z - 1 + 2f, the byte is min(2r + S, 2r + S - T) in unsigned arithmetic (the second
-// wraps exactly when the row stays in its half).  The row-pair edge table holds two
-// words per edge: col * Z | S << 16, and S - T.
-__device__ __forceinline__ int pair_addr(uint32_t g0, uint32_t g1, unsigned r2)
-{
-  return (int)((g0 & 0xffffu) + min(r2 + (g0 >> 16), r2 + g1));
-}
 // LDS byte of soft-bit index q = column * Z + j (j = q mod Z): q itself in the one-row
 // layout, the row-pair byte with PAIR.
 template <bool PAIR>
@@ -324,6 +315,19 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ i16x2 as_i16x2(uint32_t w) { return __builtin_bit_cast(i16x2, w); }
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
+
+// LDS byte of row (r + shift) mod Z of a column for thread r (and a ^ 1 for row r + h).
+// With u = r + (shift mod h), f = (shift >= h), S = 2 (shift mod h) + f and
+// T = Z - 1 + 2f, the byte is min(2r + S, 2r + S - T) in 16-bit unsigned arithmetic (the
+// second wraps exactly when the row stays in its half; both stay below 2Z otherwise).
+// The row-pair edge table holds two words per edge: col * Z | S << 16, and the 16-bit
+// pair S | (S - T) << 16, so one packed add forms both candidates.  r2x2 = 2r in both
+// halves.
+__device__ __forceinline__ int pair_addr(uint32_t g0, uint32_t g1, uint32_t r2x2)
+{
+  const u16x2 t = as_u16x2(r2x2) + as_u16x2(g1);
+  return (int)((g0 & 0xffffu) + min((uint32_t)t.x, (uint32_t)t.y));
+}
 template <typename V>
 __device__ __forceinline__ uint32_t as_u32(V v)
 {
@@ -381,6 +385,7 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
   const u16x2 k32 = as_u16x2(k32_w);
   // LDS address of the group's soft bits (even: the partner byte is address ^ 1).
   const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
+  const uint32_t r2x2 = 2u * r * 0x10001u;
   uint32_t a1[DEG];
   i16x2 sv[DEG];
   // Four edges' eight reads in flight before one wait.
@@ -392,7 +397,7 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
       const int e = e0 + k;
       raw[2 * k] = raw[2 * k + 1] = 0;
       if (e < DEG && (!RT || e < deg)) {
-        a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + gbase;
+        a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], r2x2) + gbase;
         raw[2 * k] = *lds_at(a1[e]);
         raw[2 * k + 1] = *lds_at(a1[e] ^ 1u);
       }
@@ -583,7 +588,7 @@ __device__ uint32_t syndrome_fail_pair(const int8_t* soft, int goff, const uint3
     const int e0 = bgt<BG>::rs_word(l), e1 = bgt<BG>::rs_word(l + 1);
     uint32_t p = 0;
     for (int e = e0; e < e1; ++e) {
-      const int a = pair_addr(gt[2 * e], gt[2 * e + 1], 2 * r) + goff;
+      const int a = pair_addr(gt[2 * e], gt[2 * e + 1], 2u * r * 0x10001u) + goff;
       const int s0 = soft[a], s1 = soft[a ^ 1];
       bad |= (uint32_t)(s0 == 0) | (uint32_t)(s1 == 0);
       p ^= (uint32_t)(s0 <= 0) | ((uint32_t)(s1 <= 0) << 1);
