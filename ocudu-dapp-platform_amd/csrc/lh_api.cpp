@@ -658,9 +658,11 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
   return LH_OK;
 }
 
-// Launches a packed block resident at d_block: the decoder (learned with scale_tab), then
-// with fb_iters > 0 the plain decoder as its fallback over the same descriptors; each pass
-// runs the row-pair queue, then the one-row queue (each a persistent launch).
+// Launches a packed block resident at d_block: the decoder (learned with scale_tab) over
+// the row-pair queue, then the one-row queue (each a persistent launch).  With fb_iters > 0
+// the plain decoder is the fallback over the same descriptors: after a learned decoder,
+// inside its launch (each workgroup re-decodes its item's failed codeblocks at once); after
+// the plain one, as a second pass.
 hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int8_t* rm, const int8_t* cw,
                         uint8_t* bits, int32_t* results, hipStream_t stream, const uint32_t* scale_tab,
                         unsigned fb_iters)
@@ -668,16 +670,17 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
   launch_header* d_hdr = reinterpret_cast<launch_header*>(d_block);
   const dec_item* d_items = reinterpret_cast<const dec_item*>(d_block + pb.o_items);
   const dec_job* d_jobs = reinterpret_cast<const dec_job*>(d_block + pb.o_jobs);
+  const bool inline_fb = scale_tab != nullptr && fb_iters > 0;
   hipError_t e = hipSuccess;
-  for (int pass = 0; pass < (fb_iters > 0 ? 2 : 1) && e == hipSuccess; ++pass) {
+  for (int pass = 0; pass < (fb_iters > 0 && !inline_fb ? 2 : 1) && e == hipSuccess; ++pass) {
     const uint32_t* tab = pass == 0 ? scale_tab : nullptr;
+    const int fb = inline_fb || pass == 1 ? (int)fb_iters : 0;
     for (int m = 1; m >= 0 && e == hipSuccess; --m) {
       if (pb.n_items[m] == 0) {
         continue;
       }
       e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, pb.grid[m], pb.nt[m], pb.lds[m], rm, cw, bits,
-                        results, ctx->d_crc_xz, ctx->d_gtab, tab, pass == 0 ? 0 : (int)fb_iters, m == 1,
-                        m == 0 && pb.ext8, stream);
+                        results, ctx->d_crc_xz, ctx->d_gtab, tab, fb, m == 1, m == 0 && pb.ext8, stream);
     }
   }
   return e;

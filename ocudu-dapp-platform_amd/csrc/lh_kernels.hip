@@ -1112,8 +1112,10 @@ constexpr int kQueueBytes = 16;
 // the threads beyond G*n only take part in barriers.  Both modes use the same LDS
 // amounts (only the order of the soft bits in a column differs).
 // Inlined into each kernel instance: as a call its register state would spill.
+// Returns whether this thread's group wrote a result <= 0 (a codeblock the decoder did not
+// pass).
 template <int BG, bool LEARNED, bool PAIR, bool EXT8>
-__device__ __forceinline__ void decode_item(const dec_item& item, const dec_job* __restrict__ jobs,
+__device__ __forceinline__ bool decode_item(const dec_item& item, const dec_job* __restrict__ jobs,
                                             int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base,
                                             uint8_t* __restrict__ bits_base, int32_t* __restrict__ results,
                                             const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab,
@@ -1590,6 +1592,7 @@ __device__ __forceinline__ void decode_item(const dec_item& item, const dec_job*
     }
   }
   __syncthreads(); // the next item reuses the LDS
+  return active && !skip && result <= 0;
 }
 
 // Persistent decoder.  The host sorts the items largest first.  Workgroup w takes item w
@@ -1614,12 +1617,31 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
   uint32_t* ctr = reinterpret_cast<uint32_t*>(hdr) + hdr->ctr_off + q * kCounterStride;
   for (uint32_t id = w; id < n_items;) {
     const dec_item item = items[hdr->first_item + id];
+    // LEARNED with fb_iters > 0: the learned decoder, then at once in this workgroup the
+    // plain decoder (fb_iters iterations, on the HARQ buffer the learned pass wrote) over
+    // the item's codeblocks whose result is <= 0 -- what a separate fallback launch over
+    // the same descriptors does, without waiting for the launch's other items.
+    bool failed;
     if (item.bg == 1) {
-      decode_item<1, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, scale_tab,
-                                    fb_iters, lds);
+      failed = decode_item<1, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
+                                                   scale_tab, LEARNED ? 0 : fb_iters, lds);
     } else {
-      decode_item<2, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
-                                    LEARNED ? scale_tab + kLearnedIters * 46 : nullptr, fb_iters, lds);
+      failed = decode_item<2, LEARNED, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab,
+                                                   LEARNED ? scale_tab + kLearnedIters * 46 : nullptr,
+                                                   LEARNED ? 0 : fb_iters, lds);
+    }
+    if constexpr (LEARNED) {
+      // The results the fallback's skip test reads were written before decode_item's
+      // closing barrier; the OR is itself a barrier of the whole workgroup.
+      if (fb_iters > 0 && __syncthreads_or(failed ? 1 : 0) != 0) {
+        if (item.bg == 1) {
+          decode_item<1, false, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, nullptr,
+                                            fb_iters, lds);
+        } else {
+          decode_item<2, false, PAIR, EXT8>(item, jobs, rm_base, cw_base, bits_base, results, crc_x, gtab, nullptr,
+                                            fb_iters, lds);
+        }
+      }
     }
     if (nq == 0) {
       break;
