@@ -1690,7 +1690,6 @@ __global__ void __launch_bounds__(256) k_crc(const uint64_t* __restrict__ offs, 
   }
 }
 
-// GF(2)[x] / G helpers for the transport-block CRC (order <= 24).
 // Byte of the concatenated TB data whose first bit is bit j of codeblock c (MSB-first
 // bytes, codeblocks `stride` bytes apart, D data bits each); the bits past D come from the
 // next codeblock.  Positions past the TB's last codeblock read its CRC bits, which the
