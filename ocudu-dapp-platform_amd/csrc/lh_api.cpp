@@ -27,7 +27,7 @@ size_t decode_item_lds_bytes(int bg, int Z, int lmax, int G, int stage, bool ext
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         bool pairs, bool ext8, hipStream_t s);
+                         bool pairs, bool ext8, hipStream_t s, const fused_tb_args* fused = nullptr);
 int decode_occupancy(int nt, size_t lds, bool learned, bool pairs, bool ext8);
 hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
@@ -566,7 +566,8 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
       it.stage = (uint32_t)stage;
       need = decode_item_lds_bytes(bg, Z, lmax, it.ngroups, stage, false);
     }
-    lds[m] = std::max(lds[m], need);
+    // At least the scratch of a fused TB finalize (between items) and the queue word.
+    lds[m] = std::max(lds[m], std::max(need, kFusedTbLdsBytes + 64));
     items.push_back(it);
     i = k;
   }
@@ -665,7 +666,7 @@ int build_block(lh_ctx* ctx, launch_plan& plan, plan_block& pb, uint8_t* dst = n
 // the plain one, as a second pass.
 hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int8_t* rm, const int8_t* cw,
                         uint8_t* bits, int32_t* results, hipStream_t stream, const uint32_t* scale_tab,
-                        unsigned fb_iters)
+                        unsigned fb_iters, const fused_tb_args* fused = nullptr)
 {
   launch_header* d_hdr = reinterpret_cast<launch_header*>(d_block);
   const dec_item* d_items = reinterpret_cast<const dec_item*>(d_block + pb.o_items);
@@ -680,7 +681,7 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
         continue;
       }
       e = launch_decode(d_hdr + 2 * pass + (1 - m), d_items, d_jobs, pb.grid[m], pb.nt[m], pb.lds[m], rm, cw, bits,
-                        results, ctx->d_crc_xz, ctx->d_gtab, tab, fb, m == 1, m == 0 && pb.ext8, stream);
+                        results, ctx->d_crc_xz, ctx->d_gtab, tab, fb, m == 1, m == 0 && pb.ext8, stream, fused);
     }
   }
   return e;
@@ -688,7 +689,8 @@ hipError_t launch_block(lh_ctx* ctx, const plan_block& pb, uint8_t* d_block, int
 
 // Packs, uploads (staging ring) and launches a plan.
 int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
-             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr, unsigned fb_iters = 0)
+             hipStream_t stream, slot* s, size_t off, const uint32_t* scale_tab = nullptr, unsigned fb_iters = 0,
+             const fused_tb_args* fused = nullptr)
 {
   if (plan.jobs.empty()) {
     return LH_OK;
@@ -703,7 +705,7 @@ int run_plan(lh_ctx* ctx, launch_plan& plan, int8_t* rm, const int8_t* cw, uint8
   if (rc != LH_OK) {
     return rc;
   }
-  hipError_t e = launch_block(ctx, pb, s->dev + off, rm, cw, bits, results, stream, scale_tab, fb_iters);
+  hipError_t e = launch_block(ctx, pb, s->dev + off, rm, cw, bits, results, stream, scale_tab, fb_iters, fused);
   retire(s, stream);
   return e == hipSuccess ? LH_OK : hip_fail(e, "decoder launch");
 }
@@ -2492,25 +2494,42 @@ int lh_server_step(lh_server* sv)
   sig.counter = sl->d_counter;
   sig.flag = sl->h_flag_dev;
   sig.seq = ++sl->seq;
-  // One upload carries the TB descriptors (at the start of the staging slot) and the
-  // decoder's descriptor block behind them, ahead of the decode launch.
+  // One upload carries, at the start of the staging slot, the TB descriptors, each
+  // codeblock's transport block and each transport block's codeblock count, then the
+  // decoder's descriptor block: the decoder launch reassembles and checks every transport
+  // block as soon as its last codeblock is decoded (fused TB finalize).
   slot* s = acquire_slot(ctx);
   if (s == nullptr) {
     reject_slot(sv, *sl);
     return LH_ERR_HIP;
   }
   const size_t tb_desc_bytes = align256(tbs.size() * sizeof(lh_tb_desc));
+  const size_t cb_tb_at = tb_desc_bytes, tb_left_at = cb_tb_at + align256(cbs.size() * 4);
+  const size_t prefix = tb_left_at + align256(tbs.size() * 4);
   std::memcpy(s->host, tbs.data(), tbs.size() * sizeof(lh_tb_desc));
+  uint32_t* h_cb_tb = reinterpret_cast<uint32_t*>(s->host + cb_tb_at);
+  uint32_t* h_tb_left = reinterpret_cast<uint32_t*>(s->host + tb_left_at);
+  for (size_t k = 0; k < tbs.size(); ++k) {
+    h_tb_left[k] = tbs[k].nof_cbs;
+    for (uint32_t c = 0; c < tbs[k].nof_cbs; ++c) {
+      h_cb_tb[tbs[k].first_cb + c] = (uint32_t)k;
+    }
+  }
+  fused_tb_args fa{};
+  fa.tbs = reinterpret_cast<const lh_tb_desc_dev*>(s->dev);
+  fa.cb_tb = reinterpret_cast<const uint32_t*>(s->dev + cb_tb_at);
+  fa.tb_left = reinterpret_cast<uint32_t*>(s->dev + tb_left_at);
+  fa.tb_out = sl->h_out_dev + tb_at;
+  fa.status = reinterpret_cast<int32_t*>(sl->h_out_dev);
+  fa.lut24a = ctx->d_tbcrc;
+  fa.sig = sig;
+  fa.n_tbs = (uint32_t)tbs.size();
   uint8_t* arena_bytes = reinterpret_cast<uint8_t*>(sv->d_arena);
-  rc = run_plan(ctx, plan, sv->d_arena, cw_base, arena_bytes, sl->d_res, sl->stream, s, tb_desc_bytes,
-                sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0);
+  rc = run_plan(ctx, plan, sv->d_arena, cw_base, arena_bytes, sl->d_res, sl->stream, s, prefix,
+                sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0, &fa);
   hipError_t e = hipSuccess;
   if (rc == LH_OK) {
-    e = launch_tb_finalize(reinterpret_cast<const lh_tb_desc_dev*>(s->dev), (unsigned)tbs.size(), arena_bytes,
-                           sl->d_res, sl->h_out_dev + tb_at, reinterpret_cast<int32_t*>(sl->h_out_dev), ctx->d_tbcrc,
-                           ctx->d_tbcrc + 256, sl->stream, &sig);
-    retire(s, sl->stream);
-    if (e == hipSuccess) e = hipEventRecord(sl->done, sl->stream);
+    e = hipEventRecord(sl->done, sl->stream);
   }
   if (rc != LH_OK || e != hipSuccess) {
     if (rc == LH_OK) {

@@ -9,6 +9,8 @@ namespace lh {
 
 constexpr int kMaxZ = 384;
 constexpr int kMaxThreads = 384;
+// Devices whose per-kernel launch attributes the library caches (others set them per launch).
+constexpr int kMaxCachedDevices = 16;
 // x^e mod G tables: e in [0, 22*384 + 24).
 constexpr int kCrcXLen = 22 * 384 + 24;
 // The decoder's CRC weights: per polynomial kCrcPad zeros, then x^e mod G for e in
@@ -213,6 +215,26 @@ struct tb_signal {
   uint32_t seq;
   uint32_t pad;
 };
+
+// TB finalize inside the decoder launch (the serving runtime): every workgroup, after an
+// item, counts its codeblocks off their transport blocks' tb_left; the workgroup that
+// counts a transport block's last codeblock reassembles and checks it at once (as one
+// workgroup of the TB finalize launch would, with the same signal).  tbs == nullptr: off.
+struct fused_tb_args {
+  const lh_tb_desc_dev* tbs; // the launch's transport blocks
+  const uint32_t* cb_tb;     // per codeblock (decoder result index): its transport block
+  uint32_t* tb_left;         // per transport block: codeblocks not yet counted (nof_cbs at upload)
+  uint8_t* tb_out;           // TB bytes (host-visible)
+  int32_t* status;           // TB verdicts (host-visible)
+  const uint32_t* lut24a;    // CRC24A byte table, then x^(2^k) and the byte-shift digits
+  tb_signal sig;
+  uint32_t n_tbs;
+  uint32_t pad;
+};
+// LDS the TB finalize of a fused launch uses at the start of the decoder's dynamic LDS:
+// the CRC tables (352 words), the reduction words (32), one flag, the count and the list
+// of transport blocks to finalize (up to kMaxGroups).
+constexpr size_t kFusedTbLdsBytes = 4 * (352 + 32 + 1 + 1 + 64);
 
 // Demodulation + descrambling: one block (a span of symbols with its scrambling
 // sequence start) and one tile of 256 of its symbols per workgroup.

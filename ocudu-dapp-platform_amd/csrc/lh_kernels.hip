@@ -11,6 +11,8 @@
 // workgroup).  Barriers separate layers (strict layer order of the reference).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "bg_tables.inc"
 #include "lh_gf.h"
 #include "lh_internal.h"
@@ -1595,6 +1597,52 @@ __device__ __forceinline__ bool decode_item(const dec_item& item, const dec_job*
   return active && !skip && result <= 0;
 }
 
+template <bool SIGNAL>
+__device__ __forceinline__ void tb_finalize_signal(uint32_t tb, uint32_t n_tbs, const lh_tb_desc_dev* __restrict__ tbs,
+                                                   const uint8_t* __restrict__ cb_bits,
+                                                   const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
+                                                   int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
+                                                   const uint32_t* __restrict__ pow2_24a, const tb_signal& sig,
+                                                   uint32_t* red, uint32_t* lut, int& all_ok);
+
+// Fused TB finalize (the serving runtime): the item's codeblocks are counted off their
+// transport blocks (one thread per group; its device-scope fence first makes the
+// workgroup's bits and results, complete at the item's closing barrier, visible to every
+// workgroup); the transport blocks whose last codeblock this workgroup counted are then
+// finalized here, after a fence that makes the other workgroups' codeblocks visible.
+// Scratch: the start of the dynamic LDS, free between items.
+__device__ __forceinline__ void fused_tb_complete(const dec_item& item, const dec_job* __restrict__ jobs,
+                                                  const fused_tb_args& fa, const uint8_t* __restrict__ bits,
+                                                  const int32_t* __restrict__ results, uint8_t* lds)
+{
+  uint32_t* lut = reinterpret_cast<uint32_t*>(lds);
+  uint32_t* red = lut + 352;
+  int& all_ok = *reinterpret_cast<int*>(red + 32);
+  uint32_t& n_done = red[33];
+  uint32_t* done = red + 34;
+  const int t = (int)threadIdx.x;
+  if (t == 0) {
+    n_done = 0;
+  }
+  __syncthreads();
+  if (t < (int)item.ngroups) {
+    const uint32_t tb = fa.cb_tb[jobs[item.first_job + t].out_idx];
+    __threadfence();
+    if (atomicSub(&fa.tb_left[tb], 1u) == 1u) {
+      done[atomicAdd(&n_done, 1u)] = tb;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = n_done;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t tb = done[k];
+    __threadfence();
+    tb_finalize_signal<true>(tb, fa.n_tbs, fa.tbs, bits, results, fa.tb_out, fa.status, fa.lut24a, fa.lut24a + 256,
+                             fa.sig, red, lut, all_ok);
+    __syncthreads(); // the scratch serves the next transport block
+  }
+}
+
 // Persistent decoder.  The host sorts the items largest first.  Workgroup w takes item w
 // (the first round needs no claim), then claims items of its sub-queue w mod nq through
 // that sub-queue's counter: the claims of a launch spread over nq device-scope counters
@@ -1608,7 +1656,7 @@ __global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_e
 k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ items, const dec_job* __restrict__ jobs,
               int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
               int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab,
-              const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot)
+              const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot, fused_tb_args fa)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t& claimed = *reinterpret_cast<uint32_t*>(lds + qslot);
@@ -1642,6 +1690,9 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
                                             fb_iters, lds);
         }
       }
+    }
+    if (fa.tbs != nullptr) {
+      fused_tb_complete(item, jobs, fa, bits_base, results, lds);
     }
     if (nq == 0) {
       break;
@@ -1720,7 +1771,7 @@ __device__ __forceinline__ uint32_t tb_byte(const uint8_t* __restrict__ first, u
 // WIDE: the TB bytes go out as aligned 4-byte words (the TB offset is a multiple of 4),
 // which matters when they are written across the bus into host memory.
 template <bool WIDE>
-__device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const uint8_t* __restrict__ cb_bits,
+__device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, uint32_t tb, const uint8_t* __restrict__ cb_bits,
                                                 const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
                                                 int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
                                                 const uint32_t* __restrict__ pow2_24a, uint32_t* red, uint32_t* lut,
@@ -1765,13 +1816,13 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
       }
     }
     if (t == 0) {
-      status[blockIdx.x] = all_ok ? 1 : 0;
+      status[tb] = all_ok ? 1 : 0;
     }
     return;
   }
   if (!all_ok) {
     if (t == 0) {
-      status[blockIdx.x] = 0;
+      status[tb] = 0;
     }
     return;
   }
@@ -1869,15 +1920,46 @@ __device__ __forceinline__ void tb_finalize_one(const lh_tb_desc_dev& d, const u
     const uint8_t* src = first + (size_t)last_c * d.cb_bits_stride + (start >> 3);
     const uint32_t win = ((uint32_t)src[0] << 24) | ((uint32_t)src[1] << 16) | ((uint32_t)src[2] << 8) | src[3];
     const uint32_t ck = (win >> (8u - (start & 7u))) & 0xffffffu;
-    status[blockIdx.x] = (x == ck) ? 1 : -1;
+    status[tb] = (x == ck) ? 1 : -1;
   }
 }
 
 // Threads of a TB workgroup: a large TB's concatenation and CRC chunks spread over 16 waves.
 constexpr int kTbThreads = 1024;
 
-// One workgroup per transport block.  SIGNAL: the outputs are host-visible and the launch
-// reports its completion through `sig` (no copy back, no event on the latency path).
+// TB finalize of transport block tb of n_tbs (one workgroup).  SIGNAL: the outputs are
+// host-visible; the codeblock results go to sig.res_out as well, and the last of the
+// n_tbs to finish stores sig.seq into *sig.flag (no copy back, no event on the latency
+// path).  red (32 words), lut (256 + 96 words) and all_ok are workgroup scratch.
+template <bool SIGNAL>
+__device__ __forceinline__ void tb_finalize_signal(uint32_t tb, uint32_t n_tbs, const lh_tb_desc_dev* __restrict__ tbs,
+                                                   const uint8_t* __restrict__ cb_bits,
+                                                   const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
+                                                   int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
+                                                   const uint32_t* __restrict__ pow2_24a, const tb_signal& sig,
+                                                   uint32_t* red, uint32_t* lut, int& all_ok)
+{
+  const lh_tb_desc_dev d = tbs[tb];
+  tb_finalize_one<SIGNAL>(d, tb, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
+  if constexpr (SIGNAL) {
+    for (uint32_t c = threadIdx.x; c < d.nof_cbs; c += blockDim.x) {
+      sig.res_out[d.first_cb + c] = cb_res[d.first_cb + c];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t before = atomicAdd(sig.counter, 1u);
+      if (before == n_tbs - 1) {
+        // Every transport block's writes are fenced before its increment: the launch is whole.
+        *sig.counter = 0;
+        __threadfence_system();
+        __hip_atomic_store(sig.flag, sig.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+  }
+}
+
+// One workgroup per transport block.
 template <bool SIGNAL>
 __global__ void __launch_bounds__(kTbThreads) k_tb_finalize(const lh_tb_desc_dev* __restrict__ tbs,
                                                      const uint8_t* __restrict__ cb_bits,
@@ -1888,24 +1970,8 @@ __global__ void __launch_bounds__(kTbThreads) k_tb_finalize(const lh_tb_desc_dev
   __shared__ uint32_t red[32];
   __shared__ uint32_t lut[256 + 16 * kTbShiftDigits];
   __shared__ int all_ok;
-  const lh_tb_desc_dev d = tbs[blockIdx.x];
-  tb_finalize_one<SIGNAL>(d, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, red, lut, all_ok);
-  if constexpr (SIGNAL) {
-    for (uint32_t c = threadIdx.x; c < d.nof_cbs; c += blockDim.x) {
-      sig.res_out[d.first_cb + c] = cb_res[d.first_cb + c];
-    }
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t before = atomicAdd(sig.counter, 1u);
-      if (before == gridDim.x - 1) {
-        // Every workgroup's writes are fenced before its increment: the launch is whole.
-        *sig.counter = 0;
-        __threadfence_system();
-        __hip_atomic_store(sig.flag, sig.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      }
-    }
-  }
+  tb_finalize_signal<SIGNAL>(blockIdx.x, gridDim.x, tbs, cb_bits, cb_res, tb_out, status, lut24a, pow2_24a, sig, red,
+                             lut, all_ok);
 }
 
 // ---- host-side launchers ------------------------------------------------------------
@@ -1933,20 +1999,34 @@ const void* decode_kernel(bool learned, bool pairs, bool ext8)
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         bool pairs, bool ext8, hipStream_t s)
+                         bool pairs, bool ext8, hipStream_t s, const fused_tb_args* fused)
 {
+  const fused_tb_args fa = fused != nullptr ? *fused : fused_tb_args{};
   const bool learned = scale_tab != nullptr;
   ext8 = ext8 && !pairs;
   const void* fn = decode_kernel(learned, pairs, ext8);
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) {
-    return e;
+  // The dynamic-LDS limit of each kernel instance per device only ever rises: a launch
+  // within the limit already set makes no runtime call.
+  static std::atomic<int> lds_limit[kMaxCachedDevices][6];
+  int dev = 0;
+  const bool cached = hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxCachedDevices;
+  std::atomic<int>* lim = cached ? &lds_limit[dev][(learned ? 3 : 0) + (pairs ? 0 : (ext8 ? 1 : 2))] : nullptr;
+  if (lim == nullptr || (int)lds > lim->load(std::memory_order_relaxed)) {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      return e;
+    }
+    if (lim != nullptr) {
+      int cur = lim->load(std::memory_order_relaxed);
+      while ((int)lds > cur && !lim->compare_exchange_weak(cur, (int)lds, std::memory_order_relaxed)) {
+      }
+    }
   }
   // lds (from decode_item_lds_bytes, a multiple of 4) ends with the claimed-item word.
   const unsigned qslot = (unsigned)lds - kQueueBytes;
 #define LH_LAUNCH(L, P, X)                                                                                        \
   hipLaunchKernelGGL((k_ldpc_decode<L, P, X>), dim3(grid), dim3(nt), lds, s, d_hdr, d_items, d_jobs, rm, cw, bits, \
-                     results, crc_x, gtab, scale_tab, fb_iters, qslot)
+                     results, crc_x, gtab, scale_tab, fb_iters, qslot, fa)
   if (pairs) {
     if (learned) {
       LH_LAUNCH(true, true, false);
