@@ -28,6 +28,7 @@ hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const de
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
                          bool pairs, bool ext8, hipStream_t s, const fused_tb_args* fused = nullptr);
+// (fused: a device pointer to the arguments of the fused TB finalize, or nullptr)
 int decode_occupancy(int nt, size_t lds, bool learned, bool pairs, bool ext8);
 hipError_t launch_reset_counters(uint32_t* ctr, unsigned words, hipStream_t s);
 hipError_t launch_tb_finalize(const lh_tb_desc_dev* d_tbs, unsigned n, const uint8_t* cb_bits, const int32_t* cb_res,
@@ -2505,7 +2506,8 @@ int lh_server_step(lh_server* sv)
   }
   const size_t tb_desc_bytes = align256(tbs.size() * sizeof(lh_tb_desc));
   const size_t cb_tb_at = tb_desc_bytes, tb_left_at = cb_tb_at + align256(cbs.size() * 4);
-  const size_t prefix = tb_left_at + align256(tbs.size() * 4);
+  const size_t fa_at = tb_left_at + align256(tbs.size() * 4);
+  const size_t prefix = fa_at + align256(sizeof(fused_tb_args));
   std::memcpy(s->host, tbs.data(), tbs.size() * sizeof(lh_tb_desc));
   uint32_t* h_cb_tb = reinterpret_cast<uint32_t*>(s->host + cb_tb_at);
   uint32_t* h_tb_left = reinterpret_cast<uint32_t*>(s->host + tb_left_at);
@@ -2524,9 +2526,11 @@ int lh_server_step(lh_server* sv)
   fa.lut24a = ctx->d_tbcrc;
   fa.sig = sig;
   fa.n_tbs = (uint32_t)tbs.size();
+  std::memcpy(s->host + fa_at, &fa, sizeof(fa)); // uploaded with the descriptors
   uint8_t* arena_bytes = reinterpret_cast<uint8_t*>(sv->d_arena);
   rc = run_plan(ctx, plan, sv->d_arena, cw_base, arena_bytes, sl->d_res, sl->stream, s, prefix,
-                sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0, &fa);
+                sv->cfg.learned ? ctx->d_scale : nullptr, sv->cfg.learned ? sv->cfg.nof_fallback_iterations : 0,
+                reinterpret_cast<const fused_tb_args*>(s->dev + fa_at));
   hipError_t e = hipSuccess;
   if (rc == LH_OK) {
     e = hipEventRecord(sl->done, sl->stream);

@@ -1602,19 +1602,20 @@ __device__ __forceinline__ void tb_finalize_signal(uint32_t tb, uint32_t n_tbs, 
                                                    const uint8_t* __restrict__ cb_bits,
                                                    const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
                                                    int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
-                                                   const uint32_t* __restrict__ pow2_24a, const tb_signal& sig,
+                                                   const uint32_t* __restrict__ pow2_24a, tb_signal sig,
                                                    uint32_t* red, uint32_t* lut, int& all_ok);
 
-// Fused TB finalize (the serving runtime): the item's codeblocks are counted off their
+// Fused TB finalize (the serving runtime; fap in device memory): the item's codeblocks are counted off their
 // transport blocks (one thread per group; its device-scope fence first makes the
 // workgroup's bits and results, complete at the item's closing barrier, visible to every
 // workgroup); the transport blocks whose last codeblock this workgroup counted are then
 // finalized here, after a fence that makes the other workgroups' codeblocks visible.
 // Scratch: the start of the dynamic LDS, free between items.
 __device__ __forceinline__ void fused_tb_complete(const dec_item& item, const dec_job* __restrict__ jobs,
-                                                  const fused_tb_args& fa, const uint8_t* __restrict__ bits,
+                                                  const fused_tb_args* __restrict__ fap, const uint8_t* __restrict__ bits,
                                                   const int32_t* __restrict__ results, uint8_t* lds)
 {
+  const fused_tb_args fa = *fap;
   uint32_t* lut = reinterpret_cast<uint32_t*>(lds);
   uint32_t* red = lut + 352;
   int& all_ok = *reinterpret_cast<int*>(red + 32);
@@ -1656,7 +1657,8 @@ __global__ void __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_e
 k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ items, const dec_job* __restrict__ jobs,
               int8_t* __restrict__ rm_base, const int8_t* __restrict__ cw_base, uint8_t* __restrict__ bits_base,
               int32_t* __restrict__ results, const uint32_t* __restrict__ crc_x, const uint32_t* __restrict__ gtab,
-              const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot, fused_tb_args fa)
+              const uint32_t* __restrict__ scale_tab, int fb_iters, unsigned qslot,
+              const fused_tb_args* __restrict__ fa)
 {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t& claimed = *reinterpret_cast<uint32_t*>(lds + qslot);
@@ -1691,7 +1693,7 @@ k_ldpc_decode(launch_header* __restrict__ hdr, const dec_item* __restrict__ item
         }
       }
     }
-    if (fa.tbs != nullptr) {
+    if (fa != nullptr) {
       fused_tb_complete(item, jobs, fa, bits_base, results, lds);
     }
     if (nq == 0) {
@@ -1936,7 +1938,7 @@ __device__ __forceinline__ void tb_finalize_signal(uint32_t tb, uint32_t n_tbs, 
                                                    const uint8_t* __restrict__ cb_bits,
                                                    const int32_t* __restrict__ cb_res, uint8_t* __restrict__ tb_out,
                                                    int32_t* __restrict__ status, const uint32_t* __restrict__ lut24a,
-                                                   const uint32_t* __restrict__ pow2_24a, const tb_signal& sig,
+                                                   const uint32_t* __restrict__ pow2_24a, tb_signal sig,
                                                    uint32_t* red, uint32_t* lut, int& all_ok)
 {
   const lh_tb_desc_dev d = tbs[tb];
@@ -1999,9 +2001,8 @@ const void* decode_kernel(bool learned, bool pairs, bool ext8)
 hipError_t launch_decode(launch_header* d_hdr, const dec_item* d_items, const dec_job* d_jobs, unsigned grid,
                          int nt, size_t lds, int8_t* rm, const int8_t* cw, uint8_t* bits, int32_t* results,
                          const uint32_t* crc_x, const uint32_t* gtab, const uint32_t* scale_tab, int fb_iters,
-                         bool pairs, bool ext8, hipStream_t s, const fused_tb_args* fused)
+                         bool pairs, bool ext8, hipStream_t s, const fused_tb_args* fa)
 {
-  const fused_tb_args fa = fused != nullptr ? *fused : fused_tb_args{};
   const bool learned = scale_tab != nullptr;
   ext8 = ext8 && !pairs;
   const void* fn = decode_kernel(learned, pairs, ext8);
