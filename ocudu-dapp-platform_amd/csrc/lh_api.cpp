@@ -764,7 +764,7 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   // Behind them the row-pair tables of the decoder's two-rows-per-thread mode (even Z):
   // two words per edge, col * Z | S << 16 and the 16-bit pair S | (S - T) << 16, with
   // h = Z / 2, f = (shift >= h), S = 2 (shift mod h) + f and T = Z - 1 + 2f.
-  std::vector<uint32_t> gt(3 * (size_t)kGraphTableLen);
+  std::vector<uint32_t> gt(3 * (size_t)kGraphTableLen + kGraphTablePad);
   for (int pos = 0; pos < kNofLiftingSizes; ++pos) {
     const unsigned Z = kLiftingSizes[pos];
     const int ils = lifting_index(Z);

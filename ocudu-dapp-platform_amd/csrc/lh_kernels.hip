@@ -132,6 +132,25 @@ __device__ __forceinline__ void hold8(T (&w)[8])
   asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]), "+v"(w[4]), "+v"(w[5]), "+v"(w[6]), "+v"(w[7]));
 }
 
+// The N uniform edge words of a row with a runtime degree, loaded together (merged scalar
+// loads, one wait) before any of them is used: loaded inside the per-edge degree test, each
+// word would be loaded and waited for on its own, and a scalar-load wait also waits for the
+// LDS reads in flight.  Rows shorter than their degree class read past their last edge into
+// the table's padding.  Core rows (fixed degree) read the table directly and leave the
+// schedule of their loads to the compiler.
+template <int N>
+__device__ __forceinline__ void load_edge_words(const uint32_t* __restrict__ gt, uint32_t (&w)[N])
+{
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    w[e] = gt[e];
+  }
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    asm volatile("" : "+s"(w[e]));
+  }
+}
+
 // One layer update for check row r (pass 0: gather the deg soft bits; pass 1: v2c and
 // two-min over composite keys; pass 2: c2v, promoted soft-bit update).  DEG is the
 // compile-time degree (core rows) or the maximum degree with a runtime `deg`.
@@ -149,10 +168,19 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
   const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
   uint32_t addr[DEG];
   int sv[DEG];
+  uint32_t gw[RT ? DEG : 1];
+  if constexpr (RT) {
+    load_edge_words(gt, gw);
+  }
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (!RT || e < deg) {
-      const uint32_t g = gt[e];
+      uint32_t g;
+      if constexpr (RT) {
+        g = gw[e];
+      } else {
+        g = gt[e];
+      }
       unsigned j = r + (g >> 16);
       j = min(j, j - Z);
       addr[e] = (g & 0xffffu) + j + gbase; // one three-input add
@@ -246,10 +274,12 @@ __device__ __forceinline__ void layer_update_ext8(int8_t* __restrict__ soft, int
   const uint32_t mbase = lds_addr(msg);
   uint32_t addr[DEG];
   int sv[DEG], cv[DEG];
+  uint32_t gw[DEG];
+  load_edge_words(gt, gw);
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (e < deg) {
-      const uint32_t g = gt[e];
+      const uint32_t g = gw[e];
       unsigned j = r + (g >> 16);
       j = min(j, j - Z);
       addr[e] = (g & 0xffffu) + j + gbase;
@@ -390,6 +420,10 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
   const uint32_t r2x2 = 2u * r * 0x10001u;
   uint32_t a1[DEG];
   i16x2 sv[DEG];
+  uint32_t gw[RT ? 2 * DEG : 1];
+  if constexpr (RT) {
+    load_edge_words(gt, gw);
+  }
   // Four edges' eight reads in flight before one wait.
 #pragma unroll
   for (int e0 = 0; e0 < DEG; e0 += 4) {
@@ -399,7 +433,11 @@ __device__ __forceinline__ void layer_update_pair(int8_t* __restrict__ soft, int
       const int e = e0 + k;
       raw[2 * k] = raw[2 * k + 1] = 0;
       if (e < DEG && (!RT || e < deg)) {
-        a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], r2x2) + gbase;
+        if constexpr (RT) {
+          a1[e] = (uint32_t)pair_addr(gw[2 * e], gw[2 * e + 1], r2x2) + gbase;
+        } else {
+          a1[e] = (uint32_t)pair_addr(gt[2 * e], gt[2 * e + 1], r2x2) + gbase;
+        }
         raw[2 * k] = *lds_at(a1[e]);
         raw[2 * k + 1] = *lds_at(a1[e] ^ 1u);
       }
