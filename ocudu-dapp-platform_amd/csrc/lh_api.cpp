@@ -763,8 +763,9 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
   // (ldpc_graph_impl.cpp:11-39, ldpc_luts_impl.cpp:4505-4523).
   // Behind them the row-pair tables of the decoder's two-rows-per-thread mode (even Z):
   // two words per edge, col * Z | S << 16 and the 16-bit pair S | (S - T) << 16, with
-  // h = Z / 2, f = (shift >= h), S = 2 (shift mod h) + f and T = Z - 1 + 2f.
-  std::vector<uint32_t> gt(3 * (size_t)kGraphTableLen + kGraphTablePad);
+  // h = Z / 2, f = (shift >= h), S = 2 (shift mod h) + f and T = Z - 1 + 2f.  Then the
+  // decoder's one-row tables: two words per edge, the shift and col * Z.
+  std::vector<uint32_t> gt(5 * (size_t)kGraphTableLen + kGraphTablePad);
   for (int pos = 0; pos < kNofLiftingSizes; ++pos) {
     const unsigned Z = kLiftingSizes[pos];
     const int ils = lifting_index(Z);
@@ -772,6 +773,8 @@ int lh_ctx_create(lh_ctx** out, int device, unsigned max_codeblocks)
       const unsigned sh = v % Z;
       const size_t o = graph_offset(bg, Z) + (size_t)k;
       gt[o] = (uint32_t)(col * Z) | ((uint32_t)sh << 16);
+      gt[3 * (size_t)kGraphTableLen + 2 * o] = sh;
+      gt[3 * (size_t)kGraphTableLen + 2 * o + 1] = (uint32_t)(col * Z);
       if (Z % 2 == 0) {
         const unsigned h = Z / 2, f = sh >= h ? 1u : 0u;
         const unsigned S = 2 * (sh - f * h) + f, T = Z - 1 + 2 * f;

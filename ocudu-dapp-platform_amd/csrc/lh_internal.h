@@ -25,8 +25,8 @@ constexpr uint32_t kMaxRmLength = 22 * 384 * 35; // MAX_CODEBLOCK_RM_SIZE
 constexpr int kNofLiftingSizes = 51;
 // Lifted edge tables of all graphs: BG1 graphs first (316 edges each), then BG2 (197).
 constexpr int kGraphTableLen = kNofLiftingSizes * (316 + 197);
-// Words after the tables (plain, then row pairs): an extension layer loads the edge words of
-// its degree class's largest row at once, up to 2 x 9 words past a row's last edge.
+// Words after the tables (plain, row pairs, one row): an extension layer loads the edge words
+// of its degree class's largest row at once, up to 2 x 9 words past a row's last edge.
 constexpr int kGraphTablePad = 32;
 
 // job flags

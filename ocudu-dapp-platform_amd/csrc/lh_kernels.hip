@@ -168,22 +168,24 @@ __device__ __forceinline__ void layer_update(int8_t* __restrict__ soft, int goff
   const uint32_t gbase = lds_addr(soft) + (uint32_t)goff;
   uint32_t addr[DEG];
   int sv[DEG];
-  uint32_t gw[RT ? DEG : 1];
+  uint32_t gw[RT ? 2 * DEG : 1];
   if constexpr (RT) {
     load_edge_words(gt, gw);
   }
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (!RT || e < deg) {
-      uint32_t g;
+      uint32_t sh, cz;
       if constexpr (RT) {
-        g = gw[e];
+        sh = gw[2 * e];
+        cz = gw[2 * e + 1];
       } else {
-        g = gt[e];
+        sh = gt[2 * e];
+        cz = gt[2 * e + 1];
       }
-      unsigned j = r + (g >> 16);
+      unsigned j = r + sh;
       j = min(j, j - Z);
-      addr[e] = (g & 0xffffu) + j + gbase; // one three-input add
+      addr[e] = cz + j + gbase; // one three-input add
       sv[e] = *lds_at(addr[e]);
     }
   }
@@ -274,15 +276,14 @@ __device__ __forceinline__ void layer_update_ext8(int8_t* __restrict__ soft, int
   const uint32_t mbase = lds_addr(msg);
   uint32_t addr[DEG];
   int sv[DEG], cv[DEG];
-  uint32_t gw[DEG];
+  uint32_t gw[2 * DEG];
   load_edge_words(gt, gw);
 #pragma unroll
   for (int e = 0; e < DEG; ++e) {
     if (e < deg) {
-      const uint32_t g = gw[e];
-      unsigned j = r + (g >> 16);
+      unsigned j = r + gw[2 * e];
       j = min(j, j - Z);
-      addr[e] = (g & 0xffffu) + j + gbase;
+      addr[e] = gw[2 * e + 1] + j + gbase;
       sv[e] = *lds_at(addr[e]);
       cv[e] = *lds_at(mbase + (uint32_t)e * Z);
     }
@@ -553,7 +554,7 @@ __device__ __forceinline__ void core_layer(int8_t* soft, int goff, const uint32_
   if constexpr (PAIR) {
     layer_update_pair<DEG, false, true, LEARNED>(soft, goff, gt + 2 * E0, DEG, r, cpk, LEARNED ? abt[ROW] : 0u);
   } else {
-    layer_update<DEG, false, true, LEARNED>(soft, goff, gt + E0, DEG, r, Z, cpk, LEARNED ? abt[ROW] : 0u);
+    layer_update<DEG, false, true, LEARNED>(soft, goff, gt + 2 * E0, DEG, r, Z, cpk, LEARNED ? abt[ROW] : 0u);
   }
 }
 
@@ -607,10 +608,9 @@ __device__ uint32_t syndrome_fail(const int8_t* soft, int goff, const uint32_t* 
     const int e0 = bgt<BG>::rs_word(l), e1 = bgt<BG>::rs_word(l + 1);
     uint32_t p = 0;
     for (int e = e0; e < e1; ++e) {
-      const uint32_t g = gt[e];
-      unsigned j = r + (g >> 16);
+      unsigned j = r + gt[2 * e];
       j = min(j, j - Z);
-      const int s = soft[(int)((g & 0xffffu) + j) + goff];
+      const int s = soft[(int)(gt[2 * e + 1] + j) + goff];
       bad |= (s == 0);
       p ^= (s <= 0);
     }
@@ -1430,8 +1430,9 @@ __device__ __forceinline__ bool decode_item(const dec_item& item, const dec_job*
     const int par = it & 1;
     // The edge table is re-read every iteration: values derived from it stay in the
     // layer that uses them instead of being hoisted out of the loop into registers.
-    // Row pairs read the two-word row-pair edge table behind the plain one.
-    int gto = PAIR ? kGraphTableLen + 2 * (int)jb.gt_off : (int)jb.gt_off;
+    // Two words per edge, ready for the address arithmetic: the row-pair table, or the
+    // one-row table (shift, column * Z) behind it.
+    int gto = (PAIR ? kGraphTableLen : 3 * kGraphTableLen) + 2 * (int)jb.gt_off;
     asm volatile("" : "+v"(gto));
     const uint32_t* __restrict__ gti = gtab + __builtin_amdgcn_readfirstlane(gto);
     // Learned tables: one row of M (A, B) words per iteration, the last row repeated.
@@ -1489,32 +1490,32 @@ __device__ __forceinline__ bool decode_item(const dec_item& item, const dec_job*
         } else if constexpr (EXT8) {
           if (BG == 1) {
             if (dg <= 5) {
-              layer_update_ext8<5, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+              layer_update_ext8<5, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, msg, ab);
             } else if (dg <= 7) {
-              layer_update_ext8<7, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+              layer_update_ext8<7, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, msg, ab);
             } else {
-              layer_update_ext8<10, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+              layer_update_ext8<10, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, msg, ab);
             }
           } else {
             if (dg <= 4) {
-              layer_update_ext8<4, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+              layer_update_ext8<4, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, msg, ab);
             } else {
-              layer_update_ext8<6, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, msg, ab);
+              layer_update_ext8<6, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, msg, ab);
             }
           }
         } else if (BG == 1) {
           if (dg <= 5) {
-            layer_update<5, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<5, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, st, ab);
           } else if (dg <= 7) {
-            layer_update<7, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<7, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, st, ab);
           } else {
-            layer_update<10, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<10, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, st, ab);
           }
         } else {
           if (dg <= 4) {
-            layer_update<4, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<4, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, st, ab);
           } else {
-            layer_update<6, true, false, LEARNED>(soft0, goff, gti + e0, dg, r, (unsigned)Z, st, ab);
+            layer_update<6, true, false, LEARNED>(soft0, goff, gti + 2 * e0, dg, r, (unsigned)Z, st, ab);
           }
         }
       }
