@@ -1644,12 +1644,13 @@ __device__ __forceinline__ void tb_finalize_signal(uint32_t tb, uint32_t n_tbs, 
                                                    const uint32_t* __restrict__ pow2_24a, tb_signal sig,
                                                    uint32_t* red, uint32_t* lut, int& all_ok);
 
-// Fused TB finalize (the serving runtime; fap in device memory): the item's codeblocks are counted off their
-// transport blocks (one thread per group; its device-scope fence first makes the
-// workgroup's bits and results, complete at the item's closing barrier, visible to every
-// workgroup); the transport blocks whose last codeblock this workgroup counted are then
-// finalized here, after a fence that makes the other workgroups' codeblocks visible.
-// Scratch: the start of the dynamic LDS, free between items.
+// Fused TB finalize (the serving runtime; its arguments at fap, in device memory): the
+// item's codeblocks are counted off their transport blocks (one thread per group; its
+// device-scope fence first makes the workgroup's bits and results, complete at the item's
+// closing barrier, visible to every workgroup); the transport blocks whose last codeblock
+// this workgroup counted are then finalized here, after a fence that makes the other
+// workgroups' codeblocks visible.  Scratch: the start of the dynamic LDS, free between
+// items.
 __device__ __forceinline__ void fused_tb_complete(const dec_item& item, const dec_job* __restrict__ jobs,
                                                   const fused_tb_args* __restrict__ fap, const uint8_t* __restrict__ bits,
                                                   const int32_t* __restrict__ results, uint8_t* lds)
