@@ -367,8 +367,8 @@ _ref_demod = None
 
 
 def ref_demod():
-    """The reference's scalar demappers and pseudo-random generator built from source
-    (_ref/libref_demod.so), or None when not built."""
+    """The reference's scalar demappers, LUT modulation mapper and pseudo-random generator
+    built from source (_ref/libref_demod.so), or None when not built."""
     global _ref_demod
     if _ref_demod is None:
         path = HERE / "_ref" / "libref_demod.so"
@@ -377,6 +377,7 @@ def ref_demod():
         R = C.CDLL(str(path))
         _fp = C.POINTER(C.c_float)
         R.ref_demodulate_soft.argtypes = [_i8p, _fp, _fp, C.c_uint, C.c_int]
+        R.ref_modulate.argtypes = [_fp, _u8p, C.c_uint, C.c_int]
         R.ref_scrambling_sequence.argtypes = [_u8p, C.c_uint32, C.c_uint, C.c_uint]
         R.ref_ulsch_information.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_float), C.POINTER(C.c_uint)]
         R.ref_ulsch_demultiplex.argtypes = [C.POINTER(C.c_uint), _i8p, C.c_uint, C.c_uint32, C.c_uint, _i8p, _i8p,

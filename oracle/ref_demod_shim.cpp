@@ -1,9 +1,10 @@
-// extern "C" access to the reference's own soft demappers, pseudo-random generator and
-// UL-SCH demultiplexer, built from their sources (oracle/Makefile.ref) -- TEST
-// INFRASTRUCTURE ONLY: the checker of the oracle's demodulation, scrambling and
-// demultiplexing restatements.  Built without AVX/NEON flags, so
+// extern "C" access to the reference's own soft demappers, modulation mapper,
+// pseudo-random generator and UL-SCH demultiplexer, built from their sources
+// (oracle/Makefile.ref) -- TEST INFRASTRUCTURE ONLY: the checker of the oracle's
+// modulation, demodulation, scrambling and demultiplexing restatements.  Built without AVX/NEON flags, so
 // the reference's scalar (generic) demapper path runs.
 #include "demodulation_mapper_impl.h"
+#include "modulation_mapper_lut_impl.h"
 #include "pseudo_random_generator_impl.h"
 #include "ulsch_demultiplex_impl.h"
 #include "ocudu/phy/upper/channel_processors/pusch/pusch_decoder_buffer.h"
@@ -25,6 +26,20 @@ void ref_demodulate_soft(int8_t* llrs, const float* iq, const float* nv, unsigne
   span<const cf_t> sym(reinterpret_cast<const cf_t*>(iq), nsym);
   span<const float> n(nv, nsym);
   demapper.demodulate_soft(out, sym, n, static_cast<modulation_scheme>(modulation));
+}
+
+// modulation_mapper::modulate (the reference's LUT mapper) of nbits bits, one per byte,
+// into nbits / Qm symbols (re, im interleaved).
+void ref_modulate(float* iq, const uint8_t* bits, unsigned nbits, int modulation)
+{
+  static modulation_mapper_lut_impl mapper;
+  const unsigned qm = modulation == 0 ? 1u : (unsigned)modulation;
+  dynamic_bit_buffer in(nbits);
+  for (unsigned i = 0; i < nbits; ++i) {
+    in.insert(bits[i] & 1u, i, 1);
+  }
+  span<cf_t> sym(reinterpret_cast<cf_t*>(iq), nbits / qm);
+  mapper.modulate(sym, in, static_cast<modulation_scheme>(modulation));
 }
 
 // pseudo_random_generator: init(c_init), advance(offset), generate(n bits); one bit per byte.

@@ -53,3 +53,75 @@ def test_generated_noise_statistics():
     from math import erfc, sqrt
     p = 0.5 * erfc(sqrt(10.0) / sqrt(2.0))
     assert abs(errs / tot - p) < 4 * sqrt(p / tot) + 1e-5, (errs / tot, p)
+
+
+def _modulate(fn, bits, Qm):
+    import ctypes as C
+    iq = np.zeros(2 * (bits.size // Qm), np.float32)
+    b = np.ascontiguousarray(bits, np.uint8)
+    fn(iq.ctypes.data_as(C.POINTER(C.c_float)), O._p(b, O._u8p), b.size, Qm)
+    return iq
+
+
+def _orc_modulate_fn():
+    import ctypes as C
+    L = O.lib()
+    L.orc_modulate.argtypes = [C.POINTER(C.c_float), O._u8p, C.c_uint, C.c_uint]
+    return L.orc_modulate
+
+
+def _all_patterns(Qm):
+    """Every Qm-bit label, bit k of pattern p = (p >> k) & 1, as a (2^Qm, Qm) array."""
+    return ((np.arange(1 << Qm)[:, None] >> np.arange(Qm)) & 1).astype(np.uint8)
+
+
+@pytest.mark.parametrize("Qm", [2, 4, 6, 8])
+def test_mapper_matches_reference(Qm):
+    """The generator's mapper (orc_modulate, TS38.211 5.1.3-5.1.5) equals the reference's LUT
+    modulation mapper (modulation_mapper_lut_impl.cpp, built from source) on every label and
+    on a random stream."""
+    R = O.ref_demod()
+    if R is None:
+        pytest.skip("oracle/_ref not built")
+    rnd = np.random.default_rng(Qm).integers(0, 2, Qm * 4096).astype(np.uint8)
+    for bits in (_all_patterns(Qm).reshape(-1), rnd):
+        a = _modulate(_orc_modulate_fn(), bits, Qm)
+        b = _modulate(R.ref_modulate, bits, Qm)
+        assert np.abs(a - b).max() <= 2e-7, np.abs(a - b).max()
+
+
+@pytest.mark.parametrize("Qm,esn0_db", [(2, 4.0), (4, 10.0), (6, 16.0), (8, 22.0), (8, 30.0)])
+def test_demapper_is_max_log_over_reference_constellation(Qm, esn0_db):
+    """The generator's demapper (orc_demap) is the max-log LLR -- (min over the points whose
+    bit is 1 of |y - s|^2 minus the same over the points whose bit is 0) / n0 -- over the
+    reference mapper's own constellation, quantised by the reference's quantiser with range 24
+    (QPSK) or 20 (QAM): brute force over all 2^Qm complex points in float64, against the
+    oracle's per-component form in float32 output.  Ties at a quantisation step may round
+    either way (|difference| 1, rare)."""
+    import ctypes as C
+    R, RL = O.ref_demod(), O.ref()
+    if R is None or RL is None:
+        pytest.skip("oracle/_ref not built")
+    labels = _all_patterns(Qm)
+    pts = _modulate(R.ref_modulate, labels.reshape(-1), Qm).astype(np.float64).view(np.complex128)
+    rng = np.random.default_rng(int(10 * esn0_db) + Qm)
+    n0 = float(np.float32(10 ** (-esn0_db / 10)))
+    nsym = 3000
+    tx = rng.integers(0, 1 << Qm, nsym)
+    noise = rng.normal(0, np.sqrt(n0 / 2), (nsym, 2)) @ np.array([1, 1j])
+    y = (pts[tx] + noise).astype(np.complex64)
+    d = np.abs(y.astype(np.complex128)[:, None] - pts[None, :]) ** 2
+    rng_lim = 24.0 if Qm == 2 else 20.0
+    expect = np.zeros((nsym, Qm), np.int8)
+    for k in range(Qm):
+        l = ((d[:, labels[:, k] == 1].min(1) - d[:, labels[:, k] == 0].min(1)) / n0).astype(np.float32)
+        expect[:, k] = [RL.ref_llr_quantize(float(v), rng_lim) for v in l]
+    L = O.lib()
+    L.orc_demap.argtypes = [O._i8p, C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_float]
+    got = np.zeros(nsym * Qm, np.int8)
+    yf = np.ascontiguousarray(y.view(np.float32))
+    L.orc_demap(O._p(got, O._i8p), yf.ctypes.data_as(C.POINTER(C.c_float)), nsym, Qm, n0)
+    diff = np.abs(got.reshape(nsym, Qm).astype(int) - expect.astype(int))
+    assert diff.max() <= 1 and (diff != 0).mean() < 1e-3, (diff.max(), (diff != 0).mean())
+    # Not a degenerate check: the LLRs cover the quantiser's range.
+    assert np.abs(expect).max() >= 60 and len(np.unique(expect)) > 30
